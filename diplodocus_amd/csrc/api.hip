@@ -1,0 +1,623 @@
+// C ABI of libdpl (include/dpl/dpl.h): contexts, scratch pools, build orchestration,
+// trace, downloads. No exception crosses the boundary: every entry point catches and turns
+// failures into dpl_status plus a message kept on the context.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dpl_internal.h"
+
+namespace dpl {
+
+hipError_t LaunchSah(const FNode* nodes, const int32_t* counters, double* out, hipStream_t s);
+hipError_t LaunchRender(bool adjoint, const FNode* nodes, const float4* tris, int tree_height, int64_t prim_count,
+                        const dpl_render_params& rp, const float* cam_basis, const int32_t* mat_id,
+                        const float* albedo, const float* emission, const float* d_image, float* image,
+                        float* grad_albedo, float* grad_emission, int cu_count, uint32_t* counter, hipStream_t s);
+
+struct Error : std::runtime_error {
+    dpl_status status;
+    Error(dpl_status s, const std::string& m) : std::runtime_error(m), status(s) {}
+};
+
+#define DPL_HIP(expr)                                                                                       \
+    do {                                                                                                    \
+        hipError_t e_ = (expr);                                                                             \
+        if (e_ != hipSuccess)                                                                               \
+            throw ::dpl::Error(e_ == hipErrorOutOfMemory ? DPL_ERR_OOM : DPL_ERR_DEVICE,                    \
+                               std::string(#expr) + ": " + hipGetErrorString(e_));                          \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+    template <typename T = void>
+    T* ensure(size_t bytes) {
+        bytes = std::max<size_t>(bytes, 256);
+        if (bytes > cap) {
+            release();
+            hipError_t e = hipMalloc(&p, bytes);
+            if (e != hipSuccess) {
+                p = nullptr;
+                (void)hipGetLastError();
+                throw Error(DPL_ERR_OOM, "hipMalloc(" + std::to_string(bytes) + " bytes) failed");
+            }
+            cap = bytes;
+        }
+        return static_cast<T*>(p);
+    }
+    template <typename T>
+    T* as() const {
+        return static_cast<T*>(p);
+    }
+};
+
+}  // namespace dpl
+
+using namespace dpl;
+
+struct dpl_context {
+    int device = 0;
+    int cu_count = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    std::string last_error;
+    // scratch
+    DevBuf tri, keys, vals, keys_alt, vals_alt, nodes, parent, clusters0, clusters1, ccount, status0, status1,
+        small, count, cost, collapsed, minleaf, height, visit, leafslot, keptflag, scan_tmp, sort_tmp, rays, hits,
+        trace_counter, render_tmp;
+    void* pinned = nullptr;  // host staging for small results
+    hipEvent_t ev[10] = {};
+};
+
+struct dpl_bvh {
+    dpl_context* ctx = nullptr;
+    int64_t prim_count = 0;
+    int32_t node_count = 0;
+    int32_t tree_height = 0;
+    int32_t mesh_count = 1;
+    dpl_build_params params{};
+    dpl_build_stats stats{};
+    DevBuf fnodes, tri_ordered, prim_order;
+    // intermediates (keep_intermediates)
+    DevBuf codes, sorted_idx, pnodes;
+    std::vector<int32_t> iter_counts;
+    bool has_intermediates = false;
+    bool codes64 = false;
+};
+
+namespace {
+
+dpl_status Fail(dpl_context* ctx, dpl_status s, const std::string& m) {
+    if (ctx) ctx->last_error = m;
+    return s;
+}
+
+template <typename F>
+dpl_status Guard(dpl_context* ctx, F&& f) {
+    try {
+        if (ctx) {
+            DPL_HIP(hipSetDevice(ctx->device));
+        }
+        f();
+        if (ctx) ctx->last_error.clear();
+        return DPL_OK;
+    } catch (const Error& e) {
+        return Fail(ctx, e.status, e.what());
+    } catch (const std::bad_alloc&) {
+        return Fail(ctx, DPL_ERR_OOM, "host allocation failed");
+    } catch (const std::exception& e) {
+        return Fail(ctx, DPL_ERR_INTERNAL, e.what());
+    } catch (...) {
+        return Fail(ctx, DPL_ERR_INTERNAL, "unknown error");
+    }
+}
+
+void Require(bool cond, const std::string& msg) {
+    if (!cond) throw Error(DPL_ERR_INVALID_ARG, msg);
+}
+
+dpl_build_params Normalize(const dpl_build_params* p) {
+    dpl_build_params q = *p;
+    if (q.block_size == 0) q.block_size = 256;
+    if (q.sort_digit_bits == 0) q.sort_digit_bits = 8;
+    if (q.kdop_size == 0) q.kdop_size = 32;
+    Require(q.morton_type == DPL_MORTON30 || q.morton_type == DPL_EMC60, "morton_type must be 0 or 1");
+    Require(q.bv_type == DPL_BV_AABB || q.bv_type == DPL_BV_SOBB, "bv_type must be 0 or 1");
+    Require(q.nn_search_radius >= 1 && q.nn_search_radius <= kMaxRadius, "nn_search_radius must be in [1, 32]");
+    Require(q.max_triangles_per_leaf >= 1 && q.max_triangles_per_leaf <= kMaxLeaf,
+            "max_triangles_per_leaf must be in [1, 15]");
+    Require(q.block_size == 128 || q.block_size == 256 || q.block_size == 512, "block_size must be 128, 256 or 512");
+    Require(q.sort_digit_bits == 4 || q.sort_digit_bits == 8, "sort_digit_bits must be 4 or 8");
+    if (q.bv_type == DPL_BV_SOBB) throw Error(DPL_ERR_UNSUPPORTED, "SOBB node bounds are not available in this build");
+    return q;
+}
+
+float ElapsedMs(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0.0f;
+    }
+    return ms;
+}
+
+struct PinnedResults {
+    int32_t counters[8];
+    int32_t iter_done;
+    int32_t root_height;
+    uint32_t bounds[13];
+    int32_t ccount_probe;
+    double sah;
+    PNode root;
+};
+
+void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, dpl_mem mem,
+               const dpl_build_params& p, dpl_build_stats* out_stats) {
+    Require(tri_in != nullptr, "tri_xyz is null");
+    Require(n >= 1, "at least one triangle is required");
+    Require(n < (int64_t(1) << 27), "at most 2^27 - 1 triangles per BVH");
+    Require(mem == DPL_MEM_HOST || mem == DPL_MEM_DEVICE, "mem must be DPL_MEM_HOST or DPL_MEM_DEVICE");
+    hipStream_t s = ctx->stream;
+    dpl_context& C = *ctx;
+    const int64_t nn = 2 * n - 1;
+    const int chunk = PlocChunk(p.block_size);
+    const int64_t tiles0 = (n + chunk - 1) / chunk;
+    const bool key64 = p.morton_type == DPL_EMC60;
+    const int key_bits = key64 ? 60 : 30;
+
+    // ---- allocate everything before the first timed event ----
+    const float* tri = tri_in;
+    if (mem == DPL_MEM_HOST) tri = C.tri.ensure<float>(size_t(n) * 36);
+    BuildBuffers b{};
+    b.n = n;
+    b.tri = tri;
+    b.keys = C.keys.ensure<uint64_t>(size_t(n) * 8);
+    b.vals = C.vals.ensure<uint32_t>(size_t(n) * 4);
+    b.keys_alt = C.keys_alt.ensure<uint64_t>(size_t(n) * 8);
+    b.vals_alt = C.vals_alt.ensure<uint32_t>(size_t(n) * 4);
+    b.nodes = C.nodes.ensure<PNode>(size_t(nn) * sizeof(PNode));
+    b.parent = C.parent.ensure<int32_t>(size_t(nn) * 4);
+    b.clusters[0] = C.clusters0.ensure<PNode>(size_t(n) * sizeof(PNode));
+    b.clusters[1] = C.clusters1.ensure<PNode>(size_t(n) * sizeof(PNode));
+    b.ccount = C.ccount.ensure<int32_t>(size_t(n + 2) * 4);
+    b.status[0] = C.status0.ensure<uint64_t>(size_t(tiles0 + 1) * 8);
+    b.status[1] = C.status1.ensure<uint64_t>(size_t(tiles0 + 1) * 8);
+    {
+        uint8_t* sm = C.small.ensure<uint8_t>(1024);
+        b.bounds = reinterpret_cast<uint32_t*>(sm);              // 13 words
+        b.tile_counter = reinterpret_cast<uint32_t*>(sm + 64);   // 2
+        b.iter_done = reinterpret_cast<int32_t*>(sm + 96);       // 1
+        b.counters = reinterpret_cast<int32_t*>(sm + 128);       // 8
+        b.sah = reinterpret_cast<double*>(sm + 192);             // 1
+    }
+    b.count = C.count.ensure<int32_t>(size_t(nn) * 4);
+    b.cost = C.cost.ensure<float>(size_t(nn) * 4);
+    b.collapsed = C.collapsed.ensure<uint8_t>(size_t(nn));
+    b.minleaf = C.minleaf.ensure<int32_t>(size_t(nn) * 4);
+    b.height = C.height.ensure<int32_t>(size_t(nn) * 4);
+    b.visit = C.visit.ensure<uint32_t>(size_t(n) * 4);
+    b.leafslot = C.leafslot.ensure<int32_t>(size_t(n) * 4);
+    b.keptflag = C.keptflag.ensure<int32_t>(size_t(n) * 4);
+    b.scan_tmp = C.scan_tmp.ensure<int32_t>(ScanTempInts(n) * 4);
+    const size_t sort_bytes = RadixSortTempBytes(n, key_bits, p.sort_digit_bits, key64);
+    void* sort_tmp = C.sort_tmp.ensure(sort_bytes);
+    b.fnodes = bvh->fnodes.ensure<FNode>(size_t(std::max<int64_t>(1, n - 1)) * sizeof(FNode));
+    b.tri_ordered = bvh->tri_ordered.ensure<float4>(size_t(n) * 48);
+    b.prim_order = bvh->prim_order.ensure<int32_t>(size_t(n) * 4);
+    if (p.keep_intermediates) {
+        bvh->codes.ensure(size_t(n) * 8);
+        bvh->sorted_idx.ensure(size_t(n) * 4);
+        bvh->pnodes.ensure(size_t(nn) * sizeof(PNode));
+    }
+
+    dpl_build_stats st{};
+    if (mem == DPL_MEM_HOST) {
+        DPL_HIP(hipEventRecord(C.ev[8], s));
+        DPL_HIP(hipMemcpyAsync(const_cast<float*>(tri), tri_in, size_t(n) * 36, hipMemcpyHostToDevice, s));
+        DPL_HIP(hipEventRecord(C.ev[9], s));
+    }
+
+    DPL_HIP(hipEventRecord(C.ev[0], s));
+    DPL_HIP(LaunchBuildInit(b, chunk, s));
+    DPL_HIP(LaunchBounds(b, s));
+    DPL_HIP(hipEventRecord(C.ev[1], s));
+    DPL_HIP(LaunchCodes(b, p.morton_type, s));
+    if (p.keep_intermediates) {
+        DPL_HIP(hipMemcpyAsync(bvh->codes.p, b.keys, size_t(n) * (key64 ? 8 : 4), hipMemcpyDeviceToDevice, s));
+    }
+    DPL_HIP(hipEventRecord(C.ev[2], s));
+    bool in_alt = false;
+    DPL_HIP(LaunchRadixSort(b.keys, b.vals, b.keys_alt, b.vals_alt, n, key_bits, p.sort_digit_bits, key64, sort_tmp,
+                            sort_bytes, &in_alt, s));
+    if (in_alt) {
+        std::swap(b.keys, b.keys_alt);
+        std::swap(b.vals, b.vals_alt);
+    }
+    DPL_HIP(hipEventRecord(C.ev[3], s));
+    DPL_HIP(LaunchInitClusters(b, s));
+
+    // ---- PLOC iterations: multi-workgroup kernels while clusters exceed the tail size ----
+    PinnedResults* pr = static_cast<PinnedResults*>(C.pinned);
+    const int tail = PlocTail();
+    int iter = 0;
+    int64_t c_known = n;
+    const int cap = std::max(1, C.cu_count * (p.block_size == 512 ? 2 : (p.block_size == 128 ? 8 : 4)));
+    while (c_known > tail) {
+        int est = int(std::ceil(std::log(double(c_known) / tail) / std::log(1.0 / 0.8)));
+        est = std::max(1, std::min(est, 64));
+        for (int g = 0; g < est; g++) {
+            const int64_t T = (c_known + chunk - 1) / chunk;
+            DPL_HIP(LaunchPlocIteration(b, iter, p.nn_search_radius, p.block_size, tail,
+                                        int(std::min<int64_t>(T, cap)), s));
+            iter++;
+        }
+        DPL_HIP(hipMemcpyAsync(&pr->ccount_probe, b.ccount + iter, 4, hipMemcpyDeviceToHost, s));
+        DPL_HIP(hipMemcpyAsync(&pr->bounds[12], b.bounds + 12, 4, hipMemcpyDeviceToHost, s));
+        DPL_HIP(hipStreamSynchronize(s));
+        if (pr->bounds[12] & 1u) throw Error(DPL_ERR_INVALID_ARG, "triangle coordinates must be finite");
+        if (pr->ccount_probe >= c_known) throw Error(DPL_ERR_INTERNAL, "PLOC made no progress");
+        c_known = pr->ccount_probe;
+    }
+    DPL_HIP(LaunchPlocTail(b, iter, p.nn_search_radius, tail, s));
+    DPL_HIP(hipEventRecord(C.ev[4], s));
+    DPL_HIP(LaunchCollapse(b, p.max_triangles_per_leaf, s));
+    DPL_HIP(hipEventRecord(C.ev[5], s));
+    DPL_HIP(LaunchLayout(b, p.max_triangles_per_leaf, s));
+    DPL_HIP(hipEventRecord(C.ev[6], s));
+    DPL_HIP(LaunchSah(b.fnodes, b.counters, b.sah, s));
+    DPL_HIP(hipEventRecord(C.ev[7], s));
+    if (p.keep_intermediates) {
+        DPL_HIP(hipMemcpyAsync(bvh->sorted_idx.p, b.vals, size_t(n) * 4, hipMemcpyDeviceToDevice, s));
+        DPL_HIP(hipMemcpyAsync(bvh->pnodes.p, b.nodes, size_t(nn) * sizeof(PNode), hipMemcpyDeviceToDevice, s));
+    }
+    DPL_HIP(hipMemcpyAsync(pr->counters, b.counters, sizeof(pr->counters), hipMemcpyDeviceToHost, s));
+    DPL_HIP(hipMemcpyAsync(&pr->iter_done, b.iter_done, 4, hipMemcpyDeviceToHost, s));
+    DPL_HIP(hipMemcpyAsync(&pr->root_height, b.height + (nn - 1), 4, hipMemcpyDeviceToHost, s));
+    DPL_HIP(hipMemcpyAsync(pr->bounds, b.bounds, sizeof(pr->bounds), hipMemcpyDeviceToHost, s));
+    DPL_HIP(hipMemcpyAsync(&pr->sah, b.sah, 8, hipMemcpyDeviceToHost, s));
+    DPL_HIP(hipMemcpyAsync(&pr->root, b.nodes + (nn - 1), sizeof(PNode), hipMemcpyDeviceToHost, s));
+    DPL_HIP(hipStreamSynchronize(s));
+    if (pr->bounds[12] & 1u) throw Error(DPL_ERR_INVALID_ARG, "triangle coordinates must be finite");
+    if (pr->bounds[12] & 2u) throw Error(DPL_ERR_INTERNAL, "PLOC iteration without a merge");
+
+    const int32_t K = pr->counters[0];
+    const int32_t leaves = pr->counters[1];
+    const int total_iters = pr->iter_done;
+    std::vector<int32_t> cc(size_t(total_iters) + 1);
+    DPL_HIP(hipMemcpy(cc.data(), b.ccount, cc.size() * 4, hipMemcpyDeviceToHost));
+    std::vector<int32_t> iters;
+    for (int k = 0; k < total_iters; k++)
+        if (iters.empty() || cc[k] != iters.back()) iters.push_back(cc[k]);
+    if (!iters.empty() && iters.back() == 1) iters.pop_back();
+
+    st.build_ms = ElapsedMs(C.ev[0], C.ev[7]);
+    st.upload_ms = mem == DPL_MEM_HOST ? ElapsedMs(C.ev[8], C.ev[9]) : 0.0;
+    st.stage_ms[0] = ElapsedMs(C.ev[0], C.ev[1]);
+    st.stage_ms[1] = ElapsedMs(C.ev[1], C.ev[2]);
+    st.stage_ms[2] = ElapsedMs(C.ev[2], C.ev[3]);
+    st.stage_ms[3] = ElapsedMs(C.ev[3], C.ev[4]);
+    st.stage_ms[4] = ElapsedMs(C.ev[4], C.ev[5]);
+    st.stage_ms[5] = ElapsedMs(C.ev[5], C.ev[6]);
+    st.stage_ms[6] = 0.0;
+    st.stage_ms[7] = ElapsedMs(C.ev[6], C.ev[7]);
+    const int32_t fcount = K > 0 ? K : 1;
+    st.inner_node_count = fcount;
+    st.leaf_node_count = leaves;
+    st.node_count = fcount + leaves;
+    st.ploc_iterations = int32_t(iters.size());
+    st.memory_consumption = int64_t(fcount) * 64 + n * 48 + n * 4;
+    const float* lo = pr->root.lo;
+    const float* hi = pr->root.hi;
+    const double sx = double(hi[0]) - lo[0], sy = double(hi[1]) - lo[1], sz = double(hi[2]) - lo[2];
+    const double root_sa = 2.0 * (sx * sy + sy * sz + sx * sz);
+    if (K > 0) {
+        st.sah_cost = root_sa > 0 ? (root_sa + pr->sah) / root_sa : 0.0;
+    } else {
+        st.sah_cost = double(n);  // a single leaf: Ci * SA * n / SA
+    }
+    st.tree_height = pr->root_height;
+
+    bvh->prim_count = n;
+    bvh->node_count = fcount;
+    bvh->tree_height = pr->root_height;
+    bvh->params = p;
+    bvh->stats = st;
+    bvh->iter_counts = iters;
+    bvh->has_intermediates = p.keep_intermediates != 0;
+    bvh->codes64 = key64;
+    if (out_stats) *out_stats = st;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------------
+extern "C" {
+
+dpl_status dpl_build_params_from_type(int32_t accel_type, int32_t radius, int32_t max_leaf, int32_t kdop,
+                                      dpl_build_params* p) {
+    if (!p) return DPL_ERR_INVALID_ARG;
+    dpl_build_params q{};
+    switch (accel_type) {
+        case DPL_ACCEL_PLOC: q.morton_type = DPL_MORTON30, q.bv_type = DPL_BV_AABB; break;
+        case DPL_ACCEL_PLOC_EMC: q.morton_type = DPL_EMC60, q.bv_type = DPL_BV_AABB; break;
+        case DPL_ACCEL_PLOC_SOBB: q.morton_type = DPL_MORTON30, q.bv_type = DPL_BV_SOBB; break;
+        case DPL_ACCEL_PLOC_EMC_SOBB: q.morton_type = DPL_EMC60, q.bv_type = DPL_BV_SOBB; break;
+        case DPL_ACCEL_DUMMY:
+        case DPL_ACCEL_SBVH: return DPL_ERR_UNSUPPORTED;
+        default: return DPL_ERR_INVALID_ARG;
+    }
+    q.nn_search_radius = radius;
+    q.max_triangles_per_leaf = max_leaf;
+    q.kdop_size = kdop;
+    *p = q;
+    return DPL_OK;
+}
+
+dpl_status dpl_context_create(int32_t device, dpl_context** out) {
+    if (!out) return DPL_ERR_INVALID_ARG;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        (void)hipGetLastError();
+        return DPL_ERR_DEVICE;
+    }
+    if (device < 0 || device >= count) return DPL_ERR_INVALID_ARG;
+    std::unique_ptr<dpl_context> ctx(new (std::nothrow) dpl_context());
+    if (!ctx) return DPL_ERR_OOM;
+    ctx->device = device;
+    dpl_status st = Guard(ctx.get(), [&] {
+        hipDeviceProp_t prop;
+        DPL_HIP(hipGetDeviceProperties(&prop, device));
+        ctx->cu_count = prop.multiProcessorCount;
+        DPL_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
+        ctx->stream = ctx->own_stream;
+        for (auto& e : ctx->ev) DPL_HIP(hipEventCreate(&e));
+        DPL_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+        ctx->trace_counter.ensure(256);
+    });
+    if (st != DPL_OK) return st;
+    *out = ctx.release();
+    return DPL_OK;
+}
+
+void dpl_context_destroy(dpl_context* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
+    delete ctx;
+}
+
+const char* dpl_last_error(const dpl_context* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+dpl_status dpl_context_set_stream(dpl_context* ctx, void* stream) {
+    if (!ctx) return DPL_ERR_INVALID_ARG;
+    ctx->stream = static_cast<hipStream_t>(stream);
+    return DPL_OK;
+}
+
+dpl_status dpl_context_synchronize(dpl_context* ctx) {
+    if (!ctx) return DPL_ERR_INVALID_ARG;
+    return Guard(ctx, [&] { DPL_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int32_t dpl_context_cu_count(const dpl_context* ctx) { return ctx ? ctx->cu_count : 0; }
+
+dpl_status dpl_bvh_build(dpl_context* ctx, const float* tri, int64_t n, dpl_mem mem, const dpl_build_params* params,
+                         dpl_bvh** out, dpl_build_stats* stats) {
+    if (!ctx || !out || !params) return ctx ? Fail(ctx, DPL_ERR_INVALID_ARG, "null argument") : DPL_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::unique_ptr<dpl_bvh> bvh(new (std::nothrow) dpl_bvh());
+    if (!bvh) return Fail(ctx, DPL_ERR_OOM, "host allocation failed");
+    bvh->ctx = ctx;
+    dpl_status st = Guard(ctx, [&] {
+        dpl_build_params p = Normalize(params);
+        BuildImpl(ctx, bvh.get(), tri, n, mem, p, stats);
+    });
+    if (st != DPL_OK) return st;
+    *out = bvh.release();
+    return DPL_OK;
+}
+
+dpl_status dpl_bvh_rebuild(dpl_bvh* bvh, const float* tri, int64_t n, dpl_mem mem, const dpl_build_params* params,
+                           dpl_build_stats* stats) {
+    if (!bvh || !params) return DPL_ERR_INVALID_ARG;
+    return Guard(bvh->ctx, [&] {
+        dpl_build_params p = Normalize(params);
+        BuildImpl(bvh->ctx, bvh, tri, n, mem, p, stats);
+    });
+}
+
+dpl_status dpl_bvh_build_batched(dpl_context* ctx, const float*, const int64_t*, int32_t, dpl_mem,
+                                 const dpl_build_params*, dpl_bvh** out, dpl_build_stats*) {
+    if (out) *out = nullptr;
+    return Fail(ctx, DPL_ERR_UNSUPPORTED, "batched builds are not available in this build");
+}
+
+void dpl_bvh_destroy(dpl_bvh* bvh) {
+    if (!bvh) return;
+    if (bvh->ctx) {
+        (void)hipSetDevice(bvh->ctx->device);
+        (void)hipStreamSynchronize(bvh->ctx->stream);
+    }
+    delete bvh;
+}
+
+dpl_status dpl_bvh_get_stats(const dpl_bvh* bvh, dpl_build_stats* stats) {
+    if (!bvh || !stats) return DPL_ERR_INVALID_ARG;
+    *stats = bvh->stats;
+    return DPL_OK;
+}
+
+dpl_status dpl_bvh_get_layout(const dpl_bvh* bvh, dpl_bvh_layout* l) {
+    if (!bvh || !l) return DPL_ERR_INVALID_ARG;
+    l->prim_count = bvh->prim_count;
+    l->node_count = bvh->node_count;
+    l->ploc_node_count = bvh->has_intermediates ? 2 * bvh->prim_count - 1 : 0;
+    l->iteration_count = bvh->has_intermediates ? int64_t(bvh->iter_counts.size()) : 0;
+    l->mesh_count = bvh->mesh_count;
+    l->node_bytes = 64;
+    return DPL_OK;
+}
+
+dpl_status dpl_bvh_download(const dpl_bvh* bvh, void* nodes, void* tri_ordered, int32_t* prim_order, uint64_t* codes,
+                            uint32_t* sorted_idx, void* ploc_nodes, int32_t* iter_counts) {
+    if (!bvh) return DPL_ERR_INVALID_ARG;
+    dpl_context* ctx = bvh->ctx;
+    return Guard(ctx, [&] {
+        hipStream_t s = ctx->stream;
+        DPL_HIP(hipStreamSynchronize(s));
+        const int64_t n = bvh->prim_count;
+        if (nodes) DPL_HIP(hipMemcpy(nodes, bvh->fnodes.p, size_t(bvh->node_count) * 64, hipMemcpyDeviceToHost));
+        if (tri_ordered) DPL_HIP(hipMemcpy(tri_ordered, bvh->tri_ordered.p, size_t(n) * 48, hipMemcpyDeviceToHost));
+        if (prim_order) DPL_HIP(hipMemcpy(prim_order, bvh->prim_order.p, size_t(n) * 4, hipMemcpyDeviceToHost));
+        const bool want = codes || sorted_idx || ploc_nodes || iter_counts;
+        if (want && !bvh->has_intermediates)
+            throw Error(DPL_ERR_INVALID_ARG, "intermediates were not kept (build with keep_intermediates=1)");
+        if (codes) {
+            if (bvh->codes64) {
+                DPL_HIP(hipMemcpy(codes, bvh->codes.p, size_t(n) * 8, hipMemcpyDeviceToHost));
+            } else {
+                std::vector<uint32_t> c32(n);
+                DPL_HIP(hipMemcpy(c32.data(), bvh->codes.p, size_t(n) * 4, hipMemcpyDeviceToHost));
+                for (int64_t i = 0; i < n; i++) codes[i] = c32[i];
+            }
+        }
+        if (sorted_idx) DPL_HIP(hipMemcpy(sorted_idx, bvh->sorted_idx.p, size_t(n) * 4, hipMemcpyDeviceToHost));
+        if (ploc_nodes)
+            DPL_HIP(hipMemcpy(ploc_nodes, bvh->pnodes.p, size_t(2 * n - 1) * sizeof(PNode), hipMemcpyDeviceToHost));
+        if (iter_counts) std::memcpy(iter_counts, bvh->iter_counts.data(), bvh->iter_counts.size() * 4);
+    });
+}
+
+dpl_status dpl_trace(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
+                     dpl_hit* hits, dpl_mem mem) {
+    if (!ctx || !bvh) return DPL_ERR_INVALID_ARG;
+    return Guard(ctx, [&] {
+        Require(n >= 0, "n must be non-negative");
+        Require(n == 0 || (rays && hits), "rays and hits must be non-null");
+        Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK)) == 0u, "unknown trace flags");
+        Require(mem == DPL_MEM_HOST || mem == DPL_MEM_DEVICE, "mem must be DPL_MEM_HOST or DPL_MEM_DEVICE");
+        Require(bvh->ctx == ctx, "bvh belongs to another context");
+        if (n == 0) return;
+        hipStream_t s = ctx->stream;
+        const dpl_ray* dr = rays;
+        dpl_hit* dh = hits;
+        if (mem == DPL_MEM_HOST) {
+            dr = ctx->rays.ensure<dpl_ray>(size_t(n) * sizeof(dpl_ray));
+            dh = ctx->hits.ensure<dpl_hit>(size_t(n) * sizeof(dpl_hit));
+            DPL_HIP(hipMemcpyAsync(const_cast<dpl_ray*>(dr), rays, size_t(n) * sizeof(dpl_ray), hipMemcpyHostToDevice, s));
+        }
+        TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
+                       int32_t(bvh->prim_count), bvh->tree_height};
+        hipError_t e = LaunchTrace(tree, dr, n, flags, dh, ctx->cu_count, ctx->trace_counter.as<uint32_t>(), s);
+        if (e == hipErrorNotSupported) throw Error(DPL_ERR_UNSUPPORTED, "tree height exceeds the traversal stack");
+        DPL_HIP(e);
+        if (mem == DPL_MEM_HOST) {
+            DPL_HIP(hipMemcpyAsync(hits, dh, size_t(n) * sizeof(dpl_hit), hipMemcpyDeviceToHost, s));
+            DPL_HIP(hipStreamSynchronize(s));
+        }
+    });
+}
+
+dpl_status dpl_trace_batched(dpl_context* ctx, const dpl_bvh*, const dpl_ray*, const int32_t*, int64_t, uint32_t,
+                             dpl_hit*, dpl_mem) {
+    return Fail(ctx, DPL_ERR_UNSUPPORTED, "batched traces are not available in this build");
+}
+
+dpl_status dpl_compute_codes(dpl_context* ctx, const float* tri, int64_t n, int32_t morton_type, uint64_t* codes,
+                             float* bounds12) {
+    if (!ctx) return DPL_ERR_INVALID_ARG;
+    return Guard(ctx, [&] {
+        Require(tri && codes && n >= 1, "invalid arguments");
+        Require(morton_type == 0 || morton_type == 1, "morton_type must be 0 or 1");
+        hipStream_t s = ctx->stream;
+        BuildBuffers b{};
+        b.n = n;
+        b.tri = tri;
+        b.keys = codes;
+        b.vals = ctx->vals.ensure<uint32_t>(size_t(n) * 4);
+        uint8_t* sm = ctx->small.ensure<uint8_t>(1024);
+        b.bounds = reinterpret_cast<uint32_t*>(sm);
+        // the init kernel touches these; give it valid scratch
+        b.tile_counter = reinterpret_cast<uint32_t*>(sm + 64);
+        b.iter_done = reinterpret_cast<int32_t*>(sm + 96);
+        b.counters = reinterpret_cast<int32_t*>(sm + 128);
+        b.sah = reinterpret_cast<double*>(sm + 192);
+        b.ccount = ctx->ccount.ensure<int32_t>(size_t(n + 2) * 4);
+        b.parent = ctx->parent.ensure<int32_t>(size_t(2 * n - 1) * 4);
+        b.status[0] = ctx->status0.ensure<uint64_t>(size_t(n / 256 + 2) * 8);
+        b.visit = ctx->visit.ensure<uint32_t>(size_t(n) * 4);
+        b.leafslot = ctx->leafslot.ensure<int32_t>(size_t(n) * 4);
+        DPL_HIP(LaunchBuildInit(b, 256, s));
+        DPL_HIP(LaunchBounds(b, s));
+        DPL_HIP(LaunchCodes(b, morton_type, s));
+        if (bounds12) {
+            uint32_t raw[13];
+            DPL_HIP(hipMemcpyAsync(raw, b.bounds, sizeof(raw), hipMemcpyDeviceToHost, s));
+            DPL_HIP(hipStreamSynchronize(s));
+            for (int k = 0; k < 12; k++) {
+                uint32_t key = raw[k];
+                uint32_t u = (key & 0x80000000u) ? (key & 0x7FFFFFFFu) : ~key;
+                std::memcpy(&bounds12[k], &u, 4);
+            }
+        }
+    });
+}
+
+dpl_status dpl_radix_sort_pairs(dpl_context* ctx, uint64_t* keys, uint32_t* values, int64_t n, int32_t key_bits,
+                                int32_t digit_bits) {
+    if (!ctx) return DPL_ERR_INVALID_ARG;
+    return Guard(ctx, [&] {
+        Require(keys && values && n >= 0, "invalid arguments");
+        Require(key_bits >= 1 && key_bits <= 64, "key_bits must be in [1, 64]");
+        Require(digit_bits == 4 || digit_bits == 8, "digit_bits must be 4 or 8");
+        if (n == 0) return;
+        const bool key64 = key_bits > 32;
+        hipStream_t s = ctx->stream;
+        uint64_t* ka = ctx->keys_alt.ensure<uint64_t>(size_t(n) * 8);
+        uint32_t* va = ctx->vals_alt.ensure<uint32_t>(size_t(n) * 4);
+        const size_t tb = RadixSortTempBytes(n, key_bits, digit_bits, key64);
+        void* tmp = ctx->sort_tmp.ensure(tb);
+        bool in_alt = false;
+        DPL_HIP(LaunchRadixSort(keys, values, ka, va, n, key_bits, digit_bits, key64, tmp, tb, &in_alt, s));
+        if (in_alt) {
+            DPL_HIP(hipMemcpyAsync(keys, ka, size_t(n) * (key64 ? 8 : 4), hipMemcpyDeviceToDevice, s));
+            DPL_HIP(hipMemcpyAsync(values, va, size_t(n) * 4, hipMemcpyDeviceToDevice, s));
+        }
+    });
+}
+
+dpl_status dpl_render_forward(dpl_context* ctx, const dpl_bvh*, const dpl_render_params*, const int32_t*, const float*,
+                              const float*, float*) {
+    return Fail(ctx, DPL_ERR_UNSUPPORTED, "rendering is not available in this build");
+}
+
+dpl_status dpl_render_adjoint(dpl_context* ctx, const dpl_bvh*, const dpl_render_params*, const int32_t*,
+                              const float*, const float*, const float*, float*, float*) {
+    return Fail(ctx, DPL_ERR_UNSUPPORTED, "rendering is not available in this build");
+}
+
+}  // extern "C"

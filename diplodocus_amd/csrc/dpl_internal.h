@@ -1,0 +1,202 @@
+// Internal declarations of libdpl: device record layouts, device helpers shared by the
+// build, sort and trace kernels, and the host-side launchers each .hip file exports.
+//
+// Bit-exactness contract: every stage whose output is compared bit for bit against the
+// CPU restatement (bounds, codes, order, PLOC merges, collapse, layout) uses only IEEE
+// add/sub/mul/div/sqrt in a fixed order, min/max in the std::min/std::max select form,
+// and is compiled with -ffp-contract=off (no FMA contraction) and IEEE denormals.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "../../include/dpl/dpl.h"
+
+namespace dpl {
+
+constexpr int kWave = 64;
+constexpr int kMaxRadius = 32;
+constexpr int kMaxLeaf = 15;
+
+// PLOC node / cluster record (32 B). For a leaf: left = input triangle, right = -1.
+// For a cluster: left = node id.
+struct alignas(16) PNode {
+    float lo[3], hi[3];
+    int32_t left, right;
+};
+static_assert(sizeof(PNode) == 32, "PNode is 32 bytes");
+
+// Final traversal node (64 B): both child boxes plus child references. A reference r >= 0
+// is an inner node; r < 0 is a leaf with ~r = (prim_begin << 4) | prim_count.
+struct alignas(16) FNode {
+    float lx0, lx1, ly0, ly1;
+    float rx0, rx1, ry0, ry1;
+    float lz0, lz1, rz0, rz1;
+    int32_t left, right, pad0, pad1;
+};
+static_assert(sizeof(FNode) == 64, "FNode is 64 bytes");
+
+// ---------------------------------------------------------------------------------------
+// device helpers
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ float Min2(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float Max2(float a, float b) { return (a < b) ? b : a; }
+// -0.0 -> +0.0; every loaded vertex coordinate goes through it.
+__device__ __forceinline__ float Canon(float x) { return x + 0.0f; }
+
+__device__ __forceinline__ uint32_t FloatToOrdered(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float OrderedToFloat(uint32_t k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+// The reference's avalanche hash (src/util/util.h:24-32).
+__host__ __device__ __forceinline__ uint32_t HashU32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+// Priority of an unordered cluster pair, used to break exact distance ties in the PLOC
+// nearest-neighbour search (identical for (i,j) and (j,i)).
+__device__ __forceinline__ uint32_t PairHash(int32_t i, int32_t j) {
+    uint32_t lo = uint32_t(i < j ? i : j), hi = uint32_t(i < j ? j : i);
+    return HashU32(lo * 0x9E3779B1u ^ hi);
+}
+
+__device__ __forceinline__ float SurfaceAreaUnion(float alx, float aly, float alz, float ahx, float ahy, float ahz,
+                                                  float blx, float bly, float blz, float bhx, float bhy, float bhz) {
+    float sx = Max2(ahx, bhx) - Min2(alx, blx);
+    float sy = Max2(ahy, bhy) - Min2(aly, bly);
+    float sz = Max2(ahz, bhz) - Min2(alz, blz);
+    return 2.0f * (sx * sy + sy * sz + sx * sz);
+}
+
+__device__ __forceinline__ float SurfaceAreaBox(const float* lo, const float* hi) {
+    float sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
+    return 2.0f * (sx * sy + sy * sz + sx * sz);
+}
+
+__device__ __forceinline__ int LaneId() { return __lane_id(); }
+
+// Inclusive scan of one uint32 per lane across the 64-lane wave.
+__device__ __forceinline__ uint32_t WaveInclusiveScan(uint32_t v) {
+    const int lane = LaneId();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        uint32_t o = __shfl_up(v, d, kWave);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ float WaveMin(float v) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d, kWave));
+    return v;
+}
+__device__ __forceinline__ float WaveMax(float v) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, kWave));
+    return v;
+}
+
+// Exclusive scan across a block of BLOCK threads; `total` receives the block sum.
+// `scratch` needs BLOCK/64 + 1 uint32 of LDS. Contains __syncthreads().
+template <int BLOCK>
+__device__ __forceinline__ uint32_t BlockExclusiveScan(uint32_t v, uint32_t* scratch, uint32_t& total) {
+    constexpr int W = BLOCK / kWave;
+    const int lane = LaneId();
+    const int wid = threadIdx.x / kWave;
+    uint32_t inc = WaveInclusiveScan(v);
+    if (lane == kWave - 1) scratch[wid] = inc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < W; w++) {
+            uint32_t t = scratch[w];
+            scratch[w] = acc;
+            acc += t;
+        }
+        scratch[W] = acc;
+    }
+    __syncthreads();
+    total = scratch[W];
+    uint32_t r = scratch[wid] + inc - v;
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------------------
+// launchers (host side); every one enqueues on `s` and returns the launch error
+// ---------------------------------------------------------------------------------------
+struct BuildBuffers {
+    int64_t n;
+    const float* tri;      // n x 9 (device)
+    uint32_t* bounds;      // 13: ordered vmin3 vmax3 cmin3 cmax3, error flag
+    uint64_t* keys;        // n (u32 view for Morton30)
+    uint32_t* vals;        // n
+    uint64_t* keys_alt;    // n
+    uint32_t* vals_alt;    // n
+    PNode* nodes;          // 2n-1
+    int32_t* parent;       // 2n-1
+    PNode* clusters[2];    // n each
+    int32_t* ccount;       // n+2: clusters at the start of each iteration
+    uint64_t* status[2];   // look-back words per tile
+    uint32_t* tile_counter;  // 2
+    int32_t* iter_done;    // 1: total iterations
+    int32_t* count;        // 2n-1
+    float* cost;           // 2n-1
+    uint8_t* collapsed;    // 2n-1
+    int32_t* minleaf;      // 2n-1
+    int32_t* height;       // 2n-1
+    uint32_t* visit;       // n (internal nodes)
+    int32_t* leafslot;     // n: count of the final leaf whose minimum sorted position is k
+    int32_t* keptflag;     // n: internal node kept as a final inner node
+    int32_t* scan_tmp;     // scan block sums
+    int32_t* counters;     // 8: [0] kept inner, [1] final leaves, [2] max height
+    double* sah;           // 1
+    FNode* fnodes;         // n (capacity)
+    float4* tri_ordered;   // 3n
+    int32_t* prim_order;   // n
+};
+
+hipError_t LaunchBuildInit(const BuildBuffers& b, int chunk, hipStream_t s);
+hipError_t LaunchBounds(const BuildBuffers& b, hipStream_t s);
+hipError_t LaunchCodes(const BuildBuffers& b, int morton_type, hipStream_t s);
+hipError_t LaunchInitClusters(const BuildBuffers& b, hipStream_t s);
+hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int tail, int grid,
+                               hipStream_t s);
+hipError_t LaunchPlocTail(const BuildBuffers& b, int iter, int radius, int tail, hipStream_t s);
+int PlocChunk(int block);
+int PlocTail();
+hipError_t LaunchCollapse(const BuildBuffers& b, int max_leaf, hipStream_t s);
+hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s);
+hipError_t LaunchExclusiveScan(const int32_t* in, int32_t* out, int64_t n, int32_t* tmp, int32_t* total,
+                               hipStream_t s);
+size_t ScanTempInts(int64_t n);
+
+hipError_t LaunchRadixSort(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
+                           int key_bits, int digit_bits, bool key64, void* temp, size_t temp_bytes, bool* result_in_alt,
+                           hipStream_t s);
+size_t RadixSortTempBytes(int64_t n, int key_bits, int digit_bits, bool key64);
+
+// A built tree as the traversal kernels see it.
+struct TraceTree {
+    const FNode* nodes;
+    const float4* tris;  // 3 float4 per primitive, leaf order
+    int32_t node_count;
+    int32_t prim_count;
+    int32_t height;
+};
+
+hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,
+                       int cu_count, uint32_t* counter, hipStream_t s);
+
+}  // namespace dpl

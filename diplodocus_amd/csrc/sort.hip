@@ -1,0 +1,251 @@
+// Onesweep LSD radix sort of (key, value) pairs for gfx950 (after Adinets & Merrill 2022,
+// "Onesweep: A Faster Least Significant Digit Radix Sort for GPUs").
+//
+//   1. one pass over the keys builds the digit histograms of every pass in LDS;
+//   2. one workgroup per pass turns them into exclusive digit bases;
+//   3. per pass, one kernel: each workgroup takes a tile id from an atomic counter, ranks
+//      its keys stably with 64-lane ballot matching (lanes holding the same digit form a
+//      peer mask; rank = lower peers + earlier items of the wave + earlier waves), learns
+//      the tile's global digit offsets by a decoupled look-back on per-(tile, digit)
+//      status words, and scatters keys and values.
+// Stable: equal keys keep their input order, so the result equals std::stable_sort by key.
+#include <hip/hip_runtime.h>
+
+#include "dpl_internal.h"
+
+namespace dpl {
+
+namespace {
+
+constexpr int kSortBlock = 256;
+constexpr int kSortWaves = kSortBlock / kWave;
+constexpr uint32_t kStAgg = 1u << 30;
+constexpr uint32_t kStInc = 2u << 30;
+constexpr uint32_t kStMask = (1u << 30) - 1u;
+
+template <typename K>
+struct SortTraits {
+    static constexpr int kIpt = sizeof(K) == 4 ? 16 : 12;
+    static constexpr int kTile = kSortBlock * kIpt;
+};
+
+struct SortTemp {
+    uint32_t* hist;          // P x RADIX
+    uint32_t* base;          // P x RADIX
+    uint32_t* tile_counter;  // P
+    uint32_t* status;        // P x T x RADIX
+};
+
+int Passes(int key_bits, int digit_bits) { return (key_bits + digit_bits - 1) / digit_bits; }
+
+SortTemp CarveTemp(void* temp, int passes, int radix, int64_t tiles) {
+    SortTemp t;
+    uint32_t* p = static_cast<uint32_t*>(temp);
+    t.hist = p;
+    p += size_t(passes) * radix;
+    t.base = p;
+    p += size_t(passes) * radix;
+    t.tile_counter = p;
+    p += 64;
+    t.status = p;
+    return t;
+}
+
+}  // namespace
+
+__global__ void KSortClear(uint32_t* p, int64_t words) {
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < words; i += int64_t(gridDim.x) * blockDim.x)
+        p[i] = 0u;
+}
+
+template <typename K, int DB>
+__global__ __launch_bounds__(kSortBlock) void KSortHistogram(const K* __restrict__ keys, int64_t n, int passes,
+                                                              uint32_t* hist) {
+    constexpr int RADIX = 1 << DB;
+    constexpr int MAXP = (8 * int(sizeof(K)) + DB - 1) / DB;
+    __shared__ uint32_t s_h[MAXP * RADIX];
+    for (int i = threadIdx.x; i < passes * RADIX; i += kSortBlock) s_h[i] = 0u;
+    __syncthreads();
+    for (int64_t i = blockIdx.x * int64_t(kSortBlock) + threadIdx.x; i < n; i += int64_t(gridDim.x) * kSortBlock) {
+        const K k = keys[i];
+        for (int p = 0; p < passes; p++) atomicAdd(&s_h[p * RADIX + int((k >> (p * DB)) & K(RADIX - 1))], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < passes * RADIX; i += kSortBlock) {
+        const uint32_t v = s_h[i];
+        if (v) atomicAdd(&hist[i], v);
+    }
+}
+
+template <int DB>
+__global__ __launch_bounds__(kSortBlock) void KSortBases(const uint32_t* hist, uint32_t* base) {
+    constexpr int RADIX = 1 << DB;
+    __shared__ uint32_t s_scan[kSortBlock / kWave + 1];
+    const int p = blockIdx.x;
+    const uint32_t v = threadIdx.x < RADIX ? hist[p * RADIX + threadIdx.x] : 0u;
+    uint32_t tot;
+    const uint32_t ex = BlockExclusiveScan<kSortBlock>(v, s_scan, tot);
+    if (threadIdx.x < RADIX) base[p * RADIX + threadIdx.x] = ex;
+}
+
+template <typename K, int DB>
+__global__ __launch_bounds__(kSortBlock) void KOnesweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                        K* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n,
+                                                        int pass, int shift, SortTemp tmp, int64_t tiles) {
+    constexpr int RADIX = 1 << DB;
+    constexpr int IPT = SortTraits<K>::kIpt;
+    constexpr int TILE = SortTraits<K>::kTile;
+    __shared__ uint32_t s_wh[kSortWaves][RADIX];
+    __shared__ uint32_t s_off[RADIX];
+    __shared__ int s_tile;
+
+    if (threadIdx.x == 0) s_tile = int(atomicAdd(&tmp.tile_counter[pass], 1u));
+    for (int i = threadIdx.x; i < kSortWaves * RADIX; i += kSortBlock) (&s_wh[0][0])[i] = 0u;
+    __syncthreads();
+    const int64_t tile = s_tile;
+    if (tile >= tiles) return;
+
+    const int lane = LaneId(), w = threadIdx.x / kWave;
+    const uint64_t lt_mask = (1ull << lane) - 1ull;
+    const int64_t wbase = tile * TILE + int64_t(w) * (kWave * IPT);
+    K key[IPT];
+    uint32_t rank[IPT];
+    uint32_t dig[IPT];
+    volatile uint32_t* wh = s_wh[w];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const int64_t i = wbase + int64_t(j) * kWave + lane;
+        const bool valid = i < n;
+        key[j] = valid ? kin[i] : K(0);
+        const uint32_t d = uint32_t((key[j] >> shift) & K(RADIX - 1));
+        dig[j] = valid ? d : uint32_t(RADIX);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < DB; b++) {
+            const bool bit = (d >> b) & 1u;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        uint32_t r = 0;
+        if (valid) {
+            const uint32_t before = wh[d];
+            r = before + uint32_t(__popcll(peers & lt_mask));
+            __builtin_amdgcn_wave_barrier();
+            const int leader = __ffsll(static_cast<unsigned long long>(peers)) - 1;
+            if (lane == leader) wh[d] = before + uint32_t(__popcll(peers));
+        } else {
+            __builtin_amdgcn_wave_barrier();
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        rank[j] = r;
+    }
+    __syncthreads();
+
+    // per digit: wave prefixes, tile count, decoupled look-back across tiles
+    uint32_t* st = tmp.status + size_t(pass) * size_t(tiles) * RADIX;
+    for (int r = threadIdx.x; r < RADIX; r += kSortBlock) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int ww = 0; ww < kSortWaves; ww++) {
+            const uint32_t t = s_wh[ww][r];
+            s_wh[ww][r] = acc;
+            acc += t;
+        }
+        uint32_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&st[r], kStInc | acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&st[size_t(tile) * RADIX + r], kStAgg | acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t p = tile - 1;
+            for (;;) {
+                const uint32_t v = __hip_atomic_load(&st[size_t(p) * RADIX + r], __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t flag = v & ~kStMask;
+                if (flag == 0u) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += v & kStMask;
+                if (flag == kStInc) break;
+                p--;
+            }
+            __hip_atomic_store(&st[size_t(tile) * RADIX + r], kStInc | (excl + acc), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_off[r] = tmp.base[pass * RADIX + r] + excl;
+    }
+    __syncthreads();
+
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const int64_t i = wbase + int64_t(j) * kWave + lane;
+        if (i >= n) continue;
+        const uint32_t d = dig[j];
+        const uint32_t dst = s_off[d] + s_wh[w][d] + rank[j];
+        kout[dst] = key[j];
+        vout[dst] = vin[i];
+    }
+}
+
+size_t RadixSortTempBytes(int64_t n, int key_bits, int digit_bits, bool key64) {
+    const int passes = Passes(key_bits, digit_bits);
+    const int radix = 1 << digit_bits;
+    const int64_t tile = key64 ? SortTraits<uint64_t>::kTile : SortTraits<uint32_t>::kTile;
+    const int64_t tiles = (n + tile - 1) / tile;
+    return (size_t(passes) * radix * 2 + 64 + size_t(passes) * size_t(tiles) * radix) * sizeof(uint32_t);
+}
+
+template <typename K, int DB>
+static hipError_t SortImpl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_alt, int64_t n, int key_bits,
+                           void* temp, size_t temp_bytes, bool* result_in_alt, hipStream_t s) {
+    constexpr int RADIX = 1 << DB;
+    const int passes = Passes(key_bits, DB);
+    const int64_t tile = SortTraits<K>::kTile;
+    const int64_t tiles = (n + tile - 1) / tile;
+    const size_t words = temp_bytes / sizeof(uint32_t);
+    if (RadixSortTempBytes(n, key_bits, DB, sizeof(K) == 8) > temp_bytes) return hipErrorInvalidValue;
+    SortTemp t = CarveTemp(temp, passes, RADIX, tiles);
+    {
+        const int64_t clear = int64_t(words);
+        int grid = int(std::min<int64_t>(2048, (clear + 255) / 256));
+        hipLaunchKernelGGL(KSortClear, dim3(grid), dim3(256), 0, s, static_cast<uint32_t*>(temp), clear);
+    }
+    {
+        int grid = int(std::min<int64_t>(1024, (n + kSortBlock * 16 - 1) / (kSortBlock * 16)));
+        if (grid < 1) grid = 1;
+        hipLaunchKernelGGL((KSortHistogram<K, DB>), dim3(grid), dim3(kSortBlock), 0, s, keys, n, passes, t.hist);
+    }
+    hipLaunchKernelGGL(KSortBases<DB>, dim3(passes), dim3(kSortBlock), 0, s, t.hist, t.base);
+    K* ki = keys;
+    uint32_t* vi = vals;
+    K* ko = keys_alt;
+    uint32_t* vo = vals_alt;
+    for (int p = 0; p < passes; p++) {
+        hipLaunchKernelGGL((KOnesweep<K, DB>), dim3(unsigned(tiles)), dim3(kSortBlock), 0, s, ki, vi, ko, vo, n, p,
+                           p * DB, t, tiles);
+        std::swap(ki, ko);
+        std::swap(vi, vo);
+    }
+    *result_in_alt = (passes % 2) == 1;
+    return hipGetLastError();
+}
+
+hipError_t LaunchRadixSort(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
+                           int key_bits, int digit_bits, bool key64, void* temp, size_t temp_bytes, bool* result_in_alt,
+                           hipStream_t s) {
+    *result_in_alt = false;
+    if (n <= 0) return hipSuccess;
+    if (key64) {
+        if (digit_bits == 4)
+            return SortImpl<uint64_t, 4>(keys, vals, keys_alt, vals_alt, n, key_bits, temp, temp_bytes, result_in_alt, s);
+        return SortImpl<uint64_t, 8>(keys, vals, keys_alt, vals_alt, n, key_bits, temp, temp_bytes, result_in_alt, s);
+    }
+    uint32_t* k32 = reinterpret_cast<uint32_t*>(keys);
+    uint32_t* a32 = reinterpret_cast<uint32_t*>(keys_alt);
+    if (digit_bits == 4)
+        return SortImpl<uint32_t, 4>(k32, vals, a32, vals_alt, n, key_bits, temp, temp_bytes, result_in_alt, s);
+    return SortImpl<uint32_t, 8>(k32, vals, a32, vals_alt, n, key_bits, temp, temp_bytes, result_in_alt, s);
+}
+
+}  // namespace dpl

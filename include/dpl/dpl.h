@@ -1,0 +1,215 @@
+/*
+ * dpl.h — C ABI of the MI355X BVH build-and-trace engine (libdpl.so).
+ *
+ * The reference has no C ABI for this path; its device boundary is the C++ namespace API
+ * diplodocus::cuda_kernels (src/gpu/renderer/gpu_renderer_api.h:8-11), GpuScene
+ * (src/gpu/scene/gpu_scene.h:53-92) and the acceleration-structure plug-in interface
+ * AccelerationStructure::{Build, Intersect, IntersectAny}
+ * (src/acceleration/acceleration_structure.h:12-19). Each entry point below names the
+ * reference interface it replaces.
+ *
+ * Conventions
+ *   - Every function returns dpl_status (0 = OK) and never throws; dpl_last_error() gives a
+ *     message for the last failure on a context.
+ *   - Pointer arguments are host or device memory as stated by a dpl_mem argument; nothing
+ *     is guessed. Device pointers must be valid on the context's device.
+ *   - Work is ordered on the context's stream (dpl_context_set_stream). Functions that return
+ *     host data (stats, downloads, host-memory outputs) synchronise that stream.
+ *   - A context is used by one thread at a time; distinct contexts are independent.
+ *   - The caller owns every input and output buffer; the library owns what is behind the
+ *     opaque handles.
+ */
+#ifndef DPL_DPL_H_
+#define DPL_DPL_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPL_VERSION 1
+
+typedef enum dpl_status {
+    DPL_OK = 0,
+    DPL_ERR_INVALID_ARG = 1,
+    DPL_ERR_OOM = 2,
+    DPL_ERR_DEVICE = 3,
+    DPL_ERR_UNSUPPORTED = 4,
+    DPL_ERR_INTERNAL = 5
+} dpl_status;
+
+typedef enum dpl_mem { DPL_MEM_HOST = 0, DPL_MEM_DEVICE = 1 } dpl_mem;
+
+/* Mirrors AccelerationStructureType (src/config/acceleration_structure_config.h:9). */
+typedef enum dpl_accel_type {
+    DPL_ACCEL_DUMMY = 0,
+    DPL_ACCEL_SBVH = 1,
+    DPL_ACCEL_PLOC = 2,
+    DPL_ACCEL_PLOC_EMC = 3,
+    DPL_ACCEL_PLOC_SOBB = 4,
+    DPL_ACCEL_PLOC_EMC_SOBB = 5
+} dpl_accel_type;
+
+/* Mirrors MortonType / BoundingVolumeType (src/gpu/config/gpu_acceleration_structure_config.h:5-7). */
+typedef enum dpl_morton_type { DPL_MORTON30 = 0, DPL_EMC60 = 1 } dpl_morton_type;
+typedef enum dpl_bv_type { DPL_BV_AABB = 0, DPL_BV_SOBB = 1 } dpl_bv_type;
+
+/* Build parameters. Mirrors GpuAccelerationStructureConfig
+ * (src/gpu/config/gpu_acceleration_structure_config.h:9-15) plus the engine's tuning knobs.
+ * Zero in block_size / sort_digit_bits selects the default. */
+typedef struct dpl_build_params {
+    int32_t morton_type;            /* dpl_morton_type */
+    int32_t bv_type;                /* dpl_bv_type */
+    int32_t nn_search_radius;       /* PLOC radius, 1..32 */
+    int32_t max_triangles_per_leaf; /* 1..15 */
+    int32_t kdop_size;              /* SOBB direction-set hint (reference default 32) */
+    int32_t block_size;             /* PLOC tile threads: 0, 128, 256 or 512 */
+    int32_t sort_digit_bits;        /* radix digit width: 0, 4 or 8 */
+    int32_t keep_intermediates;     /* non-zero: keep codes/order/merge list for download */
+} dpl_build_params;
+
+/* Fills *p from an acceleration-structure type and the reference's config values.
+ * Replaces BridgeAccelerationConfig (src/gpu/config/gpu_config_bridge.cc:24-60).
+ * Returns DPL_ERR_UNSUPPORTED for DUMMY and SBVH (CPU-only structures). */
+dpl_status dpl_build_params_from_type(int32_t accel_type, int32_t nn_search_radius, int32_t max_triangles_per_leaf,
+                                      int32_t kdop_size, dpl_build_params* p);
+
+/* Build statistics. Mirrors AccelerationStats (src/stats/acceleration_stats.h:5-17) and
+ * adds the SAH cost and per-stage device times. */
+typedef struct dpl_build_stats {
+    double build_ms;         /* device time of the whole build (hipEvents), upload excluded */
+    double upload_ms;        /* host-to-device copy of the triangles (0 for device input) */
+    double stage_ms[8];      /* bounds, codes, sort, ploc, collapse, layout, sobb, stats */
+    int32_t node_count;      /* final nodes: inner + leaves */
+    int32_t inner_node_count;
+    int32_t leaf_node_count;
+    int32_t ploc_iterations;
+    int64_t memory_consumption; /* bytes of the final BVH (nodes + ordered triangles + ids) */
+    double sah_cost;         /* (Ct*sum_inner SA + Ci*sum_leaf SA*|leaf|) / SA(root), Ct=Ci=1 */
+    int32_t tree_height;     /* longest root-to-leaf path of the final tree */
+    int32_t reserved;
+} dpl_build_stats;
+
+typedef struct dpl_context dpl_context;
+typedef struct dpl_bvh dpl_bvh;
+
+/* 32-byte ray and 16-byte hit. Replaces Ray / RayHit (src/scene/ray.h:7-13;
+ * src/scene/ray_hit.h:8-17) at the boundary; the reference's implicit t_min (kEpsilon) is
+ * explicit here. prim = -1 on a miss; prim is the input (original) triangle index. */
+typedef struct dpl_ray {
+    float ox, oy, oz, tmin;
+    float dx, dy, dz, tmax;
+} dpl_ray;
+
+typedef struct dpl_hit {
+    float t, b1, b2;
+    int32_t prim;
+} dpl_hit;
+
+#define DPL_TRACE_ANY 1u       /* any-hit (shadow) query: IntersectAny */
+#define DPL_TRACE_CULL_BACK 2u /* backface culling */
+
+/* Context: one device, one stream, scratch pools. The reference selects the device with
+ * cudaSetDevice(0) (src/gpu/cuda_utils.h:41). */
+dpl_status dpl_context_create(int32_t device, dpl_context** out);
+void dpl_context_destroy(dpl_context* ctx);
+const char* dpl_last_error(const dpl_context* ctx);
+/* stream: a hipStream_t; NULL selects the device's null stream. A new context uses a
+ * non-blocking stream of its own. */
+dpl_status dpl_context_set_stream(dpl_context* ctx, void* stream);
+dpl_status dpl_context_synchronize(dpl_context* ctx);
+/* Number of compute units of the context's device (for callers that size work). */
+int32_t dpl_context_cu_count(const dpl_context* ctx);
+
+/* Builds a BVH over n triangles (n x 9 floats: v0 xyz, v1 xyz, v2 xyz).
+ * Replaces AccelerationStructure::Build (src/acceleration/acceleration_structure.h:15-16)
+ * for the PLOC types that the reference's factory rejects
+ * (src/acceleration/acceleration_structure_factory.cc:16-17). stats may be NULL. */
+dpl_status dpl_bvh_build(dpl_context* ctx, const float* tri_xyz, int64_t n, dpl_mem mem,
+                         const dpl_build_params* params, dpl_bvh** out, dpl_build_stats* stats);
+
+/* Rebuilds an existing BVH in place from new triangles (same or smaller capacity reuses
+ * the device buffers). */
+dpl_status dpl_bvh_rebuild(dpl_bvh* bvh, const float* tri_xyz, int64_t n, dpl_mem mem,
+                           const dpl_build_params* params, dpl_build_stats* stats);
+
+/* Builds mesh_count independent BVHs over tri_xyz[offsets[m]..offsets[m+1]) into one handle
+ * (all meshes in one pass per stage). offsets has mesh_count+1 entries in host memory. */
+dpl_status dpl_bvh_build_batched(dpl_context* ctx, const float* tri_xyz, const int64_t* mesh_offsets,
+                                 int32_t mesh_count, dpl_mem mem, const dpl_build_params* params, dpl_bvh** out,
+                                 dpl_build_stats* stats);
+
+void dpl_bvh_destroy(dpl_bvh* bvh);
+dpl_status dpl_bvh_get_stats(const dpl_bvh* bvh, dpl_build_stats* stats);
+
+/* Closest-hit (or any-hit with DPL_TRACE_ANY) for n rays. Replaces
+ * AccelerationStructure::Intersect / IntersectAny (acceleration_structure.h:17-18) and the
+ * brute-force DummyIntersect (src/gpu/renderer/gpu_renderer_util.h:29-94).
+ * Closest-hit returns the smallest t with tmin < t < tmax, ties to the lower prim id. */
+dpl_status dpl_trace(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
+                     dpl_hit* hits, dpl_mem mem);
+
+/* Batched BVH: ray i is traced against mesh ray_mesh[i] (device or host as `mem`). */
+dpl_status dpl_trace_batched(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, const int32_t* ray_mesh,
+                             int64_t n, uint32_t flags, dpl_hit* hits, dpl_mem mem);
+
+/* Sizes (in elements) of the arrays dpl_bvh_download fills; any count may be zero when the
+ * BVH was built without keep_intermediates. */
+typedef struct dpl_bvh_layout {
+    int64_t prim_count;
+    int64_t node_count;       /* final 64-byte nodes */
+    int64_t ploc_node_count;  /* 2n-1 PLOC nodes (32 bytes) when kept */
+    int64_t iteration_count;  /* PLOC iterations when kept */
+    int32_t mesh_count;
+    int32_t node_bytes;       /* 64 */
+} dpl_bvh_layout;
+dpl_status dpl_bvh_get_layout(const dpl_bvh* bvh, dpl_bvh_layout* layout);
+
+/* Host copies of the BVH (any pointer may be NULL):
+ *   nodes        node_count x 64 B   (child boxes + child references)
+ *   tri_ordered  prim_count x 48 B   (v0 xyz id | v1 xyz 0 | v2 xyz 0, leaf order)
+ *   prim_order   prim_count int32    (leaf order -> input triangle index)
+ * and, when built with keep_intermediates:
+ *   codes        prim_count uint64   (per input triangle)
+ *   sorted_idx   prim_count uint32   (stable order by code)
+ *   ploc_nodes   (2n-1) x 32 B       (box, left, right; leaves first, merges in creation order)
+ *   iter_counts  iteration_count int32 (clusters at the start of each PLOC iteration) */
+dpl_status dpl_bvh_download(const dpl_bvh* bvh, void* nodes, void* tri_ordered, int32_t* prim_order, uint64_t* codes,
+                            uint32_t* sorted_idx, void* ploc_nodes, int32_t* iter_counts);
+
+/* Stand-alone build stages for parity checks and the autotuner (device memory). */
+dpl_status dpl_compute_codes(dpl_context* ctx, const float* tri_xyz, int64_t n, int32_t morton_type, uint64_t* codes,
+                             float* bounds12);
+dpl_status dpl_radix_sort_pairs(dpl_context* ctx, uint64_t* keys, uint32_t* values, int64_t n, int32_t key_bits,
+                                int32_t digit_bits);
+
+/* ---- Differentiable diffuse path tracing ----
+ * Replaces the GPU "bounce" kernel launcher LaunchPathtracingKernel
+ * (src/gpu/renderer/gpu_renderer_api.h:11) with a diffuse path tracer over the BVH.
+ * camera: pos3 dir3 up3 fov far (src/scene/camera.h:7-13). */
+typedef struct dpl_render_params {
+    int32_t width, height, spp, max_depth;
+    uint64_t seed;
+    float camera[11];
+    int32_t material_count;
+    int32_t reserved;
+} dpl_render_params;
+
+/* Device memory: mat_id (prim_count int32, input order), albedo/emission (M x 3 floats),
+ * image (W*H*3 floats, row-major, src/framebuffer/framebuffer.cc:47-52). */
+dpl_status dpl_render_forward(dpl_context* ctx, const dpl_bvh* bvh, const dpl_render_params* rp, const int32_t* mat_id,
+                              const float* albedo, const float* emission, float* image);
+
+/* Adjoint by path replay: given dL/dimage (W*H*3), accumulates dL/dalbedo (M x 3) and
+ * dL/demission (M x 3) into the output arrays (zeroed by the call). Device memory. */
+dpl_status dpl_render_adjoint(dpl_context* ctx, const dpl_bvh* bvh, const dpl_render_params* rp,
+                              const int32_t* mat_id, const float* albedo, const float* emission,
+                              const float* d_image, float* grad_albedo, float* grad_emission);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DPL_DPL_H_ */

@@ -1,0 +1,274 @@
+"""ctypes wrapper of the oracle (TEST INFRASTRUCTURE ONLY).
+
+Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the checker
+and the timed CPU baseline; never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+
+PNODE_DTYPE = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("left", "<i4"), ("right", "<i4")])
+FNODE_DTYPE = np.dtype([("box", "<f4", 12), ("left", "<i4"), ("right", "<i4"), ("pad", "<i4", 2)])
+HIT_DTYPE = np.dtype([("t", "<f4"), ("b1", "<f4"), ("b2", "<f4"), ("prim", "<i4")])
+
+
+def build() -> None:
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def _load():
+    if not os.path.exists(_LIB):
+        build()
+    lib = C.CDLL(_LIB)
+    P, I32, I64, U32, U64, F, D = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64, C.c_float, C.c_double
+    sig = {
+        "orc_hash_u32": (U32, [U32]),
+        "orc_u01": (F, [U32]),
+        "orc_ray_epsilon": (F, [F, F, F, F]),
+        "orc_centroid": (None, [P, P]),
+        "orc_triangle_area": (F, [P]),
+        "orc_intersect_mt": (F, [P, P, I32, P, P]),
+        "orc_intersect_watertight": (I32, [P, P, I32, P, P, P]),
+        "orc_intersect_box": (I32, [P, P, P, F, P]),
+        "orc_intersect_box_ref": (I32, [P, P, P, P, P]),
+        "orc_intersect_fp64": (D, [P, P, P, P]),
+        "orc_sbvh_build": (P, [P, I64, I32, I32, I32, I32, P, P]),
+        "orc_sbvh_free": (None, [P]),
+        "orc_sbvh_node_capacity": (I64, [P]),
+        "orc_sbvh_nodes_used": (I64, [P]),
+        "orc_sbvh_export": (None, [P, P, P, P]),
+        "orc_sbvh_sah": (D, [P]),
+        "orc_sbvh_trace": (D, [P, P, I64, U32, I32, P, I32, P]),
+        "orc_bounds": (None, [P, I64, P]),
+        "orc_expand_bits10": (U32, [U32]),
+        "orc_codes": (None, [P, I64, I32, P]),
+        "orc_emc_layout": (None, [P, P, P]),
+        "orc_ploc_nn": (None, [P, I32, I32, P]),
+        "orc_sa_union": (F, [P, P, P, P]),
+        "orc_ploc_build": (P, [P, I64, I32, I32, I32, P]),
+        "orc_ploc_free": (None, [P]),
+        "orc_ploc_iterations": (I32, [P]),
+        "orc_ploc_final_nodes": (I64, [P]),
+        "orc_ploc_export": (None, [P] * 14),
+        "orc_ploc_sah": (D, [P]),
+        "orc_ploc_trace": (None, [P, P, I64, U32, P, I32]),
+        "orc_trace_final": (None, [P, I64, P, I64, P, I64, U32, P, I32]),
+        "orc_sah_final": (D, [P, I64]),
+        "orc_trace_brute": (None, [P, I64, P, I64, U32, P, I32]),
+        "orc_pcg_stream": (None, [U64, U64, I32, P]),
+        "orc_sincos2pi": (None, [F, P, P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def nthreads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+# ---- small helpers -----------------------------------------------------------------------
+def hash_u32(x: int) -> int:
+    return int(lib.orc_hash_u32(x))
+
+
+def centroid(tri9) -> np.ndarray:
+    t = _f32(tri9)
+    out = np.zeros(3, np.float32)
+    lib.orc_centroid(_p(t), _p(out))
+    return out
+
+
+def intersect_mt(tri9, ray8, cull=False):
+    t, r = _f32(tri9), _f32(ray8)
+    b1, b2 = C.c_float(), C.c_float()
+    tt = lib.orc_intersect_mt(_p(t), _p(r), int(cull), C.byref(b1), C.byref(b2))
+    return tt, b1.value, b2.value
+
+
+def intersect_watertight(tri9, ray8, cull=False):
+    t, r = _f32(tri9), _f32(ray8)
+    tt, b1, b2 = C.c_float(), C.c_float(), C.c_float()
+    ok = lib.orc_intersect_watertight(_p(t), _p(r), int(cull), C.byref(tt), C.byref(b1), C.byref(b2))
+    return bool(ok), tt.value, b1.value, b2.value
+
+
+def intersect_fp64(tri9, ray8):
+    t, r = _f32(tri9), _f32(ray8)
+    b1, b2 = C.c_double(), C.c_double()
+    tt = lib.orc_intersect_fp64(_p(t), _p(r), C.byref(b1), C.byref(b2))
+    return tt, b1.value, b2.value
+
+
+def bounds(tris) -> np.ndarray:
+    t = _f32(tris)
+    out = np.zeros(12, np.float32)
+    lib.orc_bounds(_p(t), t.shape[0], _p(out))
+    return out
+
+
+def codes(tris, morton_type: int) -> np.ndarray:
+    t = _f32(tris)
+    out = np.zeros(t.shape[0], np.uint64)
+    lib.orc_codes(_p(t), t.shape[0], morton_type, _p(out))
+    return out
+
+
+def emc_layout(bounds12):
+    b = _f32(bounds12)
+    nb = np.zeros(3, np.int32)
+    ax = np.zeros(48, np.int8)
+    lib.orc_emc_layout(_p(b), _p(nb), _p(ax))
+    return nb, ax
+
+
+def ploc_nn(lohi, radius: int) -> np.ndarray:
+    b = _f32(lohi)
+    out = np.zeros(b.shape[0], np.int32)
+    lib.orc_ploc_nn(_p(b), b.shape[0], radius, _p(out))
+    return out
+
+
+def trace_brute(tris, rays, flags=0, threads=None) -> np.ndarray:
+    t, r = _f32(tris), _f32(rays)
+    hits = np.zeros(r.shape[0], HIT_DTYPE)
+    lib.orc_trace_brute(_p(t), t.shape[0], _p(r), r.shape[0], flags, _p(hits), threads or nthreads())
+    return hits
+
+
+def trace_final(fnodes, tri_ordered, rays, flags=0, threads=None) -> np.ndarray:
+    fn = np.ascontiguousarray(fnodes)
+    to = _f32(tri_ordered)
+    r = _f32(rays)
+    hits = np.zeros(r.shape[0], HIT_DTYPE)
+    lib.orc_trace_final(_p(fn), fn.shape[0], _p(to), to.size // 12, _p(r), r.shape[0], flags, _p(hits),
+                        threads or nthreads())
+    return hits
+
+
+def sah_final(fnodes) -> float:
+    fn = np.ascontiguousarray(fnodes)
+    return float(lib.orc_sah_final(_p(fn), fn.shape[0]))
+
+
+def pcg_stream(seed: int, stream: int, count: int) -> np.ndarray:
+    out = np.zeros(count, np.uint32)
+    lib.orc_pcg_stream(seed, stream, count, _p(out))
+    return out
+
+
+def sincos2pi(u: float):
+    s, c = C.c_float(), C.c_float()
+    lib.orc_sincos2pi(u, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+# ---- SBvh --------------------------------------------------------------------------------
+class SBvh:
+    """Restated reference SAH BVH (src/acceleration/sbvh.cc)."""
+
+    def __init__(self, tris, max_depth=32, max_leaf=4, bin_count=16, fix_child_bounds=False):
+        self.tris = _f32(tris)
+        st = np.zeros(3, np.int32)
+        ms = C.c_double()
+        self.h = lib.orc_sbvh_build(_p(self.tris), self.tris.shape[0], max_depth, max_leaf, bin_count,
+                                    int(fix_child_bounds), _p(st), C.byref(ms))
+        if not self.h:
+            raise ValueError("empty scene")
+        self.node_count, self.inner_node_count, self.leaf_node_count = (int(x) for x in st)
+        self.build_ms = ms.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_sbvh_free(self.h)
+            self.h = None
+
+    def export(self):
+        cap = lib.orc_sbvh_node_capacity(self.h)
+        mm = np.zeros((cap, 6), np.float32)
+        bc = np.zeros((cap, 2), np.int32)
+        ti = np.zeros(self.tris.shape[0], np.int32)
+        lib.orc_sbvh_export(self.h, _p(mm), _p(bc), _p(ti))
+        used = lib.orc_sbvh_nodes_used(self.h)
+        return mm[:used], bc[:used], ti
+
+    def sah(self) -> float:
+        return float(lib.orc_sbvh_sah(self.h))
+
+    def trace(self, rays, flags=0, mode=1, threads=None):
+        r = _f32(rays)
+        hits = np.zeros(r.shape[0], HIT_DTYPE)
+        vis = C.c_int64()
+        secs = lib.orc_sbvh_trace(self.h, _p(r), r.shape[0], flags, mode, _p(hits), threads or nthreads(),
+                                  C.byref(vis))
+        self.last_trace_s = secs
+        self.last_visits = vis.value
+        return hits
+
+
+# ---- PLOC restatement --------------------------------------------------------------------
+class Ploc:
+    """Sequential restatement of the engine's PLOC build (codes, order, merges, collapse,
+    final layout)."""
+
+    def __init__(self, tris, morton_type=0, radius=16, max_leaf=4):
+        self.tris = _f32(tris)
+        ms = C.c_double()
+        self.h = lib.orc_ploc_build(_p(self.tris), self.tris.shape[0], morton_type, radius, max_leaf, C.byref(ms))
+        if not self.h:
+            raise ValueError("empty scene")
+        self.build_ms = ms.value
+        n = self.tris.shape[0]
+        self.n = n
+        it = lib.orc_ploc_iterations(self.h)
+        nf = lib.orc_ploc_final_nodes(self.h)
+        self.codes = np.zeros(n, np.uint64)
+        self.sorted_codes = np.zeros(n, np.uint64)
+        self.sorted_idx = np.zeros(n, np.uint32)
+        self.nodes = np.zeros(2 * n - 1, PNODE_DTYPE)
+        self.parent = np.zeros(2 * n - 1, np.int32)
+        self.iter_counts = np.zeros(it, np.int32)
+        self.cost = np.zeros(2 * n - 1, np.float32)
+        self.count = np.zeros(2 * n - 1, np.int32)
+        self.collapsed = np.zeros(2 * n - 1, np.uint8)
+        self.fnodes = np.zeros(nf, FNODE_DTYPE)
+        self.prim_order = np.zeros(n, np.int32)
+        self.tri_ordered = np.zeros(12 * n, np.float32)
+        self.bounds = np.zeros(12, np.float32)
+        arrs = [self.codes, self.sorted_codes, self.sorted_idx, self.nodes, self.parent, self.iter_counts, self.cost,
+                self.count, self.collapsed, self.fnodes, self.prim_order, self.tri_ordered, self.bounds]
+        lib.orc_ploc_export(self.h, *[_p(a) for a in arrs])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib.orc_ploc_free(self.h)
+            self.h = None
+
+    def sah(self) -> float:
+        return float(lib.orc_ploc_sah(self.h))
+
+    def trace(self, rays, flags=0, threads=None):
+        r = _f32(rays)
+        hits = np.zeros(r.shape[0], HIT_DTYPE)
+        lib.orc_ploc_trace(self.h, _p(r), r.shape[0], flags, _p(hits), threads or nthreads())
+        return hits

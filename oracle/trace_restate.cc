@@ -1,0 +1,226 @@
+// Oracle — TEST INFRASTRUCTURE ONLY (see orc.h).
+// Closest-hit / any-hit tracing with the watertight ray-triangle test of Woop, Benthin and
+// Wald (JCGT 2013) and a conservative slab test (Ize 2013, "Robust BVH Ray Traversal").
+// Closest-hit rule: the smallest t with tmin < t < tmax, ties to the lower primitive id —
+// a result that does not depend on the tree or the traversal order.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "orc.h"
+
+namespace orc {
+
+WtRay PrepareRay(const RayRec& r) {
+    WtRay w;
+    const float d[3] = {r.dx, r.dy, r.dz};
+    float ax = std::fabs(r.dx), ay = std::fabs(r.dy), az = std::fabs(r.dz);
+    int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    if (d[kz] < 0.0f) std::swap(kx, ky);
+    w.kx = kx, w.ky = ky, w.kz = kz;
+    w.Sx = d[kx] / d[kz];
+    w.Sy = d[ky] / d[kz];
+    w.Sz = 1.0f / d[kz];
+    w.ox = r.ox, w.oy = r.oy, w.oz = r.oz;
+    w.idx = 1.0f / r.dx;
+    w.idy = 1.0f / r.dy;
+    w.idz = 1.0f / r.dz;
+    return w;
+}
+
+bool IntersectWatertight(const WtRay& w, const float* v0, const float* v1, const float* v2, bool cull, float& t,
+                         float& b1, float& b2) {
+    const float A[3] = {v0[0] - w.ox, v0[1] - w.oy, v0[2] - w.oz};
+    const float B[3] = {v1[0] - w.ox, v1[1] - w.oy, v1[2] - w.oz};
+    const float C[3] = {v2[0] - w.ox, v2[1] - w.oy, v2[2] - w.oz};
+    const float Ax = A[w.kx] - w.Sx * A[w.kz];
+    const float Ay = A[w.ky] - w.Sy * A[w.kz];
+    const float Bx = B[w.kx] - w.Sx * B[w.kz];
+    const float By = B[w.ky] - w.Sy * B[w.kz];
+    const float Cx = C[w.kx] - w.Sx * C[w.kz];
+    const float Cy = C[w.ky] - w.Sy * C[w.kz];
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        double CxBy = double(Cx) * double(By), CyBx = double(Cy) * double(Bx);
+        U = float(CxBy - CyBx);
+        double AxCy = double(Ax) * double(Cy), AyCx = double(Ay) * double(Cx);
+        V = float(AxCy - AyCx);
+        double BxAy = double(Bx) * double(Ay), ByAx = double(By) * double(Ax);
+        W = float(BxAy - ByAx);
+    }
+    if (cull) {
+        if (U < 0.0f || V < 0.0f || W < 0.0f) return false;
+    } else {
+        if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    }
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = w.Sz * A[w.kz];
+    const float Bz = w.Sz * B[w.kz];
+    const float Cz = w.Sz * C[w.kz];
+    const float T = U * Az + V * Bz + W * Cz;
+    if (det < 0.0f ? (T > 0.0f) : (T < 0.0f)) return false;
+    t = T / det;
+    b1 = V / det;
+    b2 = W / det;
+    return true;
+}
+
+bool IntersectBoxRobust(const WtRay& w, float tmin, float tmax, const float lo[3], const float hi[3], float& tnear) {
+    float tx0 = (lo[0] - w.ox) * w.idx, tx1 = (hi[0] - w.ox) * w.idx;
+    float ty0 = (lo[1] - w.oy) * w.idy, ty1 = (hi[1] - w.oy) * w.idy;
+    float tz0 = (lo[2] - w.oz) * w.idz, tz1 = (hi[2] - w.oz) * w.idz;
+    float tn = std::fmax(std::fmax(std::fmin(tx0, tx1), std::fmin(ty0, ty1)), std::fmax(std::fmin(tz0, tz1), tmin));
+    float tf = std::fmin(std::fmin(std::fmax(tx0, tx1), std::fmax(ty0, ty1)), std::fmin(std::fmax(tz0, tz1), tmax));
+    tnear = tn;
+    return tn <= tf * 1.0000004f;
+}
+
+static inline bool Better(float t, int32_t prim, float best_t, int32_t best_prim) {
+    return t < best_t || (t == best_t && best_prim >= 0 && prim < best_prim);
+}
+
+static void TraceFinalOne(const FinalTree& T, const RayRec& r, uint32_t flags, HitRec& h) {
+    const bool any = flags & 1u, cull = flags & 2u;
+    WtRay w = PrepareRay(r);
+    h = HitRec{0.0f, 0.0f, 0.0f, -1};
+    float best_t = r.tmax;
+    int32_t stack[256];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        int32_t ref = stack[--sp];
+        if (ref >= 0) {
+            const FinalTree::Node& nd = T.nodes[ref];
+            const float llo[3] = {nd.lx0, nd.ly0, nd.lz0}, lhi[3] = {nd.lx1, nd.ly1, nd.lz1};
+            const float rlo[3] = {nd.rx0, nd.ry0, nd.rz0}, rhi[3] = {nd.rx1, nd.ry1, nd.rz1};
+            float tl, tr;
+            bool hl = IntersectBoxRobust(w, r.tmin, best_t, llo, lhi, tl);
+            bool hr = IntersectBoxRobust(w, r.tmin, best_t, rlo, rhi, tr);
+            if (hl && hr) {
+                if (tr < tl) {
+                    stack[sp++] = nd.left;
+                    stack[sp++] = nd.right;
+                } else {
+                    stack[sp++] = nd.right;
+                    stack[sp++] = nd.left;
+                }
+            } else if (hl) {
+                stack[sp++] = nd.left;
+            } else if (hr) {
+                stack[sp++] = nd.right;
+            }
+            if (sp > 250) std::abort();
+        } else {
+            int32_t packed = ~ref;
+            int32_t begin = packed >> 4, cnt = packed & 15;
+            for (int32_t k = begin; k < begin + cnt; k++) {
+                const float* d = &T.tri_ordered[12 * size_t(k)];
+                int32_t prim;
+                std::memcpy(&prim, &d[3], 4);
+                float t, b1, b2;
+                if (!IntersectWatertight(w, d, d + 4, d + 8, cull, t, b1, b2)) continue;
+                if (!(t > r.tmin)) continue;
+                if (any) {
+                    if (t < r.tmax) {
+                        h = HitRec{t, b1, b2, prim};
+                        return;
+                    }
+                    continue;
+                }
+                if (Better(t, prim, best_t, h.prim)) {
+                    best_t = t;
+                    h = HitRec{t, b1, b2, prim};
+                }
+            }
+        }
+    }
+}
+
+template <typename F>
+static void ParallelFor(int64_t n, int threads, F f) {
+    if (threads <= 1 || n < 1024) {
+        for (int64_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; t++) {
+        th.emplace_back([=, &f] {
+            for (int64_t i = n * t / threads; i < n * (t + 1) / threads; i++) f(i);
+        });
+    }
+    for (auto& x : th) x.join();
+}
+
+void TraceFinal(const FinalTree& T, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads) {
+    ParallelFor(n, threads, [&](int64_t i) { TraceFinalOne(T, rays[i], flags, hits[i]); });
+}
+
+void TraceBrute(const float* tri_xyz, int64_t ntri, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits,
+                int threads) {
+    const bool any = flags & 1u, cull = flags & 2u;
+    std::vector<float> canon(9 * ntri);
+    for (int64_t i = 0; i < 9 * ntri; i++) canon[i] = Canon(tri_xyz[i]);
+    ParallelFor(n, threads, [&](int64_t i) {
+        const RayRec& r = rays[i];
+        WtRay w = PrepareRay(r);
+        HitRec h{0.0f, 0.0f, 0.0f, -1};
+        float best_t = r.tmax;
+        for (int64_t k = 0; k < ntri; k++) {
+            const float* v = &canon[9 * k];
+            float t, b1, b2;
+            if (!IntersectWatertight(w, v, v + 3, v + 6, cull, t, b1, b2)) continue;
+            if (!(t > r.tmin)) continue;
+            if (any) {
+                if (t < r.tmax) {
+                    h = HitRec{t, b1, b2, int32_t(k)};
+                    break;
+                }
+                continue;
+            }
+            if (Better(t, int32_t(k), best_t, h.prim)) {
+                best_t = t;
+                h = HitRec{t, b1, b2, int32_t(k)};
+            }
+        }
+        hits[i] = h;
+    });
+}
+
+// Exact-as-possible distance to the triangle's plane along the ray, in FP64, with the
+// barycentrics of that point (Cramer's rule in double).
+double IntersectFp64(const float* v, const RayRec& r, double& b1, double& b2) {
+    double o[3] = {r.ox, r.oy, r.oz}, d[3] = {r.dx, r.dy, r.dz};
+    double a[3] = {v[0], v[1], v[2]}, b[3] = {v[3], v[4], v[5]}, c[3] = {v[6], v[7], v[8]};
+    double e1[3], e2[3], s[3];
+    for (int k = 0; k < 3; k++) {
+        e1[k] = b[k] - a[k];
+        e2[k] = c[k] - a[k];
+        s[k] = o[k] - a[k];
+    }
+    auto cross = [](const double* x, const double* y, double* z) {
+        z[0] = x[1] * y[2] - x[2] * y[1];
+        z[1] = x[2] * y[0] - x[0] * y[2];
+        z[2] = x[0] * y[1] - x[1] * y[0];
+    };
+    auto dot = [](const double* x, const double* y) { return x[0] * y[0] + x[1] * y[1] + x[2] * y[2]; };
+    double p[3], q[3];
+    cross(d, e2, p);
+    double det = dot(e1, p);
+    if (det == 0.0) {
+        b1 = b2 = 0.0;
+        return HUGE_VAL;
+    }
+    cross(s, e1, q);
+    b1 = dot(s, p) / det;
+    b2 = dot(d, q) / det;
+    return dot(e2, q) / det;
+}
+
+}  // namespace orc

@@ -1,0 +1,71 @@
+"""The C-ABI library loads and exports every symbol include/dpl/dpl.h declares; host-side
+logic that needs no GPU (parameter bridge, error paths)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_symbols():
+    text = open(os.path.join(ROOT, "include", "dpl", "dpl.h")).read()
+    decl = r"^(?:dpl_status|void|const char\s*\*|int32_t)\s+(dpl_[a-z_0-9]+)\s*\("
+    return sorted(set(re.findall(decl, text, flags=re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    from diplodocus_amd import _lib
+
+    syms = header_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(_lib.lib, s), s
+    assert sorted(_lib.EXPORTED) == syms
+
+
+def test_params_bridge_mirrors_reference_switch():
+    # src/gpu/config/gpu_config_bridge.cc:27-53
+    from diplodocus_amd import accel
+
+    T = accel.AccelerationStructureType
+    expect = {T.kPloc: (0, 0), T.kPlocEmc: (1, 0), T.kPlocSobb: (0, 1), T.kPlocEmcSobb: (1, 1)}
+    for t, (m, b) in expect.items():
+        cfg = accel.AccelerationStructureConfig(t, nn_search_radius=8, max_triangles_per_leaf=3, kdop_size=26)
+        p = accel.bridge_acceleration_config(cfg)
+        assert (p.morton_type, p.bv_type, p.nn_search_radius, p.max_triangles_per_leaf, p.kdop_size) == (m, b, 8, 3, 26)
+    for t in (T.kDummy, T.kSbvh):
+        with pytest.raises(accel.DplError):
+            accel.bridge_acceleration_config(accel.AccelerationStructureConfig(t))
+    assert accel.AccelerationStructureType.parse("ploc_emc_sobb") == T.kPlocEmcSobb
+    with pytest.raises(RuntimeError):
+        accel.AccelerationStructureType.parse("bvh8")
+
+
+def test_null_and_invalid_arguments_without_device():
+    from diplodocus_amd import _lib
+
+    L = _lib.lib
+    assert L.dpl_build_params_from_type(2, 16, 4, 32, None) == _lib.DPL_ERR_INVALID_ARG
+    p = _lib.BuildParams()
+    assert L.dpl_build_params_from_type(99, 16, 4, 32, C.byref(p)) == _lib.DPL_ERR_INVALID_ARG
+    assert L.dpl_context_create(0, None) == _lib.DPL_ERR_INVALID_ARG
+    assert L.dpl_context_synchronize(None) == _lib.DPL_ERR_INVALID_ARG
+    assert L.dpl_bvh_get_stats(None, None) == _lib.DPL_ERR_INVALID_ARG
+    assert L.dpl_trace(None, None, None, 0, 0, None, 0) == _lib.DPL_ERR_INVALID_ARG
+    assert L.dpl_last_error(None) == b"null context"
+    L.dpl_context_destroy(None)
+    L.dpl_bvh_destroy(None)
+
+
+def test_context_create_reports_missing_device():
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    from diplodocus_amd import _lib
+
+    h = C.c_void_p()
+    assert _lib.lib.dpl_context_create(0, C.byref(h)) == _lib.DPL_ERR_DEVICE
+    assert not h.value

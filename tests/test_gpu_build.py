@@ -1,0 +1,165 @@
+"""GPU parity of the build: codes, stable order, PLOC merge list, iteration counts, collapse
+and final layout are bit-exact against the oracle; error paths; determinism."""
+import ctypes as C
+import sys
+
+import numpy as np
+import pytest
+
+from diplodocus_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def soup(n, seed, spread=10.0, size=0.2):
+    rng = np.random.default_rng(seed)
+    c = rng.uniform(0, spread, (n, 1, 3))
+    return (c + rng.normal(0, size, (n, 3, 3))).reshape(n, 9).astype(np.float32)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def build(ctx, tris, mt=0, radius=16, leaf=4, block=0, digit=0, keep=True):
+    from diplodocus_amd import accel
+
+    p = accel.make_params(morton_type=mt, radius=radius, max_leaf=leaf, block_size=block, sort_digit_bits=digit,
+                          keep_intermediates=keep)
+    return accel.GpuBvh(ctx).build(np.ascontiguousarray(tris, np.float32), p)
+
+
+def compare_to_oracle(oracle, ctx, tris, mt, radius, leaf, block=0, digit=0):
+    say(f"case n={len(tris)} mt={mt} r={radius} leaf={leaf} block={block} digit={digit}")
+    g = build(ctx, tris, mt, radius, leaf, block, digit)
+    d = g.download(intermediates=True)
+    o = oracle.Ploc(tris, mt, radius, leaf)
+    assert (d["codes"] == o.codes).all(), "codes"
+    assert (d["sorted_idx"] == o.sorted_idx).all(), "order"
+    assert (d["iter_counts"] == o.iter_counts).all(), "iteration counts"
+    pn = d["ploc_nodes"]
+    assert (pn["lo"].view(np.uint32) == o.nodes["lo"].view(np.uint32)).all(), "node boxes"
+    assert (pn["hi"].view(np.uint32) == o.nodes["hi"].view(np.uint32)).all(), "node boxes"
+    assert (pn["left"] == o.nodes["left"]).all() and (pn["right"] == o.nodes["right"]).all(), "merge list"
+    assert len(d["nodes"]) == len(o.fnodes)
+    assert (d["nodes"].view(np.uint32) == o.fnodes.view(np.uint32)).all(), "final nodes"
+    assert (d["prim_order"] == o.prim_order).all(), "prim order"
+    assert (d["tri_ordered"].view(np.uint32) == o.tri_ordered.view(np.uint32)).all(), "ordered triangles"
+    assert g.stats.ploc_iterations == len(o.iter_counts)
+    assert g.stats.sah_cost == pytest.approx(o.sah(), rel=1e-9)
+    g.close()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 64, 2047, 2048, 2049, 5000])
+def test_build_bitexact_small(oracle, ctx, n):
+    compare_to_oracle(oracle, ctx, soup(n, n), 0, 16, 4)
+
+
+@pytest.mark.parametrize("mt,radius,leaf,block,digit", [(0, 1, 1, 256, 8), (1, 8, 4, 128, 4), (0, 32, 15, 512, 8),
+                                                        (1, 16, 4, 256, 8), (0, 16, 4, 128, 4)])
+def test_build_bitexact_params(oracle, ctx, mt, radius, leaf, block, digit):
+    compare_to_oracle(oracle, ctx, soup(60000, radius * 7 + leaf, size=0.05), mt, radius, leaf, block, digit)
+
+
+@pytest.mark.parametrize("mt", [0, 1])
+def test_build_bitexact_workload_scenes(oracle, ctx, mt):
+    compare_to_oracle(oracle, ctx, scenes.cornell_room().tris, mt, 16, 4)
+    compare_to_oracle(oracle, ctx, scenes.icosphere_room().tris, mt, 16, 4)
+
+
+def test_build_degenerate_inputs(oracle, ctx):
+    t = np.repeat(soup(1, 3), 5000, 0)  # identical triangles: all distances tie
+    compare_to_oracle(oracle, ctx, t, 0, 16, 4)
+    flat = soup(6000, 4)
+    flat[:, 2::3] = 1.5  # zero z extent
+    compare_to_oracle(oracle, ctx, flat, 1, 8, 4)
+    neg0 = soup(3000, 5)
+    neg0[::7, 0] = -0.0
+    compare_to_oracle(oracle, ctx, neg0, 0, 8, 4)
+
+
+def test_codes_and_sort_entry_points(oracle, ctx):
+    import torch
+
+    tris = scenes.icosphere_room().tris
+    td = torch.from_numpy(tris).cuda()
+    for mt in (0, 1):
+        say(f"codes mt={mt}")
+        codes = torch.zeros(len(tris), dtype=torch.int64, device="cuda")
+        b = ctx.compute_codes(td, mt, codes)
+        ctx.synchronize()
+        raw = codes.cpu().numpy()
+        got = raw.view(np.uint32)[: len(tris)].astype(np.uint64) if mt == 0 else raw.view(np.uint64)
+        assert (got == oracle.codes(tris, mt)).all()
+        assert (b.view(np.uint32) == oracle.bounds(tris).view(np.uint32)).all()
+    rng = np.random.default_rng(8)
+    for n, bits, digit in [(1, 30, 8), (4095, 30, 8), (4096, 30, 4), (4097, 60, 8), (1_000_003, 60, 8),
+                           (2_000_000, 30, 8), (300_001, 60, 4)]:
+        say(f"sort n={n} bits={bits} digit={digit}")
+        k = rng.integers(0, 1 << min(bits, 20), n, dtype=np.uint64) << np.uint64(max(0, bits - 20))
+        if bits <= 32:
+            keys = torch.from_numpy(k.astype(np.uint32).view(np.int32)).cuda()
+        else:
+            keys = torch.from_numpy(k.view(np.int64)).cuda()
+        vals = torch.arange(n, dtype=torch.int32, device="cuda")
+        ctx.radix_sort_pairs(keys, vals, bits, digit)
+        ctx.synchronize()
+        order = np.argsort(k, kind="stable")
+        assert (vals.cpu().numpy().astype(np.int64) == order).all()
+        got = keys.cpu().numpy()
+        got = got.view(np.uint32).astype(np.uint64) if bits <= 32 else got.view(np.uint64)
+        assert (got == k[order]).all()
+
+
+def test_build_is_deterministic(ctx):
+    tris = soup(300000, 11, size=0.02)
+    d1 = build(ctx, tris, 1, 16, 4).download()
+    d2 = build(ctx, tris, 1, 16, 4).download()
+    for k in d1:
+        assert (d1[k].view(np.uint8) == d2[k].view(np.uint8)).all(), k
+
+
+def test_device_input_matches_host_input(ctx):
+    import torch
+
+    tris = soup(50000, 12)
+    a = build(ctx, tris, 0, 16, 4).download()
+    b = build(ctx, torch.from_numpy(tris).cuda(), 0, 16, 4).download()
+    for k in a:
+        assert (a[k].view(np.uint8) == b[k].view(np.uint8)).all(), k
+
+
+def test_build_error_paths(ctx):
+    from diplodocus_amd import accel
+
+    tris = soup(100, 1)
+    bad = tris.copy()
+    bad[7, 4] = np.nan
+    with pytest.raises(accel.DplError) as e:
+        build(ctx, bad)
+    assert e.value.status == 1
+    for kw in [dict(radius=0), dict(radius=33), dict(leaf=0), dict(leaf=16), dict(block=100), dict(digit=5)]:
+        with pytest.raises(accel.DplError) as e:
+            build(ctx, tris, **kw)
+        assert e.value.status == 1
+    p = accel.make_params(bv_type=1)
+    with pytest.raises(accel.DplError) as e:
+        accel.GpuBvh(ctx).build(tris, p)
+    assert e.value.status == 4
+    with pytest.raises(accel.DplError):
+        accel.GpuBvh(ctx).build(np.zeros((0, 9), np.float32), accel.make_params())
+    # the context still works after errors
+    build(ctx, tris)
+
+
+def test_rebuild_in_place(oracle, ctx):
+    from diplodocus_amd import accel
+
+    g = accel.GpuBvh(ctx)
+    for n in (20000, 5000, 30000):
+        tris = soup(n, n)
+        g.build(tris, accel.make_params(keep_intermediates=True))
+        d = g.download()
+        o = oracle.Ploc(tris, 0, 16, 4)
+        assert (d["nodes"].view(np.uint32) == o.fnodes.view(np.uint32)).all()

@@ -1,0 +1,134 @@
+"""Pins the oracle's geometry restatement (reference: src/scene/triangle.{h,cc}, ray.h,
+util.h, aabb.h) with closed forms, hand-derived known answers and FP64 checks."""
+import numpy as np
+import pytest
+
+TRI = np.array([0, 0, 0, 1, 0, 0, 0, 1, 0], np.float32)  # normal +z
+
+
+def ray(o, d, tmin=1e-6, tmax=1e30):
+    return np.array([*o, tmin, *d, tmax], np.float32)
+
+
+def test_hash_u32_is_a_bijection_and_fixes_zero(oracle):
+    # lowbias32-style avalanche hash (util.h:24-32): every step is invertible on u32, so the
+    # hash is a bijection; 0 -> 0 is the closed form of every xorshift/multiply step.
+    assert oracle.hash_u32(0) == 0
+    xs = np.arange(1 << 16, dtype=np.uint64)
+    hs = {oracle.hash_u32(int(x)) for x in xs}
+    assert len(hs) == len(xs)
+    # avalanche: flipping one input bit flips about half of the output bits
+    flips = [bin(oracle.hash_u32(x) ^ oracle.hash_u32(x ^ (1 << b))).count("1") for x in range(1, 400, 7)
+             for b in range(32)]
+    assert 14.0 < np.mean(flips) < 18.0
+
+
+def test_u01_from_u32_uses_top_24_bits(oracle):
+    lib = oracle.lib
+    assert lib.orc_u01(0) == 0.0
+    assert lib.orc_u01(0xFFFFFFFF) == np.float32(16777215 / 16777216)
+    assert lib.orc_u01(0x80000000) == 0.5
+    assert lib.orc_u01(0xFF) == 0.0  # low 8 bits are dropped
+
+
+def test_centroid_and_area_closed_form(oracle):
+    c = oracle.centroid(np.array([0, 0, 0, 3, 0, 0, 0, 6, 0], np.float32))
+    assert list(c) == [1.0, 2.0, 0.0]
+    assert oracle.lib.orc_triangle_area(oracle._p(oracle._f32(np.array([0, 0, 0, 3, 0, 0, 0, 4, 0])))) == 6.0
+
+
+def test_ray_epsilon_closed_form(oracle):
+    f = oracle.lib.orc_ray_epsilon
+    assert f(0, 0, 0, 0) == np.float32(1e-6)  # max(1e-6*1, 1e-7*1)
+    assert f(5.0, -3.0, 2.0, 1.0) == np.float32(5e-6)  # p scale = 5
+    assert f(0, 0, 0, 100.0) == np.float32(1e-5)  # t bias 1e-7*100 wins
+    assert f(1e5, 0, 0, 0) == np.float32(1e-3)  # clamped to 1e-3
+
+
+def test_moller_trumbore_known_answers(oracle):
+    t, b1, b2 = oracle.intersect_mt(TRI, ray((0.25, 0.5, 1.0), (0, 0, -1)))
+    assert (t, b1, b2) == (1.0, 0.25, 0.5)
+    # parallel ray: |det| = 0 < eps
+    t, _, _ = oracle.intersect_mt(TRI, ray((0.25, 0.25, 1.0), (1, 0, 0)))
+    assert t == np.finfo(np.float32).max
+    # back face: culled only with backface culling (triangle.cc:18-24)
+    r = ray((0.25, 0.25, -1.0), (0, 0, 1))
+    assert oracle.intersect_mt(TRI, r, cull=False)[0] == 1.0
+    assert oracle.intersect_mt(TRI, r, cull=True)[0] == np.finfo(np.float32).max
+    # outside the edge b1 + b2 > 1
+    assert oracle.intersect_mt(TRI, ray((0.75, 0.75, 1.0), (0, 0, -1)))[0] == np.finfo(np.float32).max
+
+
+def test_moller_trumbore_absolute_det_threshold_rejects_small_triangles(oracle):
+    # |det| = |e1 x e2 . d| = 1e-8 < 1e-6: the reference misses this hit (triangle.cc:22)
+    small = (TRI * np.float32(1e-4)).astype(np.float32)
+    t, _, _ = oracle.intersect_mt(small, ray((2.5e-5, 2.5e-5, 1.0), (0, 0, -1)))
+    assert t == np.finfo(np.float32).max
+    ok, t, _, _ = oracle.intersect_watertight(small, ray((2.5e-5, 2.5e-5, 1.0), (0, 0, -1)))
+    assert ok and t == 1.0
+
+
+def test_watertight_known_answers_and_culling(oracle):
+    ok, t, b1, b2 = oracle.intersect_watertight(TRI, ray((0.25, 0.5, 1.0), (0, 0, -1)))
+    assert ok and (t, b1, b2) == (1.0, 0.25, 0.5)
+    r = ray((0.25, 0.25, -1.0), (0, 0, 1))
+    assert oracle.intersect_watertight(TRI, r, cull=False)[0]
+    assert not oracle.intersect_watertight(TRI, r, cull=True)[0]
+    # the culling convention agrees with the reference's (front face: det_MT > 0)
+    r = ray((0.25, 0.25, 1.0), (0, 0, -1))
+    assert oracle.intersect_watertight(TRI, r, cull=True)[0]
+    assert oracle.intersect_mt(TRI, r, cull=True)[0] == 1.0
+
+
+def test_watertight_against_fp64(oracle):
+    rng = np.random.default_rng(1)
+    bad = 0
+    hits = 0
+    for _ in range(3000):
+        tri = rng.uniform(-1, 1, 9).astype(np.float32)
+        o = rng.uniform(-3, 3, 3)
+        tgt = tri.reshape(3, 3).mean(0) + rng.normal(0, 0.3, 3)
+        d = (tgt - o) / np.linalg.norm(tgt - o)
+        r = ray(o, d)
+        ok, t, b1, b2 = oracle.intersect_watertight(tri, r)
+        t64, c1, c2 = oracle.intersect_fp64(tri, r)
+        inside = c1 >= 0 and c2 >= 0 and c1 + c2 <= 1 and t64 > 0
+        margin = min(c1, c2, 1 - c1 - c2)
+        if abs(margin) < 1e-5:
+            continue  # grazing: either answer is acceptable
+        if ok != inside:
+            bad += 1
+        elif ok:
+            hits += 1
+            assert abs(t - t64) <= 1e-5 * abs(t64)
+            assert abs(b1 - c1) < 1e-4 and abs(b2 - c2) < 1e-4
+    assert bad == 0 and hits > 300
+
+
+def test_watertight_shared_edge_is_never_missed(oracle):
+    # A quad split along its diagonal: every ray through a point of the diagonal hits at
+    # least one of the two triangles (the watertight property).
+    a = np.array([0, 0, 0, 1, 0, 0, 1, 1, 0], np.float32)
+    b = np.array([0, 0, 0, 1, 1, 0, 0, 1, 0], np.float32)
+    rng = np.random.default_rng(2)
+    for s in rng.uniform(0.01, 0.99, 500).astype(np.float32):
+        p = np.array([s, s, 0.0])
+        o = p + np.concatenate([rng.uniform(-0.5, 0.5, 2), [2.0]])
+        d = (p - o) / np.linalg.norm(p - o)
+        r = ray(o, d)
+        assert oracle.intersect_watertight(a, r)[0] or oracle.intersect_watertight(b, r)[0]
+
+
+def test_box_test_is_conservative(oracle):
+    rng = np.random.default_rng(3)
+    lib = oracle.lib
+    import ctypes as C
+    for _ in range(2000):
+        lo = rng.uniform(-1, 0, 3).astype(np.float32)
+        hi = (lo + rng.uniform(0, 1, 3)).astype(np.float32)
+        p = rng.uniform(lo, hi).astype(np.float64)  # a point inside the box
+        o = rng.uniform(-4, 4, 3)
+        d = (p - o) / np.linalg.norm(p - o)
+        r = ray(o, d)
+        tn = C.c_float()
+        assert lib.orc_intersect_box(oracle._p(r), oracle._p(lo), oracle._p(hi), np.float32(1e30), C.byref(tn))
