@@ -1,8 +1,11 @@
-"""Benchmark: BVH build Mtris/s and trace Mrays/s of the 2M-triangle mixed scene on one
-MI355X, with the SAH cost ratio against the reference's CPU SAH BVH and the CPU baseline.
+"""Benchmark: BVH build Mtris/s and trace Mrays/s on one MI355X, with the SAH cost ratio
+against the reference's CPU SAH BVH and the CPU baseline timed on the same host.
 
-A step = one PLOC build of the 2,000,002-triangle scene (device-resident input) followed by
-one closest-hit trace of 4,194,304 rays. Prints one JSON line.
+A step = one PLOC build of the 2,000,002-triangle mixed scene (device-resident input)
+followed by one closest-hit trace of its 4,194,304 rays. `value` is the build rate
+(Mtris/s). Beside it: primary-ray trace rate of the 1024x1024 camera of the 69,668-triangle
+icosphere room, traversal counters, the roofline of the PLOC iteration kernel (the build's
+dominant kernel) and the CPU baseline. Prints one JSON line.
 """
 import argparse
 import json
@@ -14,6 +17,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+HBM_PEAK_GBS = 8000.0
+
 
 def cpu_model() -> str:
     try:
@@ -23,6 +28,10 @@ def cpu_model() -> str:
     except OSError:
         pass
     return platform.processor()
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
 def main() -> int:
@@ -47,8 +56,7 @@ def main() -> int:
     t0 = time.time()
     scene = scenes.mixed_2m()
     rays_np = scenes.random_rays_towards(scene.tris, 4_194_304, seed=11)
-    print(f"[bench] scene {scene.n} tris, {len(rays_np)} rays generated in {time.time() - t0:.1f}s", file=sys.stderr,
-          flush=True)
+    log(f"scene {scene.n} tris, {len(rays_np)} rays generated in {time.time() - t0:.1f}s")
     dev = torch.device("cuda:0")
     tris = torch.from_numpy(scene.tris).to(dev)
     rays = torch.from_numpy(rays_np).to(dev)
@@ -85,36 +93,70 @@ def main() -> int:
     trace_ms = np.array([e[1].elapsed_time(e[2]) for e in ev])
     n = scene.n
     nrays = len(rays_np)
-    build_dev_ms = np.array([s["build_ms"] for s in stats])
     mtris = n / (build_ms.mean() * 1e-3) / 1e6
     mrays = nrays / (trace_ms.mean() * 1e-3) / 1e6
     last = stats[-1]
     h = hits.cpu().numpy().view(np.int32)[:, 3]
-    print(f"[bench] build {build_ms.mean():.3f} ms (device {build_dev_ms.mean():.3f}), trace {trace_ms.mean():.3f} ms,"
-          f" hit rate {(h >= 0).mean():.3f}, stages {last['stage_ms']}", file=sys.stderr, flush=True)
+    log(f"build {build_ms.mean():.3f} ms (device {np.mean([s['build_ms'] for s in stats]):.3f}), trace "
+        f"{trace_ms.mean():.3f} ms, hit rate {(h >= 0).mean():.3f}, iterations {last['ploc_iterations']}, "
+        f"stages {last['stage_ms']}")
 
-    # ---- roofline of the PLOC stage: algorithmic bytes per build over its measured time ----
-    lay = bvh.download(intermediates=False)
-    iters = last["ploc_iterations"]
-    ploc_ms = np.mean([s["stage_ms"]["ploc"] for s in stats])
-    # per merged-cluster bytes: each iteration reads its clusters (32 B) and writes the
-    # survivors (32 B) plus one 32 B node and two 4 B parents per merge; iterations remove
-    # n-1 clusters in total, so sum(c_k) is measured from the build's own iteration counts.
+    # ---- traversal counters of the same trace (outside the timed region) ----
+    _, visits, tests = bvh.trace_counters(rays)
+    touched = (visits * 64 + tests * 48) / nrays
+
+    # ---- roofline of the PLOC iteration kernel: one profiled build (per-launch events) ----
+    prof = accel.GpuBvh(ctx).build(tris, accel.make_params(morton_type=mt, radius=args.radius, max_leaf=4,
+                                                             keep_intermediates=3))
+    ps = prof.stats
     roof = None
+    if ps.ploc_launches > 0 and ps.ploc_launch_ms > 0:
+        per_launch_bytes = ps.ploc_bytes / ps.ploc_launches
+        per_launch_ms = ps.ploc_launch_ms / ps.ploc_launches
+        achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
+        roof = {"kernel": "KPlocIteration<256> (one PLOC iteration: NN search, merge, compaction)",
+                "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "launches": ps.ploc_launches, "avg_launch_us": round(per_launch_ms * 1e3, 3),
+                "algorithmic_bytes_per_launch": round(per_launch_bytes)}
+        log(f"ploc iteration kernel: {ps.ploc_launches} launches, {per_launch_ms * 1e3:.2f} us avg, "
+            f"{achieved:.1f} GB/s algorithmic")
+    prof.close()
+
+    # ---- primary rays: 1024x1024 camera of the icosphere room (PLOC, Morton30, r=16) ----
+    room = scenes.icosphere_room()
+    cam_rays = torch.from_numpy(scenes.camera_rays(room.camera, 1024, 1024)).to(dev)
+    cam_hits = torch.empty((cam_rays.shape[0], 4), dtype=torch.float32, device=dev)
+    rb = accel.GpuBvh(ctx).build(torch.from_numpy(room.tris).to(dev), accel.make_params(0, 16, 4))
+    for _ in range(3):
+        rb.trace(cam_rays, 0, cam_hits)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    reps = 20
+    e0.record(stream)
+    for _ in range(reps):
+        rb.trace(cam_rays, 0, cam_hits)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    prim_ms = e0.elapsed_time(e1) / reps
+    prim_mrays = cam_rays.shape[0] / (prim_ms * 1e-3) / 1e6
+    _, pv, pt = rb.trace_counters(cam_rays)
+    log(f"primary rays: {prim_ms:.3f} ms per 1M rays = {prim_mrays:.1f} Mrays/s, {pv / cam_rays.shape[0]:.1f} "
+        f"nodes and {pt / cam_rays.shape[0]:.1f} tris per ray")
+    rb.close()
 
     # ---- CPU baseline: the reference's SAH BVH build (1 core) and tracer on a sample ----
     cpu = None
     sah_ratio = None
+    sah_ratio_fixed = None
     if not args.no_cpu:
         from oracle import pyoracle
 
-        tb = time.perf_counter()
         sb = pyoracle.SBvh(scene.tris, fix_child_bounds=False)
-        sb_ms = (time.perf_counter() - tb) * 1e3
         sah_cpu = sb.sah()
         sah_fixed = pyoracle.SBvh(scene.tris, fix_child_bounds=True).sah()
         sah_ratio = last["sah_cost"] / sah_cpu
-        sample = rays_np[:262144]
+        sah_ratio_fixed = last["sah_cost"] / sah_fixed
+        sample = rays_np[:131072]
         threads = pyoracle.nthreads()
         sb.trace(sample, flags=0, mode=0, threads=threads)
         cpu_trace_s = sb.last_trace_s
@@ -123,17 +165,17 @@ def main() -> int:
             "unit": "Mtris/s",
             "cores": 1,
             "kind": "port",
-            "sample": (f"full 2,000,002-tri SAH build, single thread ({sb.build_ms:.0f} ms); "
-                       f"trace of {len(sample)} of the rays on {threads} threads: "
-                       f"{len(sample) / cpu_trace_s / 1e6:.3f} Mrays/s"),
-            "trace_mrays_s": round(len(sample) / cpu_trace_s / 1e6, 4),
+            "sample": (f"full 2,000,002-tri SAH build on 1 thread ({sb.build_ms:.0f} ms, restated "
+                       f"src/acceleration/sbvh.cc); trace of the first {len(sample)} rays with the reference "
+                       f"rules on {threads} threads: {len(sample) / cpu_trace_s / 1e6:.4f} Mrays/s"),
+            "trace_mrays_s": round(len(sample) / cpu_trace_s / 1e6, 5),
             "trace_threads": threads,
             "cpu_model": cpu_model(),
+            "host_cores": os.cpu_count(),
             "sah_faithful": sah_cpu,
             "sah_fixed": sah_fixed,
         }
-        print(f"[bench] cpu build {sb_ms:.0f} ms, sah cpu {sah_cpu:.3f} fixed {sah_fixed:.3f} gpu "
-              f"{last['sah_cost']:.3f}", file=sys.stderr, flush=True)
+        log(f"cpu build {sb.build_ms:.0f} ms, sah cpu {sah_cpu:.3f} (fixed {sah_fixed:.3f}) gpu {last['sah_cost']:.3f}")
 
     out = {
         "metric": "build Mtris/s + trace Mrays/s at SAH <=1.10x CPU; fwd+adjoint vs fwd time",
@@ -148,15 +190,22 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": "mixed_2m: 2,000,002 tris (64 Gaussian clusters, 200k thin, 2 ground), 4,194,304 rays",
+        "config": {"workload": "mixed_2m: 2,000,002 tris (64 Gaussian clusters, 200k thin 50:1, 2 ground), "
+                               "4,194,304 rays from a bounding sphere toward the centre",
                    "variant": args.variant, "radius": args.radius, "max_leaf": 4},
         "trace_mrays_s": round(mrays, 3),
+        "primary_mrays_s": round(prim_mrays, 3),
+        "primary_workload": "icosphere_room 69,668 tris, 1024x1024 camera rays, PLOC Morton30 r=16",
         "build_ms": round(float(build_ms.mean()), 4),
         "trace_ms": round(float(trace_ms.mean()), 4),
         "sah_cost": last["sah_cost"],
         "sah_ratio_vs_cpu": sah_ratio,
-        "ploc_iterations": iters,
+        "sah_ratio_vs_cpu_fixed": sah_ratio_fixed,
+        "ploc_iterations": last["ploc_iterations"],
         "stage_ms": last["stage_ms"],
+        "trace_nodes_per_ray": round(visits / nrays, 2),
+        "trace_tris_per_ray": round(tests / nrays, 2),
+        "trace_touched_bytes_per_ray": round(touched, 1),
         "wall_s": round(wall, 3),
         "roofline": roof,
         "cpu_baseline": cpu,

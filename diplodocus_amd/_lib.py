@@ -28,10 +28,11 @@ class BuildStats(C.Structure):
     _fields_ = [("build_ms", C.c_double), ("upload_ms", C.c_double), ("stage_ms", C.c_double * 8),
                 ("node_count", C.c_int32), ("inner_node_count", C.c_int32), ("leaf_node_count", C.c_int32),
                 ("ploc_iterations", C.c_int32), ("memory_consumption", C.c_int64), ("sah_cost", C.c_double),
-                ("tree_height", C.c_int32), ("reserved", C.c_int32)]
+                ("tree_height", C.c_int32), ("ploc_launches", C.c_int32), ("ploc_launch_ms", C.c_double),
+                ("ploc_bytes", C.c_double)]
 
     def as_dict(self) -> dict:
-        d = {k: getattr(self, k) for k, _ in self._fields_ if k not in ("stage_ms", "reserved")}
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "stage_ms"}
         d["stage_ms"] = dict(zip(["bounds", "codes", "sort", "ploc", "collapse", "layout", "sobb", "stats"],
                                  list(self.stage_ms)))
         return d
@@ -58,6 +59,14 @@ def _load() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise ImportError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                           "(make -C diplodocus_amd/csrc); there is no fallback path")
+    # One HIP runtime per process: torch ships its own libamdhip64 (soname libamdhip64.so.7,
+    # but its libtorch_hip asks for the unversioned name). Loading torch first makes libdpl's
+    # libamdhip64.so.7 dependency resolve to the already-loaded copy; loading libdpl first
+    # would pull in a second runtime and torch would then see no GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = C.CDLL(LIB_PATH)
     P, I32, I64, U32, U64 = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_uint64
     sig = {
@@ -77,6 +86,7 @@ def _load() -> C.CDLL:
         "dpl_bvh_get_layout": (I32, [P, C.POINTER(BvhLayout)]),
         "dpl_bvh_download": (I32, [P, P, P, P, P, P, P, P]),
         "dpl_trace": (I32, [P, P, P, I64, U32, P, I32]),
+        "dpl_trace_counters": (I32, [P, P, P, I64, U32, P, I32, P, P]),
         "dpl_trace_batched": (I32, [P, P, P, P, I64, U32, P, I32]),
         "dpl_compute_codes": (I32, [P, P, I64, I32, P, P]),
         "dpl_radix_sort_pairs": (I32, [P, P, P, I64, I32, I32]),
@@ -96,7 +106,8 @@ lib = _load()
 EXPORTED = [
     "dpl_build_params_from_type", "dpl_context_create", "dpl_context_destroy", "dpl_last_error",
     "dpl_context_set_stream", "dpl_context_synchronize", "dpl_context_cu_count", "dpl_bvh_build", "dpl_bvh_rebuild",
-    "dpl_bvh_build_batched", "dpl_bvh_destroy", "dpl_bvh_get_stats", "dpl_trace", "dpl_trace_batched",
+    "dpl_bvh_build_batched", "dpl_bvh_destroy", "dpl_bvh_get_stats", "dpl_trace", "dpl_trace_counters",
+    "dpl_trace_batched",
     "dpl_bvh_get_layout", "dpl_bvh_download", "dpl_compute_codes", "dpl_radix_sort_pairs", "dpl_render_forward",
     "dpl_render_adjoint",
 ]

@@ -73,9 +73,11 @@ def bridge_acceleration_config(cfg: AccelerationStructureConfig, block_size: int
 
 
 def make_params(morton_type: int = 0, radius: int = 16, max_leaf: int = 4, bv_type: int = 0, block_size: int = 0,
-                sort_digit_bits: int = 0, keep_intermediates: bool = False, kdop_size: int = 32) -> BuildParams:
+                sort_digit_bits: int = 0, keep_intermediates: int = 0, kdop_size: int = 32) -> BuildParams:
+    """keep_intermediates: bit 0 keeps codes/order/merge list, bit 1 times every PLOC
+    iteration launch (True = 1)."""
     return BuildParams(morton_type, bv_type, radius, max_leaf, kdop_size, block_size, sort_digit_bits,
-                       1 if keep_intermediates else 0)
+                       int(keep_intermediates))
 
 
 def _is_torch(a) -> bool:
@@ -210,6 +212,21 @@ class GpuBvh:
             raise ValueError("rays and hits must live in the same memory")
         self.ctx.check(lib.dpl_trace(self.ctx.h, self.h, rp, n, flags, hp, mem))
         return hits
+
+    def trace_counters(self, rays, flags: int = 0):
+        """Trace plus (inner nodes visited, triangle tests) over the batch."""
+        rp, mem = _ptr(rays)
+        n = int(rays.shape[0])
+        if mem == _lib.DPL_MEM_DEVICE:
+            import torch
+
+            hits = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+        else:
+            hits = np.zeros(n, HIT_DTYPE)
+        hp, _ = _ptr(hits)
+        nv, nt = C.c_int64(), C.c_int64()
+        self.ctx.check(lib.dpl_trace_counters(self.ctx.h, self.h, rp, n, flags, hp, mem, C.byref(nv), C.byref(nt)))
+        return hits, int(nv.value), int(nt.value)
 
     def intersect(self, rays, backface_culling: bool = False):
         return self.trace(rays, _lib.DPL_TRACE_CULL_BACK if backface_culling else 0)

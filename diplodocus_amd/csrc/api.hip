@@ -16,10 +16,33 @@
 namespace dpl {
 
 hipError_t LaunchSah(const FNode* nodes, const int32_t* counters, double* out, hipStream_t s);
-hipError_t LaunchRender(bool adjoint, const FNode* nodes, const float4* tris, int tree_height, int64_t prim_count,
-                        const dpl_render_params& rp, const float* cam_basis, const int32_t* mat_id,
-                        const float* albedo, const float* emission, const float* d_image, float* image,
-                        float* grad_albedo, float* grad_emission, int cu_count, uint32_t* counter, hipStream_t s);
+
+// Camera basis of src/renderer/cpu_raytracer.cc:62-73 (pos, p00, qw, qh), evaluated once on
+// the host in FP32 with the same operation order.
+void CameraBasisHost(const float* cam, int w, int h, float* out) {
+    auto normalize = [](float x, float y, float z, float* o) {
+        const float len = std::sqrt(x * x + y * y + z * z);
+        if (len <= 1e-6f) {
+            o[0] = o[1] = o[2] = 0.0f;
+            return;
+        }
+        o[0] = x / len, o[1] = y / len, o[2] = z / len;
+    };
+    const float *dir = cam + 3, *up = cam + 6;
+    const float fov = cam[9];
+    float right[3];
+    normalize(dir[1] * up[2] - dir[2] * up[1], dir[2] * up[0] - dir[0] * up[2], dir[0] * up[1] - dir[1] * up[0], right);
+    const float gw = static_cast<float>(2.0 * std::tan(fov / 2.0f));
+    const float gh = gw * (static_cast<float>(h) / static_cast<float>(w));
+    const float sw = gw / static_cast<float>(w - 1), sh = gh / static_cast<float>(h - 1);
+    const float hw = gw / 2, hh = gh / 2;
+    for (int a = 0; a < 3; a++) {
+        out[a] = cam[a];
+        out[3 + a] = dir[a] - right[a] * hw + up[a] * hh;
+        out[6 + a] = right[a] * sw;
+        out[9 + a] = -up[a] * sh;
+    }
+}
 
 struct Error : std::runtime_error {
     dpl_status status;
@@ -80,10 +103,20 @@ struct dpl_context {
     std::string last_error;
     // scratch
     DevBuf tri, keys, vals, keys_alt, vals_alt, nodes, parent, clusters0, clusters1, ccount, status0, status1,
-        small, count, cost, collapsed, minleaf, height, visit, leafslot, keptflag, scan_tmp, sort_tmp, rays, hits,
-        trace_counter, render_tmp;
-    void* pinned = nullptr;  // host staging for small results
+        small, count, cost, collapsed, minleaf, height, leafslot, keptflag, scan_tmp, sort_tmp, rays, hits,
+        trace_counter, spill;
+    void* pinned = nullptr;          // host staging for small results (device-visible)
+    int32_t* pinned_dev = nullptr;   // device address of `pinned`
     hipEvent_t ev[10] = {};
+    std::vector<hipEvent_t> iter_ev;  // per-launch timing of PLOC iterations (on request)
+    hipEvent_t IterEvent(size_t k) {
+        while (iter_ev.size() <= k) {
+            hipEvent_t e;
+            DPL_HIP(hipEventCreate(&e));
+            iter_ev.push_back(e);
+        }
+        return iter_ev[k];
+    }
 };
 
 struct dpl_bvh {
@@ -158,15 +191,6 @@ float ElapsedMs(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-struct PinnedResults {
-    int32_t counters[8];
-    int32_t iter_done;
-    int32_t root_height;
-    uint32_t bounds[13];
-    int32_t ccount_probe;
-    double sah;
-    PNode root;
-};
 
 void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, dpl_mem mem,
                const dpl_build_params& p, dpl_build_stats* out_stats) {
@@ -212,7 +236,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     b.collapsed = C.collapsed.ensure<uint8_t>(size_t(nn));
     b.minleaf = C.minleaf.ensure<int32_t>(size_t(nn) * 4);
     b.height = C.height.ensure<int32_t>(size_t(nn) * 4);
-    b.visit = C.visit.ensure<uint32_t>(size_t(n) * 4);
+    b.results = reinterpret_cast<BuildResults*>(C.pinned_dev);
     b.leafslot = C.leafslot.ensure<int32_t>(size_t(n) * 4);
     b.keptflag = C.keptflag.ensure<int32_t>(size_t(n) * 4);
     b.scan_tmp = C.scan_tmp.ensure<int32_t>(ScanTempInts(n) * 4);
@@ -253,59 +277,83 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     DPL_HIP(hipEventRecord(C.ev[3], s));
     DPL_HIP(LaunchInitClusters(b, s));
 
-    // ---- PLOC iterations: multi-workgroup kernels while clusters exceed the tail size ----
-    PinnedResults* pr = static_cast<PinnedResults*>(C.pinned);
+    // ---- PLOC iterations: multi-workgroup kernels while clusters exceed the tail size.
+    // The host launches a batch of iterations sized from the observed reduction rate, then
+    // reads the cluster count the last finished iteration wrote to host-mapped memory;
+    // iterations launched after the count fell below the tail size pass the clusters through.
+    volatile BuildResults* pr = static_cast<volatile BuildResults*>(C.pinned);
+    pr->progress = int32_t(n);
+    pr->flags = 0u;
+    pr->multi_iters = 0;
+    pr->tail_iters = 0;
     const int tail = PlocTail();
     int iter = 0;
     int64_t c_known = n;
+    double rate = 1.15;  // clusters shrink by about this factor per iteration
+    const bool time_iters = (p.keep_intermediates & 2) != 0;
     const int cap = std::max(1, C.cu_count * (p.block_size == 512 ? 2 : (p.block_size == 128 ? 8 : 4)));
     while (c_known > tail) {
-        int est = int(std::ceil(std::log(double(c_known) / tail) / std::log(1.0 / 0.8)));
-        est = std::max(1, std::min(est, 64));
+        int est = int(std::ceil(std::log(double(c_known) / tail) / std::log(rate)));
+        est = std::max(1, std::min(est, 96));
+        const int64_t T = (c_known + chunk - 1) / chunk;
         for (int g = 0; g < est; g++) {
-            const int64_t T = (c_known + chunk - 1) / chunk;
-            DPL_HIP(LaunchPlocIteration(b, iter, p.nn_search_radius, p.block_size, tail,
+            if (time_iters) DPL_HIP(hipEventRecord(C.IterEvent(2 * size_t(iter)), s));
+            DPL_HIP(LaunchPlocIteration(b, iter, p.nn_search_radius, p.block_size, tail, p.max_triangles_per_leaf,
                                         int(std::min<int64_t>(T, cap)), s));
+            if (time_iters) DPL_HIP(hipEventRecord(C.IterEvent(2 * size_t(iter) + 1), s));
             iter++;
         }
-        DPL_HIP(hipMemcpyAsync(&pr->ccount_probe, b.ccount + iter, 4, hipMemcpyDeviceToHost, s));
-        DPL_HIP(hipMemcpyAsync(&pr->bounds[12], b.bounds + 12, 4, hipMemcpyDeviceToHost, s));
         DPL_HIP(hipStreamSynchronize(s));
-        if (pr->bounds[12] & 1u) throw Error(DPL_ERR_INVALID_ARG, "triangle coordinates must be finite");
-        if (pr->ccount_probe >= c_known) throw Error(DPL_ERR_INTERNAL, "PLOC made no progress");
-        c_known = pr->ccount_probe;
+        if (pr->flags & 1u) throw Error(DPL_ERR_INVALID_ARG, "triangle coordinates must be finite");
+        const int64_t c_now = pr->progress;
+        if (c_now >= c_known) throw Error(DPL_ERR_INTERNAL, "PLOC made no progress");
+        rate = std::max(1.02, std::min(2.0, std::pow(double(c_known) / double(c_now), 1.0 / est)));
+        c_known = c_now;
     }
-    DPL_HIP(LaunchPlocTail(b, iter, p.nn_search_radius, tail, s));
+    DPL_HIP(LaunchPlocTail(b, iter, p.nn_search_radius, p.max_triangles_per_leaf, s));
     DPL_HIP(hipEventRecord(C.ev[4], s));
-    DPL_HIP(LaunchCollapse(b, p.max_triangles_per_leaf, s));
     DPL_HIP(hipEventRecord(C.ev[5], s));
     DPL_HIP(LaunchLayout(b, p.max_triangles_per_leaf, s));
     DPL_HIP(hipEventRecord(C.ev[6], s));
     DPL_HIP(LaunchSah(b.fnodes, b.counters, b.sah, s));
+    DPL_HIP(LaunchFinish(b, s));
     DPL_HIP(hipEventRecord(C.ev[7], s));
     if (p.keep_intermediates) {
         DPL_HIP(hipMemcpyAsync(bvh->sorted_idx.p, b.vals, size_t(n) * 4, hipMemcpyDeviceToDevice, s));
         DPL_HIP(hipMemcpyAsync(bvh->pnodes.p, b.nodes, size_t(nn) * sizeof(PNode), hipMemcpyDeviceToDevice, s));
     }
-    DPL_HIP(hipMemcpyAsync(pr->counters, b.counters, sizeof(pr->counters), hipMemcpyDeviceToHost, s));
-    DPL_HIP(hipMemcpyAsync(&pr->iter_done, b.iter_done, 4, hipMemcpyDeviceToHost, s));
-    DPL_HIP(hipMemcpyAsync(&pr->root_height, b.height + (nn - 1), 4, hipMemcpyDeviceToHost, s));
-    DPL_HIP(hipMemcpyAsync(pr->bounds, b.bounds, sizeof(pr->bounds), hipMemcpyDeviceToHost, s));
-    DPL_HIP(hipMemcpyAsync(&pr->sah, b.sah, 8, hipMemcpyDeviceToHost, s));
-    DPL_HIP(hipMemcpyAsync(&pr->root, b.nodes + (nn - 1), sizeof(PNode), hipMemcpyDeviceToHost, s));
     DPL_HIP(hipStreamSynchronize(s));
-    if (pr->bounds[12] & 1u) throw Error(DPL_ERR_INVALID_ARG, "triangle coordinates must be finite");
-    if (pr->bounds[12] & 2u) throw Error(DPL_ERR_INTERNAL, "PLOC iteration without a merge");
+    if (pr->flags & 1u) throw Error(DPL_ERR_INVALID_ARG, "triangle coordinates must be finite");
+    if (pr->flags & 2u) throw Error(DPL_ERR_INTERNAL, "PLOC iteration without a merge");
 
-    const int32_t K = pr->counters[0];
-    const int32_t leaves = pr->counters[1];
-    const int total_iters = pr->iter_done;
-    std::vector<int32_t> cc(size_t(total_iters) + 1);
-    DPL_HIP(hipMemcpy(cc.data(), b.ccount, cc.size() * 4, hipMemcpyDeviceToHost));
+    const int32_t K = pr->kept;
+    const int32_t leaves = pr->leaves;
+    const int32_t root_height = pr->root_height;
+    const int32_t real_iters = pr->multi_iters + pr->tail_iters;
+    const double sah_sum = pr->sah;
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; a++) lo[a] = pr->root_lo[a], hi[a] = pr->root_hi[a];
     std::vector<int32_t> iters;
-    for (int k = 0; k < total_iters; k++)
-        if (iters.empty() || cc[k] != iters.back()) iters.push_back(cc[k]);
-    if (!iters.empty() && iters.back() == 1) iters.pop_back();
+    const int32_t multi = pr->multi_iters;
+    if (p.keep_intermediates) {
+        // clusters at the start of every iteration (pass-through launches repeat a count)
+        std::vector<int32_t> cc(size_t(pr->iter_done) + 1);
+        DPL_HIP(hipMemcpy(cc.data(), b.ccount, cc.size() * 4, hipMemcpyDeviceToHost));
+        for (int k = 0; k < pr->iter_done; k++)
+            if (iters.empty() || cc[k] != iters.back()) iters.push_back(cc[k]);
+        if (!iters.empty() && iters.back() == 1) iters.pop_back();
+        if (time_iters) {
+            // the first `multi` launches merged; per launch: read c_k clusters (32 B), write
+            // c_k+1 survivors (32 B), per merge a node (32 B), two parents (8 B), its
+            // collapse record (17 B) and the two children's records (34 B)
+            for (int k = 0; k < multi; k++) {
+                st.ploc_launch_ms += ElapsedMs(C.iter_ev[2 * k], C.iter_ev[2 * k + 1]);
+                const double ck = cc[k], ck1 = cc[k + 1];
+                st.ploc_bytes += 32.0 * ck + 32.0 * ck1 + 91.0 * (ck - ck1);
+            }
+        }
+    }
+    st.ploc_launches = multi;
 
     st.build_ms = ElapsedMs(C.ev[0], C.ev[7]);
     st.upload_ms = mem == DPL_MEM_HOST ? ElapsedMs(C.ev[8], C.ev[9]) : 0.0;
@@ -321,22 +369,20 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     st.inner_node_count = fcount;
     st.leaf_node_count = leaves;
     st.node_count = fcount + leaves;
-    st.ploc_iterations = int32_t(iters.size());
+    st.ploc_iterations = real_iters;
     st.memory_consumption = int64_t(fcount) * 64 + n * 48 + n * 4;
-    const float* lo = pr->root.lo;
-    const float* hi = pr->root.hi;
     const double sx = double(hi[0]) - lo[0], sy = double(hi[1]) - lo[1], sz = double(hi[2]) - lo[2];
     const double root_sa = 2.0 * (sx * sy + sy * sz + sx * sz);
     if (K > 0) {
-        st.sah_cost = root_sa > 0 ? (root_sa + pr->sah) / root_sa : 0.0;
+        st.sah_cost = root_sa > 0 ? (root_sa + sah_sum) / root_sa : 0.0;
     } else {
         st.sah_cost = double(n);  // a single leaf: Ci * SA * n / SA
     }
-    st.tree_height = pr->root_height;
+    st.tree_height = root_height;
 
     bvh->prim_count = n;
     bvh->node_count = fcount;
-    bvh->tree_height = pr->root_height;
+    bvh->tree_height = root_height;
     bvh->params = p;
     bvh->stats = st;
     bvh->iter_counts = iters;
@@ -391,7 +437,8 @@ dpl_status dpl_context_create(int32_t device, dpl_context** out) {
         DPL_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
         ctx->stream = ctx->own_stream;
         for (auto& e : ctx->ev) DPL_HIP(hipEventCreate(&e));
-        DPL_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+        DPL_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocMapped));
+        DPL_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->pinned_dev), ctx->pinned, 0));
         ctx->trace_counter.ensure(256);
     });
     if (st != DPL_OK) return st;
@@ -512,8 +559,8 @@ dpl_status dpl_bvh_download(const dpl_bvh* bvh, void* nodes, void* tri_ordered, 
     });
 }
 
-dpl_status dpl_trace(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
-                     dpl_hit* hits, dpl_mem mem) {
+static dpl_status TraceImpl(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
+                            dpl_hit* hits, dpl_mem mem, int64_t* counts_out) {
     if (!ctx || !bvh) return DPL_ERR_INVALID_ARG;
     return Guard(ctx, [&] {
         Require(n >= 0, "n must be non-negative");
@@ -521,6 +568,7 @@ dpl_status dpl_trace(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, 
         Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK)) == 0u, "unknown trace flags");
         Require(mem == DPL_MEM_HOST || mem == DPL_MEM_DEVICE, "mem must be DPL_MEM_HOST or DPL_MEM_DEVICE");
         Require(bvh->ctx == ctx, "bvh belongs to another context");
+        if (counts_out) counts_out[0] = counts_out[1] = 0;
         if (n == 0) return;
         hipStream_t s = ctx->stream;
         const dpl_ray* dr = rays;
@@ -532,7 +580,16 @@ dpl_status dpl_trace(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, 
         }
         TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
                        int32_t(bvh->prim_count), bvh->tree_height};
-        hipError_t e = LaunchTrace(tree, dr, n, flags, dh, ctx->cu_count, ctx->trace_counter.as<uint32_t>(), s);
+        int32_t* spill = ctx->spill.ensure<int32_t>(TraceSpillBytes(ctx->cu_count, bvh->tree_height));
+        uint8_t* tc = ctx->trace_counter.as<uint8_t>();
+        unsigned long long* counters = nullptr;
+        if (counts_out) counters = reinterpret_cast<unsigned long long*>(tc + 64);  // zeroed by the launch
+        hipError_t e = LaunchTrace(tree, dr, n, flags, dh, ctx->cu_count, reinterpret_cast<uint32_t*>(tc), spill,
+                                   counters, s);
+        if (counts_out) {
+            DPL_HIP(hipMemcpyAsync(counts_out, counters, 16, hipMemcpyDeviceToHost, s));
+            DPL_HIP(hipStreamSynchronize(s));
+        }
         if (e == hipErrorNotSupported) throw Error(DPL_ERR_UNSUPPORTED, "tree height exceeds the traversal stack");
         DPL_HIP(e);
         if (mem == DPL_MEM_HOST) {
@@ -540,6 +597,21 @@ dpl_status dpl_trace(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, 
             DPL_HIP(hipStreamSynchronize(s));
         }
     });
+}
+
+dpl_status dpl_trace(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
+                     dpl_hit* hits, dpl_mem mem) {
+    return TraceImpl(ctx, bvh, rays, n, flags, hits, mem, nullptr);
+}
+
+dpl_status dpl_trace_counters(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
+                              dpl_hit* hits, dpl_mem mem, int64_t* node_visits, int64_t* triangle_tests) {
+    if (!node_visits || !triangle_tests) return ctx ? Fail(ctx, DPL_ERR_INVALID_ARG, "null counter") : DPL_ERR_INVALID_ARG;
+    int64_t c[2] = {0, 0};
+    const dpl_status st = TraceImpl(ctx, bvh, rays, n, flags, hits, mem, c);
+    *node_visits = c[0];
+    *triangle_tests = c[1];
+    return st;
 }
 
 dpl_status dpl_trace_batched(dpl_context* ctx, const dpl_bvh*, const dpl_ray*, const int32_t*, int64_t, uint32_t,
@@ -569,7 +641,6 @@ dpl_status dpl_compute_codes(dpl_context* ctx, const float* tri, int64_t n, int3
         b.ccount = ctx->ccount.ensure<int32_t>(size_t(n + 2) * 4);
         b.parent = ctx->parent.ensure<int32_t>(size_t(2 * n - 1) * 4);
         b.status[0] = ctx->status0.ensure<uint64_t>(size_t(n / 256 + 2) * 8);
-        b.visit = ctx->visit.ensure<uint32_t>(size_t(n) * 4);
         b.leafslot = ctx->leafslot.ensure<int32_t>(size_t(n) * 4);
         DPL_HIP(LaunchBuildInit(b, 256, s));
         DPL_HIP(LaunchBounds(b, s));
@@ -610,14 +681,42 @@ dpl_status dpl_radix_sort_pairs(dpl_context* ctx, uint64_t* keys, uint32_t* valu
     });
 }
 
-dpl_status dpl_render_forward(dpl_context* ctx, const dpl_bvh*, const dpl_render_params*, const int32_t*, const float*,
-                              const float*, float*) {
-    return Fail(ctx, DPL_ERR_UNSUPPORTED, "rendering is not available in this build");
+static dpl_status Render(bool adjoint, dpl_context* ctx, const dpl_bvh* bvh, const dpl_render_params* rp,
+                         const int32_t* mat_id, const float* albedo, const float* emission, const float* d_image,
+                         float* image, float* grad_albedo, float* grad_emission) {
+    if (!ctx || !bvh || !rp) return ctx ? Fail(ctx, DPL_ERR_INVALID_ARG, "null argument") : DPL_ERR_INVALID_ARG;
+    return Guard(ctx, [&] {
+        Require(bvh->ctx == ctx, "bvh belongs to another context");
+        Require(rp->width >= 2 && rp->height >= 2, "width and height must be at least 2");
+        Require(rp->spp >= 1, "spp must be at least 1");
+        Require(rp->max_depth >= 1 && rp->max_depth <= (adjoint ? 16 : 64), "max_depth out of range");
+        Require(rp->material_count >= 1 && rp->material_count <= 256, "material_count must be in [1, 256]");
+        Require(int64_t(rp->width) * rp->height * rp->spp < (int64_t(1) << 40), "too many paths");
+        Require(mat_id && albedo && emission, "material arrays must be non-null");
+        if (adjoint) Require(d_image && grad_albedo && grad_emission, "adjoint buffers must be non-null");
+        else Require(image != nullptr, "image must be non-null");
+        float basis[12];
+        CameraBasisHost(rp->camera, rp->width, rp->height, basis);
+        TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
+                       int32_t(bvh->prim_count), bvh->tree_height};
+        int32_t* spill = ctx->spill.ensure<int32_t>(TraceSpillBytes(ctx->cu_count, bvh->tree_height));
+        hipError_t e = LaunchRender(adjoint, tree, *rp, basis, mat_id, albedo, emission, d_image, image, grad_albedo,
+                                    grad_emission, ctx->cu_count, ctx->trace_counter.as<uint32_t>(), spill,
+                                    ctx->stream);
+        if (e == hipErrorNotSupported) throw Error(DPL_ERR_UNSUPPORTED, "tree height exceeds the traversal stack");
+        DPL_HIP(e);
+    });
 }
 
-dpl_status dpl_render_adjoint(dpl_context* ctx, const dpl_bvh*, const dpl_render_params*, const int32_t*,
-                              const float*, const float*, const float*, float*, float*) {
-    return Fail(ctx, DPL_ERR_UNSUPPORTED, "rendering is not available in this build");
+dpl_status dpl_render_forward(dpl_context* ctx, const dpl_bvh* bvh, const dpl_render_params* rp, const int32_t* mat_id,
+                              const float* albedo, const float* emission, float* image) {
+    return Render(false, ctx, bvh, rp, mat_id, albedo, emission, nullptr, image, nullptr, nullptr);
+}
+
+dpl_status dpl_render_adjoint(dpl_context* ctx, const dpl_bvh* bvh, const dpl_render_params* rp,
+                              const int32_t* mat_id, const float* albedo, const float* emission,
+                              const float* d_image, float* grad_albedo, float* grad_emission) {
+    return Render(true, ctx, bvh, rp, mat_id, albedo, emission, d_image, nullptr, grad_albedo, grad_emission);
 }
 
 }  // extern "C"

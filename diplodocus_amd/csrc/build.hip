@@ -91,10 +91,7 @@ __global__ void KBuildInit(BuildBuffers b, int tiles0) {
     }
     if (t < 8) b.counters[t] = 0;
     for (int64_t k = t; k < tiles0; k += int64_t(gridDim.x) * blockDim.x) b.status[0][k] = 0ull;
-    for (int64_t k = t; k < b.n; k += int64_t(gridDim.x) * blockDim.x) {
-        b.visit[k] = 0u;
-        b.leafslot[k] = 0;
-    }
+    for (int64_t k = t; k < b.n; k += int64_t(gridDim.x) * blockDim.x) b.leafslot[k] = 0;
 }
 
 hipError_t LaunchBuildInit(const BuildBuffers& b, int chunk, hipStream_t s) {
@@ -286,28 +283,76 @@ hipError_t LaunchCodes(const BuildBuffers& b, int morton_type, hipStream_t s) {
 // ---------------------------------------------------------------------------------------
 // leaves in sorted order: node k = triangle vals[k]; cluster k = (box, node k)
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void KInitClusters(const float* __restrict__ tri, int64_t n,
-                                                      const uint32_t* __restrict__ order, PNode* nodes,
-                                                      PNode* clusters) {
+// SAH collapse data of a node (Ct = Ci = 1): it becomes a leaf when it holds at most
+// max_leaf triangles and SA * count <= SA + cost(left) + cost(right). Children are always
+// created before their parent (leaves first, then PLOC merges in iteration order), so the
+// merge that creates a node evaluates it.
+struct NodeEval {
+    int32_t count;
+    float cost;
+    int32_t minleaf;
+    int32_t height;
+    uint8_t collapsed;
+};
+
+__device__ __forceinline__ NodeEval LeafEval(const float lo[3], const float hi[3], int32_t k) {
+    return NodeEval{1, SurfaceAreaBox(lo, hi) * 1.0f, k, 0, 1};
+}
+
+__device__ __forceinline__ NodeEval InnerEval(const float lo[3], const float hi[3], const NodeEval& l,
+                                              const NodeEval& r, int max_leaf) {
+    NodeEval e;
+    e.count = l.count + r.count;
+    const float sa = SurfaceAreaBox(lo, hi);
+    const float inner = sa + l.cost + r.cost;
+    e.cost = inner;
+    e.collapsed = 0;
+    if (e.count <= max_leaf) {
+        const float leafc = sa * float(e.count);
+        if (leafc <= inner) {
+            e.cost = leafc;
+            e.collapsed = 1;
+        }
+    }
+    e.minleaf = l.minleaf < r.minleaf ? l.minleaf : r.minleaf;
+    e.height = 1 + (l.height > r.height ? l.height : r.height);
+    return e;
+}
+
+__device__ __forceinline__ NodeEval LoadEval(const BuildBuffers& b, int32_t id) {
+    return NodeEval{b.count[id], b.cost[id], b.minleaf[id], b.height[id], b.collapsed[id]};
+}
+
+__device__ __forceinline__ void StoreEval(const BuildBuffers& b, int32_t id, const NodeEval& e) {
+    b.count[id] = e.count;
+    b.cost[id] = e.cost;
+    b.minleaf[id] = e.minleaf;
+    b.height[id] = e.height;
+    b.collapsed[id] = e.collapsed;
+}
+
+__global__ __launch_bounds__(256) void KInitClusters(BuildBuffers b) {
+    const int64_t n = b.n;
     for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x) {
-        uint32_t orig = order[k];
+        const uint32_t orig = b.vals[k];
         float v[9];
-        LoadTri(tri, orig, v);
+        LoadTri(b.tri, orig, v);
         PNode nd;
         TriBox(v, nd.lo, nd.hi);
         nd.left = int32_t(orig);
         nd.right = -1;
-        nodes[k] = nd;
+        b.nodes[k] = nd;
+        StoreEval(b, int32_t(k), LeafEval(nd.lo, nd.hi, int32_t(k)));
         nd.left = int32_t(k);
         nd.right = 0;
-        clusters[k] = nd;
+        b.clusters[0][k] = nd;
     }
 }
 
 hipError_t LaunchInitClusters(const BuildBuffers& b, hipStream_t s) {
     int grid = int(std::min<int64_t>(4096, (b.n + 255) / 256));
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(KInitClusters, dim3(grid), dim3(256), 0, s, b.tri, b.n, b.vals, b.nodes, b.clusters[0]);
+    hipLaunchKernelGGL(KInitClusters, dim3(grid), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 
@@ -365,7 +410,7 @@ constexpr uint64_t kFlagInc = 2ull << 62;
 constexpr uint64_t kMask31 = 0x7FFFFFFFull;
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter, int R, int tail) {
+__global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter, int R, int tail, int max_leaf) {
     constexpr int CHUNK = BLOCK * kIpt;
     constexpr int WMAX = CHUNK + 4 * kMaxRadius;
     __shared__ float s_lx[WMAX], s_ly[WMAX], s_lz[WMAX], s_hx[WMAX], s_hy[WMAX], s_hz[WMAX];
@@ -386,6 +431,7 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
             if (threadIdx.x == 0) {
                 b.ccount[iter + 1] = c;
                 b.tile_counter[(iter + 1) & 1] = 0u;
+                b.results->progress = c;
             }
         }
         return;
@@ -469,7 +515,13 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
                                    __HIP_MEMORY_SCOPE_AGENT);
             }
             s_prefix = excl;
-            if (tile == T - 1) b.ccount[iter + 1] = int32_t(((excl >> 31) & kMask31) + aggS);
+            if (tile == T - 1) {
+                const int32_t next = int32_t(((excl >> 31) & kMask31) + aggS);
+                b.ccount[iter + 1] = next;
+                b.results->progress = next;
+                b.results->multi_iters = iter + 1;
+                b.results->flags = b.bounds[12];
+            }
         }
         __syncthreads();
         const uint64_t excl = s_prefix;
@@ -495,6 +547,7 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
                 b.nodes[id] = o;
                 b.parent[o.left] = id;
                 b.parent[o.right] = id;
+                StoreEval(b, id, InnerEval(o.lo, o.hi, LoadEval(b, o.left), LoadEval(b, o.right), max_leaf));
                 o.left = id;
                 o.right = 0;
             } else {
@@ -512,25 +565,34 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
 int PlocChunk(int block) { return block * kIpt; }
 int PlocTail() { return kTail; }
 
-hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int tail, int grid,
+hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int tail, int max_leaf, int grid,
                                hipStream_t s) {
     if (grid < 1) grid = 1;
     switch (block) {
-        case 128: hipLaunchKernelGGL(KPlocIteration<128>, dim3(grid), dim3(128), 0, s, b, iter, radius, tail); break;
-        case 512: hipLaunchKernelGGL(KPlocIteration<512>, dim3(grid), dim3(512), 0, s, b, iter, radius, tail); break;
-        default: hipLaunchKernelGGL(KPlocIteration<256>, dim3(grid), dim3(256), 0, s, b, iter, radius, tail); break;
+        case 128:
+            hipLaunchKernelGGL(KPlocIteration<128>, dim3(grid), dim3(128), 0, s, b, iter, radius, tail, max_leaf);
+            break;
+        case 512:
+            hipLaunchKernelGGL(KPlocIteration<512>, dim3(grid), dim3(512), 0, s, b, iter, radius, tail, max_leaf);
+            break;
+        default:
+            hipLaunchKernelGGL(KPlocIteration<256>, dim3(grid), dim3(256), 0, s, b, iter, radius, tail, max_leaf);
+            break;
     }
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------------------------------------
-// tail: one workgroup runs every remaining iteration with the clusters resident in LDS
+// tail: one workgroup runs every remaining iteration with the clusters (and their
+// collapse data) resident in LDS
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(kTailThreads) void KPlocTail(BuildBuffers b, int iter, int R) {
+__global__ __launch_bounds__(kTailThreads) void KPlocTail(BuildBuffers b, int iter, int R, int max_leaf) {
     constexpr int IPT = (kTail + kTailThreads - 1) / kTailThreads;
     __shared__ float s_lx[kTail], s_ly[kTail], s_lz[kTail], s_hx[kTail], s_hy[kTail], s_hz[kTail];
     __shared__ int32_t s_id[kTail];
     __shared__ int32_t s_nn[kTail];
+    __shared__ int32_t s_cnt[kTail], s_ml[kTail], s_ht[kTail];
+    __shared__ float s_cost[kTail];
     __shared__ uint32_t s_scan[kTailThreads / kWave + 1];
     const int64_t n = b.n;
     int c = b.ccount[iter];
@@ -540,6 +602,8 @@ __global__ __launch_bounds__(kTailThreads) void KPlocTail(BuildBuffers b, int it
         float4 q0 = p[0], q1 = p[1];
         s_lx[k] = q0.x, s_ly[k] = q0.y, s_lz[k] = q0.z, s_hx[k] = q0.w;
         s_hy[k] = q1.x, s_hz[k] = q1.y, s_id[k] = __float_as_int(q1.z);
+        const NodeEval e = LoadEval(b, s_id[k]);
+        s_cnt[k] = e.count, s_cost[k] = e.cost, s_ml[k] = e.minleaf, s_ht[k] = e.height;
     }
     __syncthreads();
     WindowLds w{s_lx, s_ly, s_lz, s_hx, s_hy, s_hz};
@@ -551,6 +615,7 @@ __global__ __launch_bounds__(kTailThreads) void KPlocTail(BuildBuffers b, int it
         uint32_t sflags = 0, mflags = 0;
         int32_t nnj[IPT];
         PNode keep[IPT];
+        NodeEval keep_e[IPT];
 #pragma unroll
         for (int q = 0; q < IPT; q++) {
             const int i = threadIdx.x * IPT + q;
@@ -579,6 +644,7 @@ __global__ __launch_bounds__(kTailThreads) void KPlocTail(BuildBuffers b, int it
             const int i = threadIdx.x * IPT + q;
             if (!(sflags & (1u << q))) continue;
             PNode& o = keep[q];
+            const NodeEval ei{s_cnt[i], s_cost[i], s_ml[i], s_ht[i], 0};
             if (mflags & (1u << q)) {
                 const int j = nnj[q];
                 o.lo[0] = Min2(s_lx[i], s_lx[j]);
@@ -593,11 +659,15 @@ __global__ __launch_bounds__(kTailThreads) void KPlocTail(BuildBuffers b, int it
                 b.nodes[id] = o;
                 b.parent[o.left] = id;
                 b.parent[o.right] = id;
+                const NodeEval ej{s_cnt[j], s_cost[j], s_ml[j], s_ht[j], 0};
+                keep_e[q] = InnerEval(o.lo, o.hi, ei, ej, max_leaf);
+                StoreEval(b, id, keep_e[q]);
                 o.left = id;
             } else {
                 o.lo[0] = s_lx[i], o.lo[1] = s_ly[i], o.lo[2] = s_lz[i];
                 o.hi[0] = s_hx[i], o.hi[1] = s_hy[i], o.hi[2] = s_hz[i];
                 o.left = s_id[i];
+                keep_e[q] = ei;
             }
         }
         __syncthreads();
@@ -609,6 +679,8 @@ __global__ __launch_bounds__(kTailThreads) void KPlocTail(BuildBuffers b, int it
             s_lx[k] = o.lo[0], s_ly[k] = o.lo[1], s_lz[k] = o.lo[2];
             s_hx[k] = o.hi[0], s_hy[k] = o.hi[1], s_hz[k] = o.hi[2];
             s_id[k] = o.left;
+            s_cnt[k] = keep_e[q].count, s_cost[k] = keep_e[q].cost, s_ml[k] = keep_e[q].minleaf;
+            s_ht[k] = keep_e[q].height;
         }
         c = int(total & 0xFFFFu);
         it++;
@@ -617,62 +689,13 @@ __global__ __launch_bounds__(kTailThreads) void KPlocTail(BuildBuffers b, int it
     if (threadIdx.x == 0) {
         b.ccount[it] = c;
         b.iter_done[0] = it;
+        b.results->iter_done = it;
+        b.results->tail_iters = it - iter;
     }
 }
 
-hipError_t LaunchPlocTail(const BuildBuffers& b, int iter, int radius, int tail, hipStream_t s) {
-    (void)tail;
-    hipLaunchKernelGGL(KPlocTail, dim3(1), dim3(kTailThreads), 0, s, b, iter, radius);
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------------------
-// bottom-up SAH collapse (Ct = Ci = 1): the second child to arrive at a node evaluates it
-// ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void KCollapse(BuildBuffers b, int max_leaf) {
-    const int64_t n = b.n;
-    for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x) {
-        const PNode lf = b.nodes[k];
-        b.count[k] = 1;
-        b.cost[k] = SurfaceAreaBox(lf.lo, lf.hi) * 1.0f;
-        b.collapsed[k] = 1;
-        b.minleaf[k] = int32_t(k);
-        b.height[k] = 0;
-        int32_t x = b.parent[k];
-        while (x >= 0) {
-            // release our subtree's results, acquire the sibling's
-            uint32_t old = __hip_atomic_fetch_add(&b.visit[x - n], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            if (old == 0) break;
-            const PNode nd = b.nodes[x];
-            const int32_t l = nd.left, r = nd.right;
-            const int32_t cnt = b.count[l] + b.count[r];
-            const float sa = SurfaceAreaBox(nd.lo, nd.hi);
-            const float inner = sa + b.cost[l] + b.cost[r];
-            float cost = inner;
-            uint8_t col = 0;
-            if (cnt <= max_leaf) {
-                const float leafc = sa * float(cnt);
-                if (leafc <= inner) {
-                    cost = leafc;
-                    col = 1;
-                }
-            }
-            b.count[x] = cnt;
-            b.cost[x] = cost;
-            b.collapsed[x] = col;
-            const int32_t ml = b.minleaf[l], mr = b.minleaf[r];
-            b.minleaf[x] = ml < mr ? ml : mr;
-            const int32_t hl = b.height[l], hr = b.height[r];
-            b.height[x] = 1 + (hl > hr ? hl : hr);
-            x = b.parent[x];
-        }
-    }
-}
-
-hipError_t LaunchCollapse(const BuildBuffers& b, int max_leaf, hipStream_t s) {
-    int grid = int(std::min<int64_t>(4096, (b.n + 255) / 256));
-    if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(KCollapse, dim3(grid), dim3(256), 0, s, b, max_leaf);
+hipError_t LaunchPlocTail(const BuildBuffers& b, int iter, int radius, int max_leaf, hipStream_t s) {
+    hipLaunchKernelGGL(KPlocTail, dim3(1), dim3(kTailThreads), 0, s, b, iter, radius, max_leaf);
     return hipGetLastError();
 }
 
@@ -824,6 +847,27 @@ hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(KEmit, dim3(grid), dim3(256), 0, s, b, max_leaf);
+    return hipGetLastError();
+}
+
+// Copies the build's small results into host-mapped memory (one thread, after the SAH sum).
+__global__ void KFinish(BuildBuffers b) {
+    BuildResults* r = b.results;
+    const int64_t root = 2 * b.n - 2;
+    r->kept = b.counters[0];
+    r->leaves = b.counters[1];
+    r->root_height = b.height[root];
+    r->sah = b.sah[0];
+    r->flags = b.bounds[12];
+    const PNode nd = b.nodes[root];
+    for (int a = 0; a < 3; a++) {
+        r->root_lo[a] = nd.lo[a];
+        r->root_hi[a] = nd.hi[a];
+    }
+}
+
+hipError_t LaunchFinish(const BuildBuffers& b, hipStream_t s) {
+    hipLaunchKernelGGL(KFinish, dim3(1), dim3(1), 0, s, b);
     return hipGetLastError();
 }
 

@@ -136,6 +136,20 @@ __device__ __forceinline__ uint32_t BlockExclusiveScan(uint32_t v, uint32_t* scr
 // ---------------------------------------------------------------------------------------
 // launchers (host side); every one enqueues on `s` and returns the launch error
 // ---------------------------------------------------------------------------------------
+// Small build results the kernels write straight into host-mapped memory (no copies).
+struct BuildResults {
+    int32_t progress;     // clusters after the last finished multi-workgroup iteration
+    uint32_t flags;       // 1: non-finite input, 2: an iteration without a merge
+    int32_t multi_iters;  // multi-workgroup iterations that merged (not pass-through)
+    int32_t tail_iters;   // iterations of the single-workgroup tail
+    int32_t iter_done;    // index one past the last recorded cluster count
+    int32_t kept;         // final inner nodes (0: the whole scene is one leaf)
+    int32_t leaves;       // final leaves
+    int32_t root_height;  // longest root-to-leaf path of the PLOC tree
+    double sah;           // FP64 sum of the children's SAH terms over the final nodes
+    float root_lo[3], root_hi[3];
+};
+
 struct BuildBuffers {
     int64_t n;
     const float* tri;      // n x 9 (device)
@@ -156,7 +170,7 @@ struct BuildBuffers {
     uint8_t* collapsed;    // 2n-1
     int32_t* minleaf;      // 2n-1
     int32_t* height;       // 2n-1
-    uint32_t* visit;       // n (internal nodes)
+    BuildResults* results;   // host-mapped
     int32_t* leafslot;     // n: count of the final leaf whose minimum sorted position is k
     int32_t* keptflag;     // n: internal node kept as a final inner node
     int32_t* scan_tmp;     // scan block sums
@@ -171,13 +185,13 @@ hipError_t LaunchBuildInit(const BuildBuffers& b, int chunk, hipStream_t s);
 hipError_t LaunchBounds(const BuildBuffers& b, hipStream_t s);
 hipError_t LaunchCodes(const BuildBuffers& b, int morton_type, hipStream_t s);
 hipError_t LaunchInitClusters(const BuildBuffers& b, hipStream_t s);
-hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int tail, int grid,
-                               hipStream_t s);
-hipError_t LaunchPlocTail(const BuildBuffers& b, int iter, int radius, int tail, hipStream_t s);
+hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int tail, int max_leaf,
+                               int grid, hipStream_t s);
+hipError_t LaunchPlocTail(const BuildBuffers& b, int iter, int radius, int max_leaf, hipStream_t s);
 int PlocChunk(int block);
 int PlocTail();
-hipError_t LaunchCollapse(const BuildBuffers& b, int max_leaf, hipStream_t s);
 hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s);
+hipError_t LaunchFinish(const BuildBuffers& b, hipStream_t s);
 hipError_t LaunchExclusiveScan(const int32_t* in, int32_t* out, int64_t n, int32_t* tmp, int32_t* total,
                                hipStream_t s);
 size_t ScanTempInts(int64_t n);
@@ -196,7 +210,14 @@ struct TraceTree {
     int32_t height;
 };
 
+// counters (nullable, device): accumulates {inner nodes visited, triangle tests}.
 hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,
-                       int cu_count, uint32_t* counter, hipStream_t s);
+                       int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters, hipStream_t s);
+// Bytes of the per-lane stack spill region the trace and render kernels need for a tree.
+size_t TraceSpillBytes(int cu_count, int height);
+hipError_t LaunchRender(bool adjoint, const TraceTree& tree, const dpl_render_params& rp, const float* basis,
+                        const int32_t* mat_id, const float* albedo, const float* emission, const float* d_image,
+                        float* image, float* grad_albedo, float* grad_emission, int cu_count, uint32_t* counter,
+                        int32_t* spill, hipStream_t s);
 
 }  // namespace dpl

@@ -1,0 +1,200 @@
+// Device-side traversal shared by the trace and render kernels: per-ray set-up for the
+// watertight ray-triangle test (Woop, Benthin, Wald, JCGT 2013), the test itself, a
+// conservative slab test (Ize 2013: the far distance is scaled by 1 + 2*gamma(3)), and an
+// ordered depth-first traversal of the 64-byte two-child nodes with the stack in LDS.
+//
+// Closest-hit rule: smallest t with tmin < t < tmax, ties to the lower primitive id.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dpl_internal.h"
+
+namespace dpl {
+
+struct WtRay {
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    float ox, oy, oz;
+    float idx, idy, idz;
+    float tmin, tmax;
+};
+
+__device__ __forceinline__ float Sel3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
+
+__device__ __forceinline__ WtRay PrepareRay(float ox, float oy, float oz, float tmin, float dx, float dy, float dz,
+                                            float tmax) {
+    WtRay w;
+    const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+    const int kz = (ax >= ay && ax >= az) ? 0 : (ay >= az ? 1 : 2);
+    int kx = kz + 1 == 3 ? 0 : kz + 1;
+    int ky = kx + 1 == 3 ? 0 : kx + 1;
+    const float dkz = Sel3(dx, dy, dz, kz);
+    if (dkz < 0.0f) {
+        const int t = kx;
+        kx = ky;
+        ky = t;
+    }
+    w.kx = kx, w.ky = ky, w.kz = kz;
+    w.Sx = Sel3(dx, dy, dz, kx) / dkz;
+    w.Sy = Sel3(dx, dy, dz, ky) / dkz;
+    w.Sz = 1.0f / dkz;
+    w.ox = ox, w.oy = oy, w.oz = oz;
+    w.idx = 1.0f / dx;
+    w.idy = 1.0f / dy;
+    w.idz = 1.0f / dz;
+    w.tmin = tmin;
+    w.tmax = tmax;
+    return w;
+}
+
+__device__ __forceinline__ bool Watertight(const WtRay& w, const float4 p0, const float4 p1, const float4 p2,
+                                           bool cull, float& t, float& b1, float& b2) {
+    const float A0 = p0.x - w.ox, A1 = p0.y - w.oy, A2 = p0.z - w.oz;
+    const float B0 = p1.x - w.ox, B1 = p1.y - w.oy, B2 = p1.z - w.oz;
+    const float C0 = p2.x - w.ox, C1 = p2.y - w.oy, C2 = p2.z - w.oz;
+    const float Akz = Sel3(A0, A1, A2, w.kz), Bkz = Sel3(B0, B1, B2, w.kz), Ckz = Sel3(C0, C1, C2, w.kz);
+    const float Ax = Sel3(A0, A1, A2, w.kx) - w.Sx * Akz;
+    const float Ay = Sel3(A0, A1, A2, w.ky) - w.Sy * Akz;
+    const float Bx = Sel3(B0, B1, B2, w.kx) - w.Sx * Bkz;
+    const float By = Sel3(B0, B1, B2, w.ky) - w.Sy * Bkz;
+    const float Cx = Sel3(C0, C1, C2, w.kx) - w.Sx * Ckz;
+    const float Cy = Sel3(C0, C1, C2, w.ky) - w.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        const double CxBy = double(Cx) * double(By), CyBx = double(Cy) * double(Bx);
+        U = float(CxBy - CyBx);
+        const double AxCy = double(Ax) * double(Cy), AyCx = double(Ay) * double(Cx);
+        V = float(AxCy - AyCx);
+        const double BxAy = double(Bx) * double(Ay), ByAx = double(By) * double(Ax);
+        W = float(BxAy - ByAx);
+    }
+    if (cull) {
+        if (U < 0.0f || V < 0.0f || W < 0.0f) return false;
+    } else {
+        if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    }
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float Az = w.Sz * Akz, Bz = w.Sz * Bkz, Cz = w.Sz * Ckz;
+    const float T = U * Az + V * Bz + W * Cz;
+    if (det < 0.0f ? (T > 0.0f) : (T < 0.0f)) return false;
+    t = T / det;
+    b1 = V / det;
+    b2 = W / det;
+    return true;
+}
+
+__device__ __forceinline__ bool Box(const WtRay& w, float tfar_limit, float x0, float x1, float y0, float y1, float z0,
+                                    float z1, float& tnear) {
+    const float tx0 = (x0 - w.ox) * w.idx, tx1 = (x1 - w.ox) * w.idx;
+    const float ty0 = (y0 - w.oy) * w.idy, ty1 = (y1 - w.oy) * w.idy;
+    const float tz0 = (z0 - w.oz) * w.idz, tz1 = (z1 - w.oz) * w.idz;
+    const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), w.tmin));
+    const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tfar_limit));
+    tnear = tn;
+    return tn <= tf * 1.0000004f;
+}
+
+struct HitResult {
+    float t, b1, b2;
+    int32_t prim;  // input triangle index, -1 on a miss
+    int32_t slot;  // leaf-order position of the hit triangle
+};
+
+// Traversal stack: the top S entries live in the lane's LDS column (a ring of S slots,
+// col[k * BLOCK]); deeper entries spill to a per-lane global region that is only touched
+// when the stack is deeper than S (the spill region holds height - S + 1 entries).
+template <int S, int BLOCK>
+struct LaneStack {
+    int32_t* col;
+    int32_t* spill;
+    int sp;
+    __device__ __forceinline__ void Push(int32_t x) {
+        int32_t* slot = &col[(sp & (S - 1)) * BLOCK];
+        if (sp >= S) spill[sp - S] = *slot;
+        *slot = x;
+        sp++;
+    }
+    __device__ __forceinline__ int32_t Pop() {
+        sp--;
+        int32_t* slot = &col[(sp & (S - 1)) * BLOCK];
+        const int32_t x = *slot;
+        if (sp >= S) *slot = spill[sp - S];
+        return x;
+    }
+};
+
+struct TraceCounters {
+    uint32_t nodes;  // inner nodes visited (two box tests each)
+    uint32_t tris;   // triangle tests
+};
+
+// Ordered DFS: nearer child first, the other pushed. ANY stops at the first hit inside
+// (tmin, tmax). Bounded: a tree visits each node at most once.
+template <bool ANY, int S, int BLOCK>
+__device__ __forceinline__ HitResult Traverse(const TraceTree& tree, const WtRay& w, bool cull,
+                                              LaneStack<S, BLOCK>& st, TraceCounters& cnt) {
+    HitResult r{0.0f, 0.0f, 0.0f, -1, -1};
+    float best_t = w.tmax;
+    st.sp = 0;
+    int32_t ref = 0;
+    int64_t budget = 2 * int64_t(tree.node_count) + 2;
+    for (;;) {
+        if (--budget < 0 || ref >= tree.node_count) break;
+        if (ref >= 0) {
+            cnt.nodes++;
+            const float4* np = reinterpret_cast<const float4*>(tree.nodes + ref);
+            const float4 a = np[0], b = np[1], c = np[2];
+            const int4 d = reinterpret_cast<const int4*>(np)[3];
+            float tl, tr;
+            const bool hl = Box(w, best_t, a.x, a.y, a.z, a.w, c.x, c.y, tl);
+            const bool hr = Box(w, best_t, b.x, b.y, b.z, b.w, c.z, c.w, tr);
+            if (hl && hr) {
+                const bool rfirst = tr < tl;
+                st.Push(rfirst ? d.x : d.y);
+                ref = rfirst ? d.y : d.x;
+            } else if (hl) {
+                ref = d.x;
+            } else if (hr) {
+                ref = d.y;
+            } else {
+                if (st.sp == 0) break;
+                ref = st.Pop();
+            }
+        } else {
+            const int32_t packed = ~ref;
+            const int32_t b0 = packed >> 4;
+            const int32_t np = b0 + (packed & 15) <= tree.prim_count ? (packed & 15) : 0;
+            bool done = false;
+            for (int32_t k = b0; k < b0 + np; k++) {
+                cnt.tris++;
+                const float4 p0 = tree.tris[3 * int64_t(k)], p1 = tree.tris[3 * int64_t(k) + 1],
+                             p2 = tree.tris[3 * int64_t(k) + 2];
+                float t, u1, u2;
+                if (!Watertight(w, p0, p1, p2, cull, t, u1, u2)) continue;
+                if (!(t > w.tmin)) continue;
+                const int32_t prim = __float_as_int(p0.w);
+                if (ANY) {
+                    if (t < w.tmax) {
+                        r = HitResult{t, u1, u2, prim, k};
+                        done = true;
+                        break;
+                    }
+                    continue;
+                }
+                if (t < best_t || (t == best_t && r.prim >= 0 && prim < r.prim)) {
+                    best_t = t;
+                    r = HitResult{t, u1, u2, prim, k};
+                }
+            }
+            if (done || st.sp == 0) break;
+            ref = st.Pop();
+        }
+    }
+    return r;
+}
+
+}  // namespace dpl

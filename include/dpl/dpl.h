@@ -67,7 +67,8 @@ typedef struct dpl_build_params {
     int32_t kdop_size;              /* SOBB direction-set hint (reference default 32) */
     int32_t block_size;             /* PLOC tile threads: 0, 128, 256 or 512 */
     int32_t sort_digit_bits;        /* radix digit width: 0, 4 or 8 */
-    int32_t keep_intermediates;     /* non-zero: keep codes/order/merge list for download */
+    int32_t keep_intermediates;     /* bit 0: keep codes/order/merge list for download;
+                                       bit 1: time every PLOC iteration launch (adds events) */
 } dpl_build_params;
 
 /* Fills *p from an acceleration-structure type and the reference's config values.
@@ -89,7 +90,9 @@ typedef struct dpl_build_stats {
     int64_t memory_consumption; /* bytes of the final BVH (nodes + ordered triangles + ids) */
     double sah_cost;         /* (Ct*sum_inner SA + Ci*sum_leaf SA*|leaf|) / SA(root), Ct=Ci=1 */
     int32_t tree_height;     /* longest root-to-leaf path of the final tree */
-    int32_t reserved;
+    int32_t ploc_launches;   /* multi-workgroup PLOC iteration launches that merged */
+    double ploc_launch_ms;   /* their summed device time (keep_intermediates bit 1), else 0 */
+    double ploc_bytes;       /* their algorithmic bytes: 32 c_k + 32 c_k+1 + 91 m_k (bit 1), else 0 */
 } dpl_build_stats;
 
 typedef struct dpl_context dpl_context;
@@ -150,6 +153,12 @@ dpl_status dpl_bvh_get_stats(const dpl_bvh* bvh, dpl_build_stats* stats);
  * Closest-hit returns the smallest t with tmin < t < tmax, ties to the lower prim id. */
 dpl_status dpl_trace(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
                      dpl_hit* hits, dpl_mem mem);
+
+/* dpl_trace plus traversal counters (inner nodes visited, triangle tests), the GPU
+ * counterparts of AccelerationStats::traversal_count / intersection_count
+ * (src/stats/acceleration_stats.h:15-16). Synchronises. */
+dpl_status dpl_trace_counters(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
+                              dpl_hit* hits, dpl_mem mem, int64_t* node_visits, int64_t* triangle_tests);
 
 /* Batched BVH: ray i is traced against mesh ray_mesh[i] (device or host as `mem`). */
 dpl_status dpl_trace_batched(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, const int32_t* ray_mesh,

@@ -249,3 +249,122 @@ void orc_pcg_stream(uint64_t seed, uint64_t stream, int32_t count, uint32_t* out
 void orc_sincos2pi(float u, float* s, float* c) { SinCos2Pi(u, *s, *c); }
 
 }  // extern "C"
+
+// ---- diffuse path tracing ----
+extern "C" {
+
+void orc_camera_basis(const float* cam, int w, int h, float* out12) { CameraBasis(cam, w, h, out12); }
+
+// Forward render over the oracle's PLOC tree. image: W*H*3 (acc / spp per pixel, samples
+// summed in order). path_mats (nullable): W*H*spp*depth vertex materials, -1 padded.
+void orc_render(void* ploc, const float* tri_xyz, const int32_t* mat_id, const float* albedo, const float* emission,
+                int W, int H, int spp, int depth, uint64_t seed, const float* cam, float* image, int32_t* path_mats,
+                int threads) {
+    const FinalTree& T = static_cast<PlocHandle*>(ploc)->r.tree;
+    float basis[12];
+    CameraBasis(cam, W, H, basis);
+    const float far = cam[10];
+    auto rows = [&](int y0, int y1) {
+        std::vector<int32_t> mats(depth);
+        std::vector<Vec3> betas(depth);
+        for (int y = y0; y < y1; y++) {
+            for (int x = 0; x < W; x++) {
+                Vec3 acc(0.0f);
+                for (int s = 0; s < spp; s++) {
+                    int k = TracePathVertices(T, tri_xyz, mat_id, albedo, basis, far, x, y, W, spp, s, depth, seed,
+                                              mats.data(), betas.data());
+                    Vec3 L(0.0f);
+                    for (int v = 0; v < k; v++) {
+                        const int32_t m = mats[v];
+                        L = L + betas[v] * Vec3{emission[3 * m], emission[3 * m + 1], emission[3 * m + 2]};
+                    }
+                    acc = acc + L;
+                    if (path_mats) {
+                        int32_t* pm = path_mats + ((int64_t(y) * W + x) * spp + s) * depth;
+                        for (int v = 0; v < depth; v++) pm[v] = v < k ? mats[v] : -1;
+                    }
+                }
+                float* px = image + (int64_t(y) * W + x) * 3;
+                px[0] = acc.x / float(spp);
+                px[1] = acc.y / float(spp);
+                px[2] = acc.z / float(spp);
+            }
+        }
+    };
+    if (threads <= 1) {
+        rows(0, H);
+    } else {
+        std::vector<std::thread> th;
+        for (int t = 0; t < threads; t++) th.emplace_back(rows, H * t / threads, H * (t + 1) / threads);
+        for (auto& x : th) x.join();
+    }
+}
+
+// FP64 image of the recorded paths for parameters (albedo, emission) — the paths do not
+// depend on them (albedo only scales throughput, it drives no sampling decision).
+static void Image64(const int32_t* pm, int64_t pixels, int spp, int depth, const double* albedo,
+                    const double* emission, double* img) {
+    for (int64_t p = 0; p < pixels; p++) {
+        double acc[3] = {0, 0, 0};
+        for (int s = 0; s < spp; s++) {
+            const int32_t* v = pm + (p * spp + s) * depth;
+            double beta[3] = {1, 1, 1};
+            for (int k = 0; k < depth && v[k] >= 0; k++) {
+                for (int c = 0; c < 3; c++) {
+                    acc[c] += beta[c] * emission[3 * v[k] + c];
+                    beta[c] *= albedo[3 * v[k] + c];
+                }
+            }
+        }
+        for (int c = 0; c < 3; c++) img[3 * p + c] = acc[c] / spp;
+    }
+}
+
+// L = sum (I - target)^2 in FP64.
+double orc_path_loss_fp64(const int32_t* pm, int64_t pixels, int spp, int depth, const double* albedo,
+                          const double* emission, const float* target) {
+    std::vector<double> img(size_t(pixels) * 3);
+    Image64(pm, pixels, spp, depth, albedo, emission, img.data());
+    double l = 0.0;
+    for (size_t i = 0; i < img.size(); i++) {
+        const double r = img[i] - double(target[i]);
+        l += r * r;
+    }
+    return l;
+}
+
+// Analytic dL/dalbedo and dL/demission in FP64 (M x 3 each, overwritten).
+void orc_path_grad_fp64(const int32_t* pm, int64_t pixels, int spp, int depth, int M, const double* albedo,
+                        const double* emission, const float* target, double* ga, double* ge) {
+    std::vector<double> img(size_t(pixels) * 3);
+    Image64(pm, pixels, spp, depth, albedo, emission, img.data());
+    for (int i = 0; i < 3 * M; i++) ga[i] = ge[i] = 0.0;
+    std::vector<double> beta(3 * size_t(depth));
+    for (int64_t p = 0; p < pixels; p++) {
+        double delta[3];
+        for (int c = 0; c < 3; c++) delta[c] = 2.0 * (img[3 * p + c] - double(target[3 * p + c])) / spp;
+        for (int s = 0; s < spp; s++) {
+            const int32_t* v = pm + (p * spp + s) * depth;
+            int K = 0;
+            double b[3] = {1, 1, 1};
+            while (K < depth && v[K] >= 0) {
+                for (int c = 0; c < 3; c++) {
+                    beta[3 * K + c] = b[c];
+                    b[c] *= albedo[3 * v[K] + c];
+                }
+                K++;
+            }
+            double S[3] = {0, 0, 0};
+            for (int k = K - 1; k >= 0; k--) {
+                const int m = v[k];
+                for (int c = 0; c < 3; c++) {
+                    ge[3 * m + c] += delta[c] * beta[3 * k + c];
+                    ga[3 * m + c] += delta[c] * beta[3 * k + c] * S[c];
+                    S[c] = emission[3 * m + c] + albedo[3 * m + c] * S[c];
+                }
+            }
+        }
+    }
+}
+
+}  // extern "C"

@@ -288,3 +288,10 @@ float PcgFloat(Pcg32& r);
 void SinCos2Pi(float u, float& s, float& c);  // polynomial sin/cos of 2*pi*u, u in [0,1)
 
 }  // namespace orc
+
+namespace orc {
+void CameraBasis(const float* cam, int w, int h, float* out12);
+int TracePathVertices(const FinalTree& T, const float* tri_xyz, const int32_t* mat_id, const float* albedo,
+                      const float* basis, float far, int x, int y, int W, int spp, int s, int depth, uint64_t seed,
+                      int32_t* mats, Vec3* betas);
+}  // namespace orc

@@ -272,3 +272,52 @@ class Ploc:
         hits = np.zeros(r.shape[0], HIT_DTYPE)
         lib.orc_ploc_trace(self.h, _p(r), r.shape[0], flags, _p(hits), threads or nthreads())
         return hits
+
+
+# ---- diffuse path tracing ------------------------------------------------------------------
+def _sig_render():
+    P, I32, I64, U64, D = C.c_void_p, C.c_int32, C.c_int64, C.c_uint64, C.c_double
+    for name, res, args in [
+        ("orc_camera_basis", None, [P, I32, I32, P]),
+        ("orc_render", None, [P, P, P, P, P, I32, I32, I32, I32, U64, P, P, P, I32]),
+        ("orc_path_loss_fp64", D, [P, I64, I32, I32, P, P, P]),
+        ("orc_path_grad_fp64", None, [P, I64, I32, I32, I32, P, P, P, P, P]),
+    ]:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+_sig_render()
+
+
+def render(ploc: "Ploc", scene, W, H, spp, depth, seed, albedo=None, emission=None, record_paths=False,
+           threads=None):
+    """CPU restatement of the engine's diffuse path tracer over the oracle's PLOC tree.
+    Returns (image (H,W,3) float32, path materials (W*H*spp, depth) int32 or None)."""
+    alb = _f32(scene.albedo if albedo is None else albedo)
+    emi = _f32(scene.emission if emission is None else emission)
+    mat = np.ascontiguousarray(scene.mat_id, np.int32)
+    cam = scene.camera.as_array()
+    img = np.zeros((H, W, 3), np.float32)
+    pm = np.full((W * H * spp, depth), -1, np.int32) if record_paths else None
+    lib.orc_render(ploc.h, _p(ploc.tris), _p(mat), _p(alb), _p(emi), W, H, spp, depth, seed, _p(cam), _p(img),
+                   _p(pm) if pm is not None else None, threads or nthreads())
+    return img, pm
+
+
+def path_loss_fp64(pm, pixels, spp, depth, albedo64, emission64, target) -> float:
+    a = np.ascontiguousarray(albedo64, np.float64)
+    e = np.ascontiguousarray(emission64, np.float64)
+    t = _f32(target)
+    return float(lib.orc_path_loss_fp64(_p(pm), pixels, spp, depth, _p(a), _p(e), _p(t)))
+
+
+def path_grad_fp64(pm, pixels, spp, depth, albedo64, emission64, target):
+    a = np.ascontiguousarray(albedo64, np.float64)
+    e = np.ascontiguousarray(emission64, np.float64)
+    t = _f32(target)
+    ga = np.zeros_like(a)
+    ge = np.zeros_like(e)
+    lib.orc_path_grad_fp64(_p(pm), pixels, spp, depth, a.shape[0], _p(a), _p(e), _p(t), _p(ga), _p(ge))
+    return ga, ge

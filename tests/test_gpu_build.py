@@ -27,7 +27,9 @@ def build(ctx, tris, mt=0, radius=16, leaf=4, block=0, digit=0, keep=True):
 
     p = accel.make_params(morton_type=mt, radius=radius, max_leaf=leaf, block_size=block, sort_digit_bits=digit,
                           keep_intermediates=keep)
-    return accel.GpuBvh(ctx).build(np.ascontiguousarray(tris, np.float32), p)
+    if isinstance(tris, np.ndarray):
+        tris = np.ascontiguousarray(tris, np.float32)
+    return accel.GpuBvh(ctx).build(tris, p)
 
 
 def compare_to_oracle(oracle, ctx, tris, mt, radius, leaf, block=0, digit=0):
