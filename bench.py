@@ -18,6 +18,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# AccelerationStructureType -> (Morton type, bounding volume), src/gpu/config/gpu_config_bridge.cc:27-53
+VARIANTS = {"ploc": (0, 0), "ploc_emc": (1, 0), "ploc_sobb": (0, 1), "ploc_emc_sobb": (1, 1)}
 
 
 def cpu_model() -> str:
@@ -39,7 +41,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--variant", default="ploc_emc", choices=["ploc", "ploc_emc"])
+    ap.add_argument("--variant", default="ploc_emc_sobb", choices=list(VARIANTS))
+    ap.add_argument("--no-sweep", action="store_true", help="skip the variant x radius sweep")
     ap.add_argument("--radius", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
@@ -62,8 +65,8 @@ def main() -> int:
     rays = torch.from_numpy(rays_np).to(dev)
     hits = torch.empty((len(rays_np), 4), dtype=torch.float32, device=dev)
     ctx = accel.Context(0)
-    mt = 1 if args.variant == "ploc_emc" else 0
-    params = accel.make_params(morton_type=mt, radius=args.radius, max_leaf=4)
+    mt, bv = VARIANTS[args.variant]
+    params = accel.make_params(morton_type=mt, radius=args.radius, max_leaf=4, bv_type=bv)
     bvh = accel.GpuBvh(ctx)
     stream = torch.cuda.current_stream()
 
@@ -107,7 +110,7 @@ def main() -> int:
 
     # ---- roofline of the PLOC iteration kernel: one profiled build (per-launch events) ----
     prof = accel.GpuBvh(ctx).build(tris, accel.make_params(morton_type=mt, radius=args.radius, max_leaf=4,
-                                                             keep_intermediates=3))
+                                                             bv_type=bv, keep_intermediates=3))
     ps = prof.stats
     roof = None
     if ps.ploc_launches > 0 and ps.ploc_launch_ms > 0:
@@ -144,6 +147,29 @@ def main() -> int:
         f"nodes and {pt / cam_rays.shape[0]:.1f} tris per ray")
     rb.close()
 
+    # ---- variant x radius sweep of the 2M scene (build time, SAH, trace rate) ----
+    sweep = []
+    if not args.no_sweep:
+        sb_ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        g = accel.GpuBvh(ctx)
+        for name, (vm, vb) in VARIANTS.items():
+            for r in (8, 16, 32):
+                p = accel.make_params(morton_type=vm, radius=r, max_leaf=4, bv_type=vb)
+                g.build(tris, p)
+                g.trace(rays, 0, hits)
+                sb_ev[0].record(stream)
+                g.build(tris, p)
+                sb_ev[1].record(stream)
+                g.trace(rays, 0, hits)
+                sb_ev[2].record(stream)
+                torch.cuda.synchronize()
+                sweep.append({"variant": name, "radius": r, "build_ms": round(sb_ev[0].elapsed_time(sb_ev[1]), 4),
+                              "trace_ms": round(sb_ev[1].elapsed_time(sb_ev[2]), 4),
+                              "sah_cost": g.stats.sah_cost, "iterations": g.stats.ploc_iterations})
+        g.close()
+        log("sweep: " + ", ".join(f"{s['variant']}/r{s['radius']}: {s['build_ms']:.2f} ms sah {s['sah_cost']:.1f} "
+                                  f"trace {s['trace_ms']:.1f} ms" for s in sweep))
+
     # ---- CPU baseline: the reference's SAH BVH build (1 core) and tracer on a sample ----
     cpu = None
     sah_ratio = None
@@ -153,6 +179,8 @@ def main() -> int:
 
         sb = pyoracle.SBvh(scene.tris, fix_child_bounds=False)
         sah_cpu = sb.sah()
+        for s in sweep:
+            s["sah_ratio_vs_cpu"] = round(s["sah_cost"] / sah_cpu, 4)
         sah_fixed = pyoracle.SBvh(scene.tris, fix_child_bounds=True).sah()
         sah_ratio = last["sah_cost"] / sah_cpu
         sah_ratio_fixed = last["sah_cost"] / sah_fixed
@@ -207,6 +235,7 @@ def main() -> int:
         "trace_tris_per_ray": round(tests / nrays, 2),
         "trace_touched_bytes_per_ray": round(touched, 1),
         "wall_s": round(wall, 3),
+        "variant_sweep": sweep,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

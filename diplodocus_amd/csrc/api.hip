@@ -15,7 +15,55 @@
 
 namespace dpl {
 
-hipError_t LaunchSah(const FNode* nodes, const int32_t* counters, double* out, hipStream_t s);
+hipError_t LaunchSah(const FNode* nodes, const int32_t* counters, double* out, const float* sobb_dirs,
+                     hipStream_t s);
+
+// SOBB direction set (same construction as the oracle): the 13 directions of the 26-DOP or a
+// subset chosen by kdop_size, and every triple with |det| >= 1e-3 in lexicographic order
+// with 1/|det| computed in FP64 and rounded once.
+struct SobbSetHost {
+    int ndirs = 0, ntriples = 0;
+    float dirs[3 * kMaxDirs];
+    int4 triples[kMaxTriples];
+};
+
+SobbSetHost MakeSobbSetHost(int kdop_size) {
+    const float s2 = 0.70710677f, s3 = 0.57735026f;
+    const float all[13][3] = {{1, 0, 0},      {0, 1, 0},      {0, 0, 1},       {s2, s2, 0},     {s2, -s2, 0},
+                              {s2, 0, s2},    {s2, 0, -s2},   {0, s2, s2},     {0, s2, -s2},    {s3, s3, s3},
+                              {s3, s3, -s3},  {s3, -s3, s3},  {-s3, s3, s3}};
+    std::vector<int> pick;
+    if (kdop_size >= 26) {
+        for (int k = 0; k < 13; k++) pick.push_back(k);
+    } else if (kdop_size >= 18) {
+        for (int k = 0; k < 9; k++) pick.push_back(k);
+    } else if (kdop_size >= 14) {
+        pick = {0, 1, 2, 9, 10, 11, 12};
+    } else {
+        pick = {0, 1, 2};
+    }
+    SobbSetHost S;
+    S.ndirs = int(pick.size());
+    for (int k = 0; k < S.ndirs; k++)
+        for (int a = 0; a < 3; a++) S.dirs[3 * k + a] = all[pick[k]][a];
+    for (int a = 0; a < S.ndirs; a++)
+        for (int b = a + 1; b < S.ndirs; b++)
+            for (int c = b + 1; c < S.ndirs; c++) {
+                const double n1[3] = {S.dirs[3 * a], S.dirs[3 * a + 1], S.dirs[3 * a + 2]};
+                const double n2[3] = {S.dirs[3 * b], S.dirs[3 * b + 1], S.dirs[3 * b + 2]};
+                const double n3[3] = {S.dirs[3 * c], S.dirs[3 * c + 1], S.dirs[3 * c + 2]};
+                const double cx = n2[1] * n3[2] - n2[2] * n3[1];
+                const double cy = n2[2] * n3[0] - n2[0] * n3[2];
+                const double cz = n2[0] * n3[1] - n2[1] * n3[0];
+                const double det = n1[0] * cx + n1[1] * cy + n1[2] * cz;
+                if (std::fabs(det) < 1e-3) continue;
+                const float inv = float(1.0 / std::fabs(det));
+                int32_t bits;
+                std::memcpy(&bits, &inv, 4);
+                S.triples[S.ntriples++] = make_int4(a, b, c, bits);
+            }
+    return S;
+}
 
 // Camera basis of src/renderer/cpu_raytracer.cc:62-73 (pos, p00, qw, qh), evaluated once on
 // the host in FP32 with the same operation order.
@@ -104,7 +152,8 @@ struct dpl_context {
     // scratch
     DevBuf tri, keys, vals, keys_alt, vals_alt, nodes, parent, clusters0, clusters1, ccount, status0, status1,
         small, count, cost, collapsed, minleaf, height, leafslot, keptflag, scan_tmp, sort_tmp, rays, hits,
-        trace_counter, spill;
+        trace_counter, spill, kdop, sobb_set;
+    int sobb_kdop = -1;  // kdop_size the uploaded direction set was made for
     void* pinned = nullptr;          // host staging for small results (device-visible)
     int32_t* pinned_dev = nullptr;   // device address of `pinned`
     hipEvent_t ev[10] = {};
@@ -128,6 +177,9 @@ struct dpl_bvh {
     dpl_build_params params{};
     dpl_build_stats stats{};
     DevBuf fnodes, tri_ordered, prim_order;
+    DevBuf sobb_dirs;  // SOBB trees: the direction set's unit normals
+    int32_t ndirs = 0;  // 0 for AABB trees
+    int sobb_kdop = -1;
     // intermediates (keep_intermediates)
     DevBuf codes, sorted_idx, pnodes;
     std::vector<int32_t> iter_counts;
@@ -178,7 +230,6 @@ dpl_build_params Normalize(const dpl_build_params* p) {
             "max_triangles_per_leaf must be in [1, 15]");
     Require(q.block_size == 128 || q.block_size == 256 || q.block_size == 512, "block_size must be 128, 256 or 512");
     Require(q.sort_digit_bits == 4 || q.sort_digit_bits == 8, "sort_digit_bits must be 4 or 8");
-    if (q.bv_type == DPL_BV_SOBB) throw Error(DPL_ERR_UNSUPPORTED, "SOBB node bounds are not available in this build");
     return q;
 }
 
@@ -245,6 +296,25 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     b.fnodes = bvh->fnodes.ensure<FNode>(size_t(std::max<int64_t>(1, n - 1)) * sizeof(FNode));
     b.tri_ordered = bvh->tri_ordered.ensure<float4>(size_t(n) * 48);
     b.prim_order = bvh->prim_order.ensure<int32_t>(size_t(n) * 4);
+    if (p.bv_type == DPL_BV_SOBB) {
+        const SobbSetHost S = MakeSobbSetHost(p.kdop_size);
+        uint8_t* sd = C.sobb_set.ensure<uint8_t>(sizeof(S.dirs) + sizeof(S.triples));
+        if (C.sobb_kdop != p.kdop_size) {
+            DPL_HIP(hipMemcpy(sd, S.dirs, sizeof(S.dirs), hipMemcpyHostToDevice));
+            DPL_HIP(hipMemcpy(sd + sizeof(S.dirs), S.triples, sizeof(S.triples), hipMemcpyHostToDevice));
+            C.sobb_kdop = p.kdop_size;
+        }
+        b.ndirs = S.ndirs;
+        b.ntriples = S.ntriples;
+        b.dirs = reinterpret_cast<const float*>(sd);
+        b.triples = reinterpret_cast<const int4*>(sd + sizeof(S.dirs));
+        b.kdop = C.kdop.ensure<float>(size_t(nn) * 2 * S.ndirs * 4);
+        if (bvh->sobb_kdop != p.kdop_size) {
+            float* bd = bvh->sobb_dirs.ensure<float>(sizeof(S.dirs));
+            DPL_HIP(hipMemcpy(bd, S.dirs, sizeof(S.dirs), hipMemcpyHostToDevice));
+            bvh->sobb_kdop = p.kdop_size;
+        }
+    }
     if (p.keep_intermediates) {
         bvh->codes.ensure(size_t(n) * 8);
         bvh->sorted_idx.ensure(size_t(n) * 4);
@@ -315,7 +385,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     DPL_HIP(hipEventRecord(C.ev[5], s));
     DPL_HIP(LaunchLayout(b, p.max_triangles_per_leaf, s));
     DPL_HIP(hipEventRecord(C.ev[6], s));
-    DPL_HIP(LaunchSah(b.fnodes, b.counters, b.sah, s));
+    DPL_HIP(LaunchSah(b.fnodes, b.counters, b.sah, b.ndirs > 0 ? b.dirs : nullptr, s));
     DPL_HIP(LaunchFinish(b, s));
     DPL_HIP(hipEventRecord(C.ev[7], s));
     if (p.keep_intermediates) {
@@ -383,6 +453,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     bvh->prim_count = n;
     bvh->node_count = fcount;
     bvh->tree_height = root_height;
+    bvh->ndirs = b.ndirs;
     bvh->params = p;
     bvh->stats = st;
     bvh->iter_counts = iters;
@@ -452,6 +523,7 @@ void dpl_context_destroy(dpl_context* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->iter_ev) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
     delete ctx;
@@ -579,7 +651,7 @@ static dpl_status TraceImpl(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray*
             DPL_HIP(hipMemcpyAsync(const_cast<dpl_ray*>(dr), rays, size_t(n) * sizeof(dpl_ray), hipMemcpyHostToDevice, s));
         }
         TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
-                       int32_t(bvh->prim_count), bvh->tree_height};
+                       int32_t(bvh->prim_count), bvh->tree_height, bvh->ndirs, bvh->sobb_dirs.as<float>()};
         int32_t* spill = ctx->spill.ensure<int32_t>(TraceSpillBytes(ctx->cu_count, bvh->tree_height));
         uint8_t* tc = ctx->trace_counter.as<uint8_t>();
         unsigned long long* counters = nullptr;
@@ -698,7 +770,7 @@ static dpl_status Render(bool adjoint, dpl_context* ctx, const dpl_bvh* bvh, con
         float basis[12];
         CameraBasisHost(rp->camera, rp->width, rp->height, basis);
         TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
-                       int32_t(bvh->prim_count), bvh->tree_height};
+                       int32_t(bvh->prim_count), bvh->tree_height, bvh->ndirs, bvh->sobb_dirs.as<float>()};
         int32_t* spill = ctx->spill.ensure<int32_t>(TraceSpillBytes(ctx->cu_count, bvh->tree_height));
         hipError_t e = LaunchRender(adjoint, tree, *rp, basis, mat_id, albedo, emission, d_image, image, grad_albedo,
                                     grad_emission, ctx->cu_count, ctx->trace_counter.as<uint32_t>(), spill,

@@ -331,8 +331,73 @@ __device__ __forceinline__ void StoreEval(const BuildBuffers& b, int32_t id, con
     b.collapsed[id] = e.collapsed;
 }
 
+// ---------------------------------------------------------------------------------------
+// SOBB support: k-DOP projections (n . v, FP32, left to right) padded by 1e-6 x the largest
+// scene coordinate, exact unions, and the direction triple of least parallelepiped area
+// (w_a w_b + w_b w_c + w_a w_c) / |det| per child box (ties: first triple).
+// ---------------------------------------------------------------------------------------
+__device__ __forceinline__ float KdopPad(const uint32_t* bounds) {
+    const DecodedBounds d = Decode(bounds);
+    float s = 0.0f;
+    for (int a = 0; a < 3; a++) s = Max2(s, Max2(fabsf(d.vmin[a]), fabsf(d.vmax[a])));
+    return 1e-6f * s;
+}
+
+__device__ __forceinline__ void LeafKdop(const BuildBuffers& b, const float v[9], float pad, float* kd) {
+    const int nd = b.ndirs;
+    for (int k = 0; k < nd; k++) {
+        const float nx = b.dirs[3 * k], ny = b.dirs[3 * k + 1], nz = b.dirs[3 * k + 2];
+        const float p0 = nx * v[0] + ny * v[1] + nz * v[2];
+        const float p1 = nx * v[3] + ny * v[4] + nz * v[5];
+        const float p2 = nx * v[6] + ny * v[7] + nz * v[8];
+        kd[k] = Min2(Min2(p0, p1), p2) - pad;
+        kd[nd + k] = Max2(Max2(p0, p1), p2) + pad;
+    }
+}
+
+__device__ __forceinline__ void UnionKdop(const BuildBuffers& b, int32_t id, int32_t l, int32_t r) {
+    const int nd = b.ndirs, w = 2 * nd;
+    const float* kl = b.kdop + int64_t(l) * w;
+    const float* kr = b.kdop + int64_t(r) * w;
+    float* ko = b.kdop + int64_t(id) * w;
+    for (int k = 0; k < nd; k++) {
+        ko[k] = Min2(kl[k], kr[k]);
+        ko[nd + k] = Max2(kl[nd + k], kr[nd + k]);
+    }
+}
+
+// Writes the child's SOBB (lo_a hi_a lo_b hi_b lo_c hi_c) to f and returns a | b<<4 | c<<8.
+__device__ __forceinline__ uint32_t FitSobb(const BuildBuffers& b, const int4* triples, const float* kd, float* f) {
+    const int nd = b.ndirs;
+    float w[kMaxDirs];
+    for (int k = 0; k < kMaxDirs; k++) w[k] = k < nd ? kd[nd + k] - kd[k] : 0.0f;
+    int bt = 0;
+    float best = 0.0f;
+    for (int t = 0; t < b.ntriples; t++) {
+        const int4 T = triples[t];
+        float wa = 0.0f, wb = 0.0f, wc = 0.0f;
+#pragma unroll
+        for (int k = 0; k < kMaxDirs; k++) {  // register-array lookup without scratch
+            wa = k == T.x ? w[k] : wa;
+            wb = k == T.y ? w[k] : wb;
+            wc = k == T.z ? w[k] : wc;
+        }
+        const float sa = (wa * wb + wb * wc + wa * wc) * __int_as_float(T.w);
+        if (t == 0 || sa < best) {
+            best = sa;
+            bt = t;
+        }
+    }
+    const int4 T = triples[bt];
+    f[0] = kd[T.x], f[1] = kd[nd + T.x];
+    f[2] = kd[T.y], f[3] = kd[nd + T.y];
+    f[4] = kd[T.z], f[5] = kd[nd + T.z];
+    return uint32_t(T.x) | (uint32_t(T.y) << 4) | (uint32_t(T.z) << 8);
+}
+
 __global__ __launch_bounds__(256) void KInitClusters(BuildBuffers b) {
     const int64_t n = b.n;
+    const float pad = b.ndirs > 0 ? KdopPad(b.bounds) : 0.0f;
     for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x) {
         const uint32_t orig = b.vals[k];
         float v[9];
@@ -343,6 +408,7 @@ __global__ __launch_bounds__(256) void KInitClusters(BuildBuffers b) {
         nd.right = -1;
         b.nodes[k] = nd;
         StoreEval(b, int32_t(k), LeafEval(nd.lo, nd.hi, int32_t(k)));
+        if (b.ndirs > 0) LeafKdop(b, v, pad, b.kdop + k * 2 * b.ndirs);
         nd.left = int32_t(k);
         nd.right = 0;
         b.clusters[0][k] = nd;
@@ -437,6 +503,9 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
         return;
     }
     const int T = (c + CHUNK - 1) / CHUNK;
+    // The grid is sized from a count the host read some iterations ago; surplus workgroups
+    // leave before touching the (device-scope, cross-XCD) tile counter.
+    if (int(blockIdx.x) >= T) return;
     uint64_t* status = b.status[iter & 1];
     uint64_t* status_next = b.status[(iter + 1) & 1];
     const int32_t node_base = int32_t(2 * n - c);
@@ -548,6 +617,7 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
                 b.parent[o.left] = id;
                 b.parent[o.right] = id;
                 StoreEval(b, id, InnerEval(o.lo, o.hi, LoadEval(b, o.left), LoadEval(b, o.right), max_leaf));
+                if (b.ndirs > 0) UnionKdop(b, id, o.left, o.right);
                 o.left = id;
                 o.right = 0;
             } else {
@@ -662,6 +732,7 @@ __global__ __launch_bounds__(kTailThreads) void KPlocTail(BuildBuffers b, int it
                 const NodeEval ej{s_cnt[j], s_cost[j], s_ml[j], s_ht[j], 0};
                 keep_e[q] = InnerEval(o.lo, o.hi, ei, ej, max_leaf);
                 StoreEval(b, id, keep_e[q]);
+                if (b.ndirs > 0) UnionKdop(b, id, o.left, o.right);
                 o.left = id;
             } else {
                 o.lo[0] = s_lx[i], o.lo[1] = s_ly[i], o.lo[2] = s_lz[i];
@@ -753,6 +824,12 @@ __global__ __launch_bounds__(256) void KEmit(BuildBuffers b, int max_leaf) {
         return !b.collapsed[id] && !Inside(b, id, max_leaf);
     };
     auto leaf_ref = [&](int32_t id) -> int32_t { return ~((begin[b.minleaf[id]] << 4) | b.count[id]); };
+    __shared__ int4 s_tri[kMaxTriples];
+    if (b.ndirs > 0) {
+        for (int t = threadIdx.x; t < b.ntriples; t += blockDim.x) s_tri[t] = b.triples[t];
+        __syncthreads();
+    }
+    const int64_t kw = 2 * int64_t(b.ndirs);
     for (int64_t id = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; id < nn; id += int64_t(gridDim.x) * blockDim.x) {
         const bool col = b.collapsed[id] != 0;
         const bool in = Inside(b, int32_t(id), max_leaf);
@@ -790,6 +867,12 @@ __global__ __launch_bounds__(256) void KEmit(BuildBuffers b, int max_leaf) {
                 f.left = leaf_ref(int32_t(id));
                 f.right = ~0;
                 f.pad0 = f.pad1 = 0;
+                if (b.ndirs > 0) {
+                    float* fl = &f.lx0;
+                    const uint32_t d = FitSobb(b, s_tri, b.kdop + id * kw, fl);
+                    for (int k = 0; k < 6; k++) fl[6 + k] = fl[k];
+                    f.pad0 = int32_t(d | (d << 12));
+                }
                 b.fnodes[0] = f;
             }
         } else if (id >= n && !col && !in) {
@@ -802,6 +885,12 @@ __global__ __launch_bounds__(256) void KEmit(BuildBuffers b, int max_leaf) {
             f.left = kept(nd.left) ? K - 1 - rank[nd.left - n] : leaf_ref(nd.left);
             f.right = kept(nd.right) ? K - 1 - rank[nd.right - n] : leaf_ref(nd.right);
             f.pad0 = f.pad1 = 0;
+            if (b.ndirs > 0) {
+                float* fl = &f.lx0;
+                const uint32_t dl = FitSobb(b, s_tri, b.kdop + int64_t(nd.left) * kw, fl);
+                const uint32_t dr = FitSobb(b, s_tri, b.kdop + int64_t(nd.right) * kw, fl + 6);
+                f.pad0 = int32_t(dl | (dr << 12));
+            }
             b.fnodes[K - 1 - rank[id - n]] = f;
         }
     }
@@ -809,7 +898,20 @@ __global__ __launch_bounds__(256) void KEmit(BuildBuffers b, int max_leaf) {
 
 // SAH of the final tree in FP64 (Ct = Ci = 1, divided by SA(root) on the host); the node
 // count is the number of kept inner nodes (0 when the whole scene is one leaf).
-__global__ __launch_bounds__(256) void KSah(const FNode* nodes, const int32_t* counters, double* out) {
+// SOBB area 2 (wa wb + wb wc + wa wc) / |det(n_a, n_b, n_c)| in FP64.
+__device__ double SobbAreaF64(const float* f, uint32_t d, const float* dirs) {
+    const float* n1 = dirs + 3 * (d & 15);
+    const float* n2 = dirs + 3 * ((d >> 4) & 15);
+    const float* n3 = dirs + 3 * ((d >> 8) & 15);
+    const double det = double(n1[0]) * (double(n2[1]) * n3[2] - double(n2[2]) * n3[1]) +
+                       double(n1[1]) * (double(n2[2]) * n3[0] - double(n2[0]) * n3[2]) +
+                       double(n1[2]) * (double(n2[0]) * n3[1] - double(n2[1]) * n3[0]);
+    const double wa = double(f[1]) - f[0], wb = double(f[3]) - f[2], wc = double(f[5]) - f[4];
+    return 2.0 * (wa * wb + wb * wc + wa * wc) / fabs(det);
+}
+
+__global__ __launch_bounds__(256) void KSah(const FNode* nodes, const int32_t* counters, double* out,
+                                            const float* dirs) {
     const int32_t count = counters[0];
     double acc = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
@@ -818,8 +920,14 @@ __global__ __launch_bounds__(256) void KSah(const FNode* nodes, const int32_t* c
             double sx = double(x1) - x0, sy = double(y1) - y0, sz = double(z1) - z0;
             return 2.0 * (sx * sy + sy * sz + sx * sz);
         };
-        double sl = sa(f.lx0, f.lx1, f.ly0, f.ly1, f.lz0, f.lz1);
-        double sr = sa(f.rx0, f.rx1, f.ry0, f.ry1, f.rz0, f.rz1);
+        double sl, sr;
+        if (dirs) {
+            sl = SobbAreaF64(&f.lx0, uint32_t(f.pad0) & 0xFFFu, dirs);
+            sr = SobbAreaF64(&f.lx0 + 6, (uint32_t(f.pad0) >> 12) & 0xFFFu, dirs);
+        } else {
+            sl = sa(f.lx0, f.lx1, f.ly0, f.ly1, f.lz0, f.lz1);
+            sr = sa(f.rx0, f.rx1, f.ry0, f.ry1, f.rz0, f.rz1);
+        }
         acc += f.left >= 0 ? sl : sl * double((~f.left) & 15);
         acc += f.right >= 0 ? sr : sr * double((~f.right) & 15);
     }
@@ -871,8 +979,9 @@ hipError_t LaunchFinish(const BuildBuffers& b, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t LaunchSah(const FNode* nodes, const int32_t* counters, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(KSah, dim3(512), dim3(256), 0, s, nodes, counters, out);
+hipError_t LaunchSah(const FNode* nodes, const int32_t* counters, double* out, const float* sobb_dirs,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(KSah, dim3(512), dim3(256), 0, s, nodes, counters, out, sobb_dirs);
     return hipGetLastError();
 }
 

@@ -179,6 +179,12 @@ struct BuildBuffers {
     FNode* fnodes;         // n (capacity)
     float4* tri_ordered;   // 3n
     int32_t* prim_order;   // n
+    // SOBB builds (ndirs > 0): k-DOP of every node, 2*ndirs floats (lo..., hi...)
+    int32_t ndirs;
+    int32_t ntriples;
+    float* kdop;           // (2n-1) x 2*ndirs
+    const float* dirs;     // ndirs x 3 unit normals
+    const int4* triples;   // (a, b, c, bits of 1/|det|) per non-coplanar direction triple
 };
 
 hipError_t LaunchBuildInit(const BuildBuffers& b, int chunk, hipStream_t s);
@@ -208,7 +214,12 @@ struct TraceTree {
     int32_t node_count;
     int32_t prim_count;
     int32_t height;
+    int32_t ndirs;        // SOBB trees: directions of the set (0 for AABB trees)
+    const float* dirs;    // SOBB trees: ndirs x 3 unit normals (device)
 };
+
+constexpr int kMaxDirs = 13;
+constexpr int kMaxTriples = 286;
 
 // counters (nullable, device): accumulates {inner nodes visited, triangle tests}.
 hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,

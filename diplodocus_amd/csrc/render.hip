@@ -120,14 +120,22 @@ constexpr int kRenderStack = 16;
 
 }  // namespace
 
-template <bool ADJ, int BLOCK, int MAXD>
+template <bool ADJ, bool SOBB, int BLOCK, int MAXD>
 __global__ __launch_bounds__(BLOCK) void KRender(RenderArgs a, uint32_t* counter) {
     __shared__ int32_t s_stack[kRenderStack][BLOCK];
     __shared__ uint32_t s_base[BLOCK / kWave];
     __shared__ float s_grad[ADJ ? 6 * kMaxMaterials : 1];
+    __shared__ float s_on[SOBB ? kMaxDirs : 1][BLOCK];
+    __shared__ float s_idn[SOBB ? kMaxDirs : 1][BLOCK];
+    __shared__ float s_dirs[SOBB ? 3 * kMaxDirs : 1];
+    if (SOBB) {
+        for (int k = threadIdx.x; k < 3 * a.tree.ndirs; k += BLOCK) s_dirs[k] = a.tree.dirs[k];
+        __syncthreads();
+    }
     const int lane = LaneId(), wid = threadIdx.x / kWave;
     LaneStack<kRenderStack, BLOCK> st{&s_stack[0][threadIdx.x],
                                       a.spill + (int64_t(blockIdx.x) * BLOCK + threadIdx.x) * a.spill_stride, 0};
+    SobbRay sr{&s_on[0][threadIdx.x], &s_idn[0][threadIdx.x], 0.0f};
     TraceCounters cnt{0u, 0u};
     if (ADJ) {
         for (int i = threadIdx.x; i < 6 * a.M; i += BLOCK) s_grad[i] = 0.0f;
@@ -170,7 +178,8 @@ __global__ __launch_bounds__(BLOCK) void KRender(RenderArgs a, uint32_t* counter
                 int K = 0;
                 for (int dep = 0; dep < a.depth; dep++) {
                     const WtRay w = PrepareRay(o.x, o.y, o.z, kRayEps, d.x, d.y, d.z, a.far);
-                    const HitResult h = Traverse<false>(a.tree, w, false, st, cnt);
+                    if (SOBB) PrepareSobb<BLOCK>(sr, s_dirs, a.tree.ndirs, o.x, o.y, o.z, d.x, d.y, d.z);
+                    const HitResult h = Traverse<false, SOBB>(a.tree, w, sr, false, st, cnt);
                     if (h.prim < 0) break;
                     const int32_t m = a.mat_id[h.prim];
                     if (ADJ) {
@@ -292,15 +301,23 @@ hipError_t LaunchRender(bool adjoint, const TraceTree& tree, const dpl_render_pa
     }
     constexpr int B = 256;  // same block and grid policy as the trace kernel (shared spill sizing)
     const int grid = TraceGrid(cu_count, int64_t(a.W) * a.H);
+#define DPL_RENDER(ADJ_, SOBB_, MD_) \
+    hipLaunchKernelGGL((KRender<ADJ_, SOBB_, B, MD_>), dim3(grid), dim3(B), 0, s, a, counter)
+    const bool sobb = tree.ndirs > 0;
     if (!adjoint) {
-        hipLaunchKernelGGL((KRender<false, B, 1>), dim3(grid), dim3(B), 0, s, a, counter);
+        if (sobb) DPL_RENDER(false, true, 1);
+        else DPL_RENDER(false, false, 1);
     } else if (a.depth <= 4) {
-        hipLaunchKernelGGL((KRender<true, B, 4>), dim3(grid), dim3(B), 0, s, a, counter);
+        if (sobb) DPL_RENDER(true, true, 4);
+        else DPL_RENDER(true, false, 4);
     } else if (a.depth <= 8) {
-        hipLaunchKernelGGL((KRender<true, B, 8>), dim3(grid), dim3(B), 0, s, a, counter);
+        if (sobb) DPL_RENDER(true, true, 8);
+        else DPL_RENDER(true, false, 8);
     } else {
-        hipLaunchKernelGGL((KRender<true, B, 16>), dim3(grid), dim3(B), 0, s, a, counter);
+        if (sobb) DPL_RENDER(true, true, 16);
+        else DPL_RENDER(true, false, 16);
     }
+#undef DPL_RENDER
     return hipGetLastError();
 }
 

@@ -1,6 +1,7 @@
 // Ray-batch trace kernel for gfx950: persistent grid, rays fetched 64 at a time per wave
 // from a global atomic counter, a 16-entry LDS stack ring per lane with a global spill
-// region for deeper stacks (see traverse.h), optional traversal counters.
+// region for deeper stacks (see traverse.h), AABB or SOBB child boxes, optional
+// traversal counters.
 #include <hip/hip_runtime.h>
 
 #include "dpl_internal.h"
@@ -14,17 +15,25 @@ constexpr int kTraceBlock = 256;
 constexpr int kTraceWgPerCu = 8;
 }  // namespace
 
-template <int BLOCK, bool COUNT>
+template <int BLOCK, bool COUNT, bool SOBB>
 __global__ __launch_bounds__(BLOCK) void KTrace(TraceTree tree, const float4* __restrict__ rays, int64_t n,
                                                 uint32_t flags, dpl_hit* __restrict__ hits, uint32_t* counter,
                                                 int32_t* spill, int32_t spill_stride,
                                                 unsigned long long* counters) {
     __shared__ int32_t s_stack[kStackLds][BLOCK];
     __shared__ uint32_t s_base[BLOCK / kWave];
+    __shared__ float s_on[SOBB ? kMaxDirs : 1][BLOCK];
+    __shared__ float s_idn[SOBB ? kMaxDirs : 1][BLOCK];
+    __shared__ float s_dirs[SOBB ? 3 * kMaxDirs : 1];
+    if (SOBB) {
+        for (int k = threadIdx.x; k < 3 * tree.ndirs; k += BLOCK) s_dirs[k] = tree.dirs[k];
+        __syncthreads();
+    }
     const bool any = flags & DPL_TRACE_ANY, cull = flags & DPL_TRACE_CULL_BACK;
     const int lane = LaneId(), wid = threadIdx.x / kWave;
     LaneStack<kStackLds, BLOCK> st{&s_stack[0][threadIdx.x],
                                    spill + (int64_t(blockIdx.x) * BLOCK + threadIdx.x) * spill_stride, 0};
+    SobbRay sr{&s_on[0][threadIdx.x], &s_idn[0][threadIdx.x], 0.0f};
     TraceCounters cnt{0u, 0u};
     for (;;) {
         if (lane == 0) s_base[wid] = atomicAdd(counter, uint32_t(kWave));
@@ -36,7 +45,9 @@ __global__ __launch_bounds__(BLOCK) void KTrace(TraceTree tree, const float4* __
         if (i < n) {
             const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
             const WtRay w = PrepareRay(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w);
-            const HitResult h = any ? Traverse<true>(tree, w, cull, st, cnt) : Traverse<false>(tree, w, cull, st, cnt);
+            if (SOBB) PrepareSobb<BLOCK>(sr, s_dirs, tree.ndirs, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z);
+            const HitResult h = any ? Traverse<true, SOBB>(tree, w, sr, cull, st, cnt)
+                                    : Traverse<false, SOBB>(tree, w, sr, cull, st, cnt);
             dpl_hit o;
             o.t = h.t;
             o.b1 = h.b1;
@@ -89,13 +100,18 @@ hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, ui
     const float4* r4 = reinterpret_cast<const float4*>(rays);
     const int grid = TraceGrid(cu_count, n);
     const int32_t stride = TraceSpillStride(tree.height);
+#define DPL_TRACE(COUNT_, SOBB_)                                                                                   \
+    hipLaunchKernelGGL((KTrace<kTraceBlock, COUNT_, SOBB_>), dim3(grid), dim3(kTraceBlock), 0, s, tree, r4, n, flags, \
+                       hits, counter, spill, stride, counters)
+    const bool sobb = tree.ndirs > 0;
     if (counters) {
-        hipLaunchKernelGGL((KTrace<kTraceBlock, true>), dim3(grid), dim3(kTraceBlock), 0, s, tree, r4, n, flags, hits,
-                           counter, spill, stride, counters);
+        if (sobb) DPL_TRACE(true, true);
+        else DPL_TRACE(true, false);
     } else {
-        hipLaunchKernelGGL((KTrace<kTraceBlock, false>), dim3(grid), dim3(kTraceBlock), 0, s, tree, r4, n, flags, hits,
-                           counter, spill, stride, counters);
+        if (sobb) DPL_TRACE(false, true);
+        else DPL_TRACE(false, false);
     }
+#undef DPL_TRACE
     return hipGetLastError();
 }
 

@@ -98,6 +98,42 @@ __device__ __forceinline__ bool Box(const WtRay& w, float tfar_limit, float x0, 
     return tn <= tf * 1.0000004f;
 }
 
+// SOBB slab test. Per ray and direction k of the set: n_k . o and 1 / (n_k . d), kept in
+// the lane's LDS columns (on[k * BLOCK], idn[k * BLOCK]); po = 4e-7 |o|_1 bounds the
+// rounding of n_k . o and pads every slab.
+struct SobbRay {
+    float* on;
+    float* idn;
+    float po;
+};
+
+template <int BLOCK>
+__device__ __forceinline__ void PrepareSobb(SobbRay& s, const float* dirs, int ndirs, float ox, float oy, float oz,
+                                            float dx, float dy, float dz) {
+    for (int k = 0; k < ndirs; k++) {
+        const float nx = dirs[3 * k], ny = dirs[3 * k + 1], nz = dirs[3 * k + 2];
+        s.on[k * BLOCK] = nx * ox + ny * oy + nz * oz;
+        s.idn[k * BLOCK] = 1.0f / (nx * dx + ny * dy + nz * dz);
+    }
+    s.po = 4e-7f * (fabsf(ox) + fabsf(oy) + fabsf(oz));
+}
+
+template <int BLOCK>
+__device__ __forceinline__ bool SobbBox(const SobbRay& s, float tmin, float tfar_limit, float la, float ha, float lb,
+                                        float hb, float lc, float hc, uint32_t dirs, float& tnear) {
+    const int a = dirs & 15, b = (dirs >> 4) & 15, c = (dirs >> 8) & 15;
+    const float oa = s.on[a * BLOCK], ia = s.idn[a * BLOCK];
+    const float ob = s.on[b * BLOCK], ib = s.idn[b * BLOCK];
+    const float oc = s.on[c * BLOCK], ic = s.idn[c * BLOCK];
+    const float ta0 = ((la - s.po) - oa) * ia, ta1 = ((ha + s.po) - oa) * ia;
+    const float tb0 = ((lb - s.po) - ob) * ib, tb1 = ((hb + s.po) - ob) * ib;
+    const float tc0 = ((lc - s.po) - oc) * ic, tc1 = ((hc + s.po) - oc) * ic;
+    const float tn = fmaxf(fmaxf(fminf(ta0, ta1), fminf(tb0, tb1)), fmaxf(fminf(tc0, tc1), tmin));
+    const float tf = fminf(fminf(fmaxf(ta0, ta1), fmaxf(tb0, tb1)), fminf(fmaxf(tc0, tc1), tfar_limit));
+    tnear = tn;
+    return tn <= tf * 1.0000004f;
+}
+
 struct HitResult {
     float t, b1, b2;
     int32_t prim;  // input triangle index, -1 on a miss
@@ -133,9 +169,11 @@ struct TraceCounters {
 };
 
 // Ordered DFS: nearer child first, the other pushed. ANY stops at the first hit inside
-// (tmin, tmax). Bounded: a tree visits each node at most once.
-template <bool ANY, int S, int BLOCK>
-__device__ __forceinline__ HitResult Traverse(const TraceTree& tree, const WtRay& w, bool cull,
+// (tmin, tmax). Bounded: a tree visits each node at most once. SOBB selects the skewed
+// child boxes (12 floats = left (lo hi) x 3 slabs, right the same; d.z packs the slab
+// directions, 4 bits each, left in bits 0-11 and right in bits 12-23).
+template <bool ANY, bool SOBB, int S, int BLOCK>
+__device__ __forceinline__ HitResult Traverse(const TraceTree& tree, const WtRay& w, const SobbRay& sr, bool cull,
                                               LaneStack<S, BLOCK>& st, TraceCounters& cnt) {
     HitResult r{0.0f, 0.0f, 0.0f, -1, -1};
     float best_t = w.tmax;
@@ -150,8 +188,15 @@ __device__ __forceinline__ HitResult Traverse(const TraceTree& tree, const WtRay
             const float4 a = np[0], b = np[1], c = np[2];
             const int4 d = reinterpret_cast<const int4*>(np)[3];
             float tl, tr;
-            const bool hl = Box(w, best_t, a.x, a.y, a.z, a.w, c.x, c.y, tl);
-            const bool hr = Box(w, best_t, b.x, b.y, b.z, b.w, c.z, c.w, tr);
+            bool hl, hr;
+            if (SOBB) {
+                const uint32_t dirs = uint32_t(d.z);
+                hl = SobbBox<BLOCK>(sr, w.tmin, best_t, a.x, a.y, a.z, a.w, b.x, b.y, dirs & 0xFFFu, tl);
+                hr = SobbBox<BLOCK>(sr, w.tmin, best_t, b.z, b.w, c.x, c.y, c.z, c.w, (dirs >> 12) & 0xFFFu, tr);
+            } else {
+                hl = Box(w, best_t, a.x, a.y, a.z, a.w, c.x, c.y, tl);
+                hr = Box(w, best_t, b.x, b.y, b.z, b.w, c.z, c.w, tr);
+            }
             if (hl && hr) {
                 const bool rfirst = tr < tl;
                 st.Push(rfirst ? d.x : d.y);

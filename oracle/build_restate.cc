@@ -5,6 +5,7 @@
 // any of these (src/gpu/acceleration/ is empty; README.md:154-161 lists them as planned),
 // so this file is the bit-exact specification the HIP kernels are checked against.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <numeric>
 #include <thread>
@@ -232,6 +233,10 @@ static void BuildFinalTree(const float* tri_xyz, int64_t n, const BuildParams& p
     const int root = int(nn - 1);
     const int maxleaf = p.max_leaf;
     FinalTree& T = R.tree;
+    for (int a = 0; a < 3; a++) {
+        T.root_lo[a] = R.nodes[root].lo[a];
+        T.root_hi[a] = R.nodes[root].hi[a];
+    }
     T.nodes.clear();
     T.prim_order.assign(n, -1);
     T.tri_ordered.assign(12 * n, 0.0f);
@@ -426,29 +431,73 @@ void PlocBuild(const float* tri_xyz, int64_t n, const BuildParams& p, PlocResult
         R.collapsed[id] = col;
     }
     BuildFinalTree(tri_xyz, n, p, R);
+    if (p.bv_type == 1) {
+        const SobbSet S = MakeSobbSet(p.kdop_size);
+        const float pad = KdopPad(R.bounds);
+        const size_t w = size_t(2 * S.ndirs);
+        R.kdop.assign(size_t(nn) * w, 0.0f);
+        for (int64_t k = 0; k < n; k++)
+            LeafKdop(S, LoadTri(tri_xyz + 9 * int64_t(R.sorted_idx[k])), pad, &R.kdop[size_t(k) * w]);
+        for (int64_t id = n; id < nn; id++) {
+            const float* l = &R.kdop[size_t(R.nodes[id].left) * w];
+            const float* r = &R.kdop[size_t(R.nodes[id].right) * w];
+            float* o = &R.kdop[size_t(id) * w];
+            for (int k = 0; k < S.ndirs; k++) {
+                o[k] = Min2(l[k], r[k]);
+                o[S.ndirs + k] = Max2(l[S.ndirs + k], r[S.ndirs + k]);
+            }
+        }
+        BuildSobbNodes(S, R, R.kdop, n, p.max_leaf, R.tree.nodes);
+        R.tree.sobb = true;
+        R.tree.set = S;
+    }
 }
 
+// SOBB surface area 2 (wa wb + wb wc + wa wc) / |det| in FP64 for child s (0 left, 1 right).
+static double SobbArea(const FinalTree& t, const FinalTree::Node& nd, int s) {
+    const float* f = &nd.lx0 + 6 * s;
+    const uint32_t d = (uint32_t(nd.pad0) >> (12 * s)) & 0xFFFu;
+    const int a = d & 15, b = (d >> 4) & 15, c = (d >> 8) & 15;
+    const float* n1 = t.set.dirs[a];
+    const float* n2 = t.set.dirs[b];
+    const float* n3 = t.set.dirs[c];
+    const double det = double(n1[0]) * (double(n2[1]) * n3[2] - double(n2[2]) * n3[1]) +
+                       double(n1[1]) * (double(n2[2]) * n3[0] - double(n2[0]) * n3[2]) +
+                       double(n1[2]) * (double(n2[0]) * n3[1] - double(n2[1]) * n3[0]);
+    const double wa = double(f[1]) - f[0], wb = double(f[3]) - f[2], wc = double(f[5]) - f[4];
+    return 2.0 * (wa * wb + wb * wc + wa * wc) / std::fabs(det);
+}
+
+static double AabbArea(const FinalTree::Node& nd, int s) {
+    const double x0 = s ? nd.rx0 : nd.lx0, x1 = s ? nd.rx1 : nd.lx1;
+    const double y0 = s ? nd.ry0 : nd.ly0, y1 = s ? nd.ry1 : nd.ly1;
+    const double z0 = s ? nd.rz0 : nd.lz0, z1 = s ? nd.rz1 : nd.lz1;
+    const double sx = x1 - x0, sy = y1 - y0, sz = z1 - z0;
+    return 2.0 * (sx * sy + sy * sz + sx * sz);
+}
+
+// Ct * SA(root) + sum over the children of every node of (Ct SA for an inner child,
+// Ci SA |leaf| for a leaf), divided by SA(root); SA is the child's AABB or SOBB area and
+// SA(root) the area of the root's axis-aligned box (both trees share the scene bounds).
 double SahCostFinal(const FinalTree& t, double ct, double ci) {
-    auto sa = [](double x0, double x1, double y0, double y1, double z0, double z1) {
-        double sx = x1 - x0, sy = y1 - y0, sz = z1 - z0;
-        return 2.0 * (sx * sy + sy * sz + sx * sz);
-    };
     if (t.nodes.empty()) return 0.0;
-    const auto& r = t.nodes[0];
-    double root = sa(std::min(r.lx0, r.rx0), std::max(r.lx1, r.rx1), std::min(r.ly0, r.ry0), std::max(r.ly1, r.ry1),
-                     std::min(r.lz0, r.rz0), std::max(r.lz1, r.rz1));
-    double sum = 0.0;
-    if (r.right == ~0 && r.left < 0) {  // single-leaf tree
-        return root > 0 ? ci * sa(r.lx0, r.lx1, r.ly0, r.ly1, r.lz0, r.lz1) * ((~r.left) & 15) / root : 0.0;
-    }
-    sum += ct * root;
+    const double rx = double(t.root_hi[0]) - t.root_lo[0], ry = double(t.root_hi[1]) - t.root_lo[1],
+                 rz = double(t.root_hi[2]) - t.root_lo[2];
+    const double root = 2.0 * (rx * ry + ry * rz + rx * rz);
+    if (!(root > 0)) return 0.0;
+    auto area = [&](const FinalTree::Node& nd, int s) { return t.sobb ? SobbArea(t, nd, s) : AabbArea(nd, s); };
+    const FinalTree::Node& r = t.nodes[0];
+    if (r.right == ~0 && r.left < 0) return ci * area(r, 0) * ((~r.left) & 15) / root;
+    double sum = ct * root;
     for (const auto& nd : t.nodes) {
-        double sl = sa(nd.lx0, nd.lx1, nd.ly0, nd.ly1, nd.lz0, nd.lz1);
-        double sr = sa(nd.rx0, nd.rx1, nd.ry0, nd.ry1, nd.rz0, nd.rz1);
-        sum += nd.left >= 0 ? ct * sl : ci * sl * ((~nd.left) & 15);
-        sum += nd.right >= 0 ? ct * sr : ci * sr * ((~nd.right) & 15);
+        for (int s = 0; s < 2; s++) {
+            const int32_t ref = s == 0 ? nd.left : nd.right;
+            const double a = area(nd, s);
+            sum += ref >= 0 ? ct * a : ci * a * ((~ref) & 15);
+        }
     }
-    return root > 0 ? sum / root : 0.0;
+    return sum / root;
 }
 
 }  // namespace orc
+

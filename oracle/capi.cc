@@ -169,7 +169,8 @@ float orc_sa_union(const float* alo, const float* ahi, const float* blo, const f
     return SurfaceAreaUnion(alo, ahi, blo, bhi);
 }
 
-void* orc_ploc_build(const float* tri, int64_t n, int morton_type, int radius, int max_leaf, double* build_ms) {
+void* orc_ploc_build(const float* tri, int64_t n, int morton_type, int radius, int max_leaf, int bv_type,
+                     int kdop_size, double* build_ms) {
     if (n <= 0) return nullptr;
     PlocHandle* h = new PlocHandle();
     h->n = n;
@@ -177,6 +178,8 @@ void* orc_ploc_build(const float* tri, int64_t n, int morton_type, int radius, i
     p.morton_type = morton_type;
     p.radius = radius;
     p.max_leaf = max_leaf;
+    p.bv_type = bv_type;
+    p.kdop_size = kdop_size;
     auto t0 = std::chrono::steady_clock::now();
     PlocBuild(tri, n, p, h->r);
     if (build_ms) *build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -216,23 +219,52 @@ void orc_ploc_export(void* hv, uint64_t* codes, uint64_t* sorted_codes, uint32_t
     }
 }
 double orc_ploc_sah(void* h) { return SahCostFinal(static_cast<PlocHandle*>(h)->r.tree); }
+// k-DOPs of every PLOC node (SOBB builds): (2n-1) x 2*ndirs floats; returns ndirs (0 if none).
+int32_t orc_ploc_kdop(void* hv, float* out) {
+    PlocHandle* h = static_cast<PlocHandle*>(hv);
+    if (!h->r.tree.sobb) return 0;
+    if (out) std::memcpy(out, h->r.kdop.data(), h->r.kdop.size() * 4);
+    return h->r.tree.set.ndirs;
+}
+// Direction set: dirs (13x3), triples (286 x {a,b,c}), 1/|det| (286); returns the triple count.
+int32_t orc_sobb_set(int32_t kdop_size, int32_t* ndirs, float* dirs, int32_t* abc, float* inv_det) {
+    const SobbSet S = MakeSobbSet(kdop_size);
+    *ndirs = S.ndirs;
+    std::memcpy(dirs, S.dirs, sizeof(S.dirs));
+    for (int t = 0; t < S.ntriples; t++) {
+        abc[3 * t] = S.triples[t].a, abc[3 * t + 1] = S.triples[t].b, abc[3 * t + 2] = S.triples[t].c;
+        inv_det[t] = S.triples[t].inv_det;
+    }
+    return S.ntriples;
+}
 void orc_ploc_trace(void* h, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads) {
     TraceFinal(static_cast<PlocHandle*>(h)->r.tree, rays, n, flags, hits, threads);
 }
 
 // Trace over any tree in the final layout (e.g. one downloaded from the device).
+// sobb_kdop: 0 for AABB child boxes, else the kdop_size of the SOBB direction set.
 void orc_trace_final(const void* fnodes, int64_t nnodes, const float* tri_ordered, int64_t nprim, const RayRec* rays,
-                     int64_t n, uint32_t flags, HitRec* hits, int threads) {
+                     int64_t n, uint32_t flags, HitRec* hits, int threads, int32_t sobb_kdop) {
     FinalTree t;
     t.nodes.resize(nnodes);
     std::memcpy(t.nodes.data(), fnodes, size_t(nnodes) * sizeof(FinalTree::Node));
     t.tri_ordered.assign(tri_ordered, tri_ordered + 12 * nprim);
+    if (sobb_kdop) {
+        t.sobb = true;
+        t.set = MakeSobbSet(sobb_kdop);
+    }
     TraceFinal(t, rays, n, flags, hits, threads);
 }
-double orc_sah_final(const void* fnodes, int64_t nnodes) {
+// SAH of a downloaded tree; root_box6 (lo xyz, hi xyz) is the tree's axis-aligned box.
+double orc_sah_final(const void* fnodes, int64_t nnodes, const float* root_box6, int32_t sobb_kdop) {
     FinalTree t;
     t.nodes.resize(nnodes);
     std::memcpy(t.nodes.data(), fnodes, size_t(nnodes) * sizeof(FinalTree::Node));
+    for (int a = 0; a < 3; a++) t.root_lo[a] = root_box6[a], t.root_hi[a] = root_box6[3 + a];
+    if (sobb_kdop) {
+        t.sobb = true;
+        t.set = MakeSobbSet(sobb_kdop);
+    }
     return SahCostFinal(t);
 }
 void orc_trace_brute(const float* tri, int64_t ntri, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits,

@@ -219,9 +219,24 @@ struct PNode {       // PLOC node (leaves 0..n-1 in sorted order, internal nodes
     int32_t left, right;  // for a leaf: left = original triangle index, right = -1
 };
 
+// SOBB direction set: unit normals and the non-coplanar direction triples with 1/|det|.
+struct SobbSet {
+    int ndirs = 0;
+    float dirs[13][3];
+    int ntriples = 0;
+    struct Triple {
+        int a, b, c;
+        float inv_det;
+    } triples[286];
+};
+SobbSet MakeSobbSet(int kdop_size);
+
 struct FinalTree {
     // 64-byte two-child node: child boxes + child references. A reference r >= 0 is an
     // inner node index; r < 0 is a leaf whose ~r packs (prim_begin << 4) | prim_count.
+    // AABB boxes: (lx0 lx1 ly0 ly1 | rx0 rx1 ry0 ry1 | lz0 lz1 rz0 rz1).
+    // SOBB boxes: 12 floats = left (lo_a hi_a lo_b hi_b lo_c hi_c), right (same); pad0
+    // packs the direction indices 4 bits each (left a,b,c bits 0-11, right bits 12-23).
     struct Node {
         float lx0, lx1, ly0, ly1;
         float rx0, rx1, ry0, ry1;
@@ -231,6 +246,9 @@ struct FinalTree {
     std::vector<Node> nodes;
     std::vector<int32_t> prim_order;  // leaf-order position -> original triangle
     std::vector<float> tri_ordered;   // 12 floats per prim: v0 xyz,id | v1 xyz,0 | v2 xyz,0
+    bool sobb = false;
+    SobbSet set;
+    float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};  // axis-aligned box of the whole tree
 };
 
 struct PlocResult {
@@ -244,8 +262,15 @@ struct PlocResult {
     std::vector<float> cost;            // collapse SAH cost per node
     std::vector<int32_t> count;         // triangles below each node
     std::vector<uint8_t> collapsed;     // node becomes a leaf
+    std::vector<float> kdop;            // SOBB builds: 2 * ndirs floats per node (lo..., hi...)
     FinalTree tree;
 };
+
+float KdopPad(const Bounds& b);
+void LeafKdop(const SobbSet& S, const Tri& t, float pad, float* kd);
+int BestTriple(const SobbSet& S, const float* kd, float& best_sa);
+void BuildSobbNodes(const SobbSet& S, const PlocResult& R, const std::vector<float>& kdop, int64_t n, int max_leaf,
+                    std::vector<FinalTree::Node>& out);
 
 void PlocBuild(const float* tri_xyz, int64_t n, const BuildParams& p, PlocResult& out);
 void PlocNearestNeighbours(const std::vector<PNode>& clusters_box, int c, int radius, std::vector<int32_t>& nn);

@@ -52,15 +52,17 @@ def _load():
         "orc_emc_layout": (None, [P, P, P]),
         "orc_ploc_nn": (None, [P, I32, I32, P]),
         "orc_sa_union": (F, [P, P, P, P]),
-        "orc_ploc_build": (P, [P, I64, I32, I32, I32, P]),
+        "orc_ploc_build": (P, [P, I64, I32, I32, I32, I32, I32, P]),
+        "orc_ploc_kdop": (I32, [P, P]),
+        "orc_sobb_set": (I32, [I32, P, P, P, P]),
         "orc_ploc_free": (None, [P]),
         "orc_ploc_iterations": (I32, [P]),
         "orc_ploc_final_nodes": (I64, [P]),
         "orc_ploc_export": (None, [P] * 14),
         "orc_ploc_sah": (D, [P]),
         "orc_ploc_trace": (None, [P, P, I64, U32, P, I32]),
-        "orc_trace_final": (None, [P, I64, P, I64, P, I64, U32, P, I32]),
-        "orc_sah_final": (D, [P, I64]),
+        "orc_trace_final": (None, [P, I64, P, I64, P, I64, U32, P, I32, I32]),
+        "orc_sah_final": (D, [P, I64, P, I32]),
         "orc_trace_brute": (None, [P, I64, P, I64, U32, P, I32]),
         "orc_pcg_stream": (None, [U64, U64, I32, P]),
         "orc_sincos2pi": (None, [F, P, P]),
@@ -156,19 +158,31 @@ def trace_brute(tris, rays, flags=0, threads=None) -> np.ndarray:
     return hits
 
 
-def trace_final(fnodes, tri_ordered, rays, flags=0, threads=None) -> np.ndarray:
+def trace_final(fnodes, tri_ordered, rays, flags=0, threads=None, sobb_kdop=0) -> np.ndarray:
+    """Traces a tree in the engine's final layout (e.g. downloaded from the device);
+    sobb_kdop: 0 for AABB child boxes, else the kdop_size of the SOBB direction set."""
     fn = np.ascontiguousarray(fnodes)
     to = _f32(tri_ordered)
     r = _f32(rays)
     hits = np.zeros(r.shape[0], HIT_DTYPE)
     lib.orc_trace_final(_p(fn), fn.shape[0], _p(to), to.size // 12, _p(r), r.shape[0], flags, _p(hits),
-                        threads or nthreads())
+                        threads or nthreads(), sobb_kdop)
     return hits
 
 
-def sah_final(fnodes) -> float:
+def sah_final(fnodes, root_box6, sobb_kdop=0) -> float:
     fn = np.ascontiguousarray(fnodes)
-    return float(lib.orc_sah_final(_p(fn), fn.shape[0]))
+    rb = _f32(root_box6)
+    return float(lib.orc_sah_final(_p(fn), fn.shape[0], _p(rb), sobb_kdop))
+
+
+def sobb_set(kdop_size=32):
+    nd = C.c_int32()
+    dirs = np.zeros((13, 3), np.float32)
+    abc = np.zeros((286, 3), np.int32)
+    inv = np.zeros(286, np.float32)
+    nt = lib.orc_sobb_set(kdop_size, C.byref(nd), _p(dirs), _p(abc), _p(inv))
+    return dirs[: nd.value], abc[:nt], inv[:nt]
 
 
 def pcg_stream(seed: int, stream: int, count: int) -> np.ndarray:
@@ -231,15 +245,22 @@ class Ploc:
     """Sequential restatement of the engine's PLOC build (codes, order, merges, collapse,
     final layout)."""
 
-    def __init__(self, tris, morton_type=0, radius=16, max_leaf=4):
+    def __init__(self, tris, morton_type=0, radius=16, max_leaf=4, bv_type=0, kdop_size=32):
         self.tris = _f32(tris)
         ms = C.c_double()
-        self.h = lib.orc_ploc_build(_p(self.tris), self.tris.shape[0], morton_type, radius, max_leaf, C.byref(ms))
+        self.h = lib.orc_ploc_build(_p(self.tris), self.tris.shape[0], morton_type, radius, max_leaf, bv_type,
+                                    kdop_size, C.byref(ms))
         if not self.h:
             raise ValueError("empty scene")
         self.build_ms = ms.value
+        self.sobb_kdop = kdop_size if bv_type == 1 else 0
         n = self.tris.shape[0]
         self.n = n
+        nd = lib.orc_ploc_kdop(self.h, None)
+        self.kdop = None
+        if nd:
+            self.kdop = np.zeros((2 * n - 1, 2 * nd), np.float32)
+            lib.orc_ploc_kdop(self.h, _p(self.kdop))
         it = lib.orc_ploc_iterations(self.h)
         nf = lib.orc_ploc_final_nodes(self.h)
         self.codes = np.zeros(n, np.uint64)

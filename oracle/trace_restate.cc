@@ -82,6 +82,35 @@ bool IntersectBoxRobust(const WtRay& w, float tmin, float tmax, const float lo[3
     return tn <= tf * 1.0000004f;
 }
 
+// SOBB slab test: per ray, n_k . o and 1 / (n_k . d) for every direction of the set, and
+// an absolute pad po = 4e-7 |o|_1 that covers the rounding of n_k . o.
+struct SobbRay {
+    float on[13], idn[13];
+    float po;
+};
+
+static SobbRay PrepareSobb(const SobbSet& S, const RayRec& r) {
+    SobbRay s;
+    for (int k = 0; k < S.ndirs; k++) {
+        const float* n = S.dirs[k];
+        s.on[k] = n[0] * r.ox + n[1] * r.oy + n[2] * r.oz;
+        s.idn[k] = 1.0f / (n[0] * r.dx + n[1] * r.dy + n[2] * r.dz);
+    }
+    s.po = 4e-7f * (std::fabs(r.ox) + std::fabs(r.oy) + std::fabs(r.oz));
+    return s;
+}
+
+static bool IntersectSobb(const SobbRay& s, float tmin, float tmax, const float* f, uint32_t dirs, float& tnear) {
+    const int a = dirs & 15, b = (dirs >> 4) & 15, c = (dirs >> 8) & 15;
+    const float ta0 = ((f[0] - s.po) - s.on[a]) * s.idn[a], ta1 = ((f[1] + s.po) - s.on[a]) * s.idn[a];
+    const float tb0 = ((f[2] - s.po) - s.on[b]) * s.idn[b], tb1 = ((f[3] + s.po) - s.on[b]) * s.idn[b];
+    const float tc0 = ((f[4] - s.po) - s.on[c]) * s.idn[c], tc1 = ((f[5] + s.po) - s.on[c]) * s.idn[c];
+    const float tn = std::fmax(std::fmax(std::fmin(ta0, ta1), std::fmin(tb0, tb1)), std::fmax(std::fmin(tc0, tc1), tmin));
+    const float tf = std::fmin(std::fmin(std::fmax(ta0, ta1), std::fmax(tb0, tb1)), std::fmin(std::fmax(tc0, tc1), tmax));
+    tnear = tn;
+    return tn <= tf * 1.0000004f;
+}
+
 static inline bool Better(float t, int32_t prim, float best_t, int32_t best_prim) {
     return t < best_t || (t == best_t && best_prim >= 0 && prim < best_prim);
 }
@@ -89,6 +118,8 @@ static inline bool Better(float t, int32_t prim, float best_t, int32_t best_prim
 static void TraceFinalOne(const FinalTree& T, const RayRec& r, uint32_t flags, HitRec& h) {
     const bool any = flags & 1u, cull = flags & 2u;
     WtRay w = PrepareRay(r);
+    SobbRay sr{};
+    if (T.sobb) sr = PrepareSobb(T.set, r);
     h = HitRec{0.0f, 0.0f, 0.0f, -1};
     float best_t = r.tmax;
     int32_t stack[256];
@@ -98,11 +129,18 @@ static void TraceFinalOne(const FinalTree& T, const RayRec& r, uint32_t flags, H
         int32_t ref = stack[--sp];
         if (ref >= 0) {
             const FinalTree::Node& nd = T.nodes[ref];
-            const float llo[3] = {nd.lx0, nd.ly0, nd.lz0}, lhi[3] = {nd.lx1, nd.ly1, nd.lz1};
-            const float rlo[3] = {nd.rx0, nd.ry0, nd.rz0}, rhi[3] = {nd.rx1, nd.ry1, nd.rz1};
             float tl, tr;
-            bool hl = IntersectBoxRobust(w, r.tmin, best_t, llo, lhi, tl);
-            bool hr = IntersectBoxRobust(w, r.tmin, best_t, rlo, rhi, tr);
+            bool hl, hr;
+            if (T.sobb) {
+                const uint32_t dirs = uint32_t(nd.pad0);
+                hl = IntersectSobb(sr, r.tmin, best_t, &nd.lx0, dirs & 0xFFFu, tl);
+                hr = IntersectSobb(sr, r.tmin, best_t, &nd.lx0 + 6, (dirs >> 12) & 0xFFFu, tr);
+            } else {
+                const float llo[3] = {nd.lx0, nd.ly0, nd.lz0}, lhi[3] = {nd.lx1, nd.ly1, nd.lz1};
+                const float rlo[3] = {nd.rx0, nd.ry0, nd.rz0}, rhi[3] = {nd.rx1, nd.ry1, nd.rz1};
+                hl = IntersectBoxRobust(w, r.tmin, best_t, llo, lhi, tl);
+                hr = IntersectBoxRobust(w, r.tmin, best_t, rlo, rhi, tr);
+            }
             if (hl && hr) {
                 if (tr < tl) {
                     stack[sp++] = nd.left;

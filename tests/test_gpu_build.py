@@ -22,21 +22,21 @@ def say(*a):
     sys.stdout.flush()
 
 
-def build(ctx, tris, mt=0, radius=16, leaf=4, block=0, digit=0, keep=True):
+def build(ctx, tris, mt=0, radius=16, leaf=4, block=0, digit=0, keep=True, bv=0, kdop=32):
     from diplodocus_amd import accel
 
     p = accel.make_params(morton_type=mt, radius=radius, max_leaf=leaf, block_size=block, sort_digit_bits=digit,
-                          keep_intermediates=keep)
+                          keep_intermediates=keep, bv_type=bv, kdop_size=kdop)
     if isinstance(tris, np.ndarray):
         tris = np.ascontiguousarray(tris, np.float32)
     return accel.GpuBvh(ctx).build(tris, p)
 
 
-def compare_to_oracle(oracle, ctx, tris, mt, radius, leaf, block=0, digit=0):
-    say(f"case n={len(tris)} mt={mt} r={radius} leaf={leaf} block={block} digit={digit}")
-    g = build(ctx, tris, mt, radius, leaf, block, digit)
+def compare_to_oracle(oracle, ctx, tris, mt, radius, leaf, block=0, digit=0, bv=0, kdop=32):
+    say(f"case n={len(tris)} mt={mt} r={radius} leaf={leaf} block={block} digit={digit} bv={bv} kdop={kdop}")
+    g = build(ctx, tris, mt, radius, leaf, block, digit, bv=bv, kdop=kdop)
     d = g.download(intermediates=True)
-    o = oracle.Ploc(tris, mt, radius, leaf)
+    o = oracle.Ploc(tris, mt, radius, leaf, bv_type=bv, kdop_size=kdop)
     assert (d["codes"] == o.codes).all(), "codes"
     assert (d["sorted_idx"] == o.sorted_idx).all(), "order"
     assert (d["iter_counts"] == o.iter_counts).all(), "iteration counts"
@@ -68,6 +68,17 @@ def test_build_bitexact_params(oracle, ctx, mt, radius, leaf, block, digit):
 def test_build_bitexact_workload_scenes(oracle, ctx, mt):
     compare_to_oracle(oracle, ctx, scenes.cornell_room().tris, mt, 16, 4)
     compare_to_oracle(oracle, ctx, scenes.icosphere_room().tris, mt, 16, 4)
+
+
+@pytest.mark.parametrize("n,mt,kdop", [(1, 0, 32), (3, 1, 32), (2049, 0, 32), (30000, 1, 32), (20000, 0, 18),
+                                       (20000, 1, 14), (5000, 0, 6)])
+def test_build_sobb_bitexact(oracle, ctx, n, mt, kdop):
+    compare_to_oracle(oracle, ctx, soup(n, n + kdop, size=0.3), mt, 16, 4, bv=1, kdop=kdop)
+
+
+def test_build_sobb_workload_scenes(oracle, ctx):
+    compare_to_oracle(oracle, ctx, scenes.icosphere_room().tris, 1, 16, 4, bv=1)
+    compare_to_oracle(oracle, ctx, scenes.mixed_2m(n_small=60_000, n_thin=8_000).tris, 1, 8, 4, bv=1)
 
 
 def test_build_degenerate_inputs(oracle, ctx):

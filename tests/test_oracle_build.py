@@ -249,6 +249,45 @@ def test_final_layout_sah_closed_form(oracle):
     assert p.sah() == pytest.approx((root + sa(la, ha) + sa(lb, hb)) / root, rel=1e-12)
 
 
+def test_sobb_direction_set(oracle):
+    dirs, abc, inv = oracle.sobb_set(32)
+    assert len(dirs) == 13 and np.allclose(np.linalg.norm(dirs.astype(np.float64), axis=1), 1, atol=1e-7)
+    assert list(abc[0]) == [0, 1, 2] and inv[0] == 1.0  # the axis-aligned box comes first
+    for (a, b, c), iv in zip(abc, inv):
+        det = abs(np.linalg.det(dirs[[a, b, c]].astype(np.float64)))
+        assert det >= 1e-3 and iv == np.float32(1.0 / det)
+    assert len(oracle.sobb_set(6)[1]) == 1 and len(oracle.sobb_set(18)[0]) == 9 and len(oracle.sobb_set(14)[0]) == 7
+
+
+def test_sobb_kdops_contain_their_triangles_in_fp64(oracle):
+    tris = soup(3000, 21, size=0.4)
+    p = oracle.Ploc(tris, 1, 16, 4, bv_type=1, kdop_size=32)
+    dirs, _, _ = oracle.sobb_set(32)
+    nd = len(dirs)
+    n = len(tris)
+    v = tris.reshape(n, 3, 3).astype(np.float64)[p.sorted_idx]  # leaf k = sorted triangle k
+    proj = np.einsum("kd,nvd->nvk", dirs.astype(np.float64), v)  # exact-ish projections
+    kd = p.kdop.astype(np.float64)
+    assert (proj.min(1) >= kd[:n, :nd]).all() and (proj.max(1) <= kd[:n, nd:]).all()
+    # unions are exact: every inner k-DOP is the min/max of its children's
+    nodes = p.nodes[n:]
+    L, R = p.kdop[nodes["left"]], p.kdop[nodes["right"]]
+    assert (p.kdop[n:, :nd] == np.minimum(L[:, :nd], R[:, :nd])).all()
+    assert (p.kdop[n:, nd:] == np.maximum(L[:, nd:], R[:, nd:])).all()
+
+
+def test_sobb_tree_traces_like_brute_force_and_lowers_sah(oracle):
+    s = scenes.mixed_2m(n_small=30_000, n_thin=5_000)
+    rays = scenes.random_rays_towards(s.tris, 3000, seed=4)
+    brute = oracle.trace_brute(s.tris, rays)
+    aabb = oracle.Ploc(s.tris, 1, 16, 4)
+    for kdop in (32, 18, 14, 6):
+        p = oracle.Ploc(s.tris, 1, 16, 4, bv_type=1, kdop_size=kdop)
+        assert (p.trace(rays) == brute).all()
+        if kdop > 6:
+            assert p.sah() < aabb.sah()
+
+
 @pytest.mark.parametrize("scene", ["cornell", "icosphere"])
 def test_ploc_on_workload_scenes(oracle, scene):
     s = scenes.cornell_room() if scene == "cornell" else scenes.icosphere_room()
