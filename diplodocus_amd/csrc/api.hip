@@ -230,6 +230,7 @@ dpl_build_params Normalize(const dpl_build_params* p) {
             "max_triangles_per_leaf must be in [1, 15]");
     Require(q.block_size == 128 || q.block_size == 256 || q.block_size == 512, "block_size must be 128, 256 or 512");
     Require(q.sort_digit_bits == 4 || q.sort_digit_bits == 8, "sort_digit_bits must be 4 or 8");
+    Require(q.kdop_size >= 6, "kdop_size must be at least 6");
     return q;
 }
 
@@ -308,7 +309,20 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
         b.ntriples = S.ntriples;
         b.dirs = reinterpret_cast<const float*>(sd);
         b.triples = reinterpret_cast<const int4*>(sd + sizeof(S.dirs));
-        b.kdop = C.kdop.ensure<float>(size_t(nn) * 2 * S.ndirs * 4);
+        if (S.ndirs == 13) {  // the 26-DOP fit enumerates its triples at compile time
+            bool same = S.ntriples == SobbTriples13();
+            for (int t = 0; same && t < S.ntriples; t++) {
+                int a, c, d;
+                float inv;
+                SobbTriple13(t, &a, &c, &d, &inv);
+                int32_t bits;
+                std::memcpy(&bits, &inv, 4);
+                same = a == S.triples[t].x && c == S.triples[t].y && d == S.triples[t].z && bits == S.triples[t].w;
+            }
+            if (!same) throw Error(DPL_ERR_INTERNAL, "26-DOP triple table mismatch");
+        }
+        // non-axis slabs only (the axis slabs are the node box -/+ pad)
+        b.kdop = C.kdop.ensure<float>(std::max<size_t>(16, size_t(nn) * 2 * (S.ndirs - 3) * 4));
         if (bvh->sobb_kdop != p.kdop_size) {
             float* bd = bvh->sobb_dirs.ensure<float>(sizeof(S.dirs));
             DPL_HIP(hipMemcpy(bd, S.dirs, sizeof(S.dirs), hipMemcpyHostToDevice));

@@ -335,64 +335,165 @@ __device__ __forceinline__ void StoreEval(const BuildBuffers& b, int32_t id, con
 // SOBB support: k-DOP projections (n . v, FP32, left to right) padded by 1e-6 x the largest
 // scene coordinate, exact unions, and the direction triple of least parallelepiped area
 // (w_a w_b + w_b w_c + w_a w_c) / |det| per child box (ties: first triple).
+//
+// Stored k-DOP per node: only the non-axis directions 3..ndirs-1 (the axis slabs are the
+// node's box -/+ pad, exactly: an axis projection is the coordinate itself and rounding
+// of x - pad is monotone), as [lo_3 .. lo_nd-1, -hi_3 .. -hi_nd-1] so that a union is one
+// element-wise min over 2 (ndirs - 3) floats (20 for the 13-direction set, a multiple of 4).
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ float KdopPad(const uint32_t* bounds) {
     const DecodedBounds d = Decode(bounds);
     float s = 0.0f;
     for (int a = 0; a < 3; a++) s = Max2(s, Max2(fabsf(d.vmin[a]), fabsf(d.vmax[a])));
-    return 1e-6f * s;
+    return Max2(1e-6f * s, 1.40129846e-45f);  // never zero: x - pad is monotone and sign-exact
 }
 
+__device__ __forceinline__ int KdopWidth(int ndirs) { return 2 * (ndirs - 3); }
+
 __device__ __forceinline__ void LeafKdop(const BuildBuffers& b, const float v[9], float pad, float* kd) {
-    const int nd = b.ndirs;
-    for (int k = 0; k < nd; k++) {
-        const float nx = b.dirs[3 * k], ny = b.dirs[3 * k + 1], nz = b.dirs[3 * k + 2];
-        const float p0 = nx * v[0] + ny * v[1] + nz * v[2];
-        const float p1 = nx * v[3] + ny * v[4] + nz * v[5];
-        const float p2 = nx * v[6] + ny * v[7] + nz * v[8];
+    const int nx = b.ndirs - 3;
+    for (int k = 0; k < nx; k++) {
+        const float* n = b.dirs + 3 * (3 + k);
+        const float nx_ = n[0], ny = n[1], nz = n[2];
+        const float p0 = nx_ * v[0] + ny * v[1] + nz * v[2];
+        const float p1 = nx_ * v[3] + ny * v[4] + nz * v[5];
+        const float p2 = nx_ * v[6] + ny * v[7] + nz * v[8];
         kd[k] = Min2(Min2(p0, p1), p2) - pad;
-        kd[nd + k] = Max2(Max2(p0, p1), p2) + pad;
+        kd[nx + k] = -(Max2(Max2(p0, p1), p2) + pad);
     }
 }
 
 __device__ __forceinline__ void UnionKdop(const BuildBuffers& b, int32_t id, int32_t l, int32_t r) {
-    const int nd = b.ndirs, w = 2 * nd;
-    const float* kl = b.kdop + int64_t(l) * w;
-    const float* kr = b.kdop + int64_t(r) * w;
-    float* ko = b.kdop + int64_t(id) * w;
-    for (int k = 0; k < nd; k++) {
-        ko[k] = Min2(kl[k], kr[k]);
-        ko[nd + k] = Max2(kl[nd + k], kr[nd + k]);
+    const int w4 = KdopWidth(b.ndirs) / 4;
+    const float4* kl = reinterpret_cast<const float4*>(b.kdop) + int64_t(l) * w4;
+    const float4* kr = reinterpret_cast<const float4*>(b.kdop) + int64_t(r) * w4;
+    float4* ko = reinterpret_cast<float4*>(b.kdop) + int64_t(id) * w4;
+    for (int k = 0; k < w4; k++) {
+        const float4 x = kl[k], y = kr[k];
+        ko[k] = make_float4(Min2(x.x, y.x), Min2(x.y, y.y), Min2(x.z, y.z), Min2(x.w, y.w));
     }
 }
 
-// Writes the child's SOBB (lo_a hi_a lo_b hi_b lo_c hi_c) to f and returns a | b<<4 | c<<8.
-__device__ __forceinline__ uint32_t FitSobb(const BuildBuffers& b, const int4* triples, const float* kd, float* f) {
-    const int nd = b.ndirs;
-    float w[kMaxDirs];
-    for (int k = 0; k < kMaxDirs; k++) w[k] = k < nd ? kd[nd + k] - kd[k] : 0.0f;
-    int bt = 0;
-    float best = 0.0f;
-    for (int t = 0; t < b.ntriples; t++) {
-        const int4 T = triples[t];
-        float wa = 0.0f, wb = 0.0f, wc = 0.0f;
-#pragma unroll
-        for (int k = 0; k < kMaxDirs; k++) {  // register-array lookup without scratch
-            wa = k == T.x ? w[k] : wa;
-            wb = k == T.y ? w[k] : wb;
-            wc = k == T.z ? w[k] : wc;
-        }
-        const float sa = (wa * wb + wb * wc + wa * wc) * __int_as_float(T.w);
-        if (t == 0 || sa < best) {
+// The 13 unit normals of the 26-DOP (axes, face diagonals, corner diagonals) and, at compile
+// time, the non-coplanar triples among them in lexicographic order (|det| >= 1e-3, FP64),
+// identical to the run-time table the host uploads (checked when it is made).
+constexpr float kS2 = 0.70710677f, kS3 = 0.57735026f;
+constexpr float kDirs13[13][3] = {{1, 0, 0},     {0, 1, 0},       {0, 0, 1},       {kS2, kS2, 0},    {kS2, -kS2, 0},
+                                  {kS2, 0, kS2}, {kS2, 0, -kS2},  {0, kS2, kS2},   {0, kS2, -kS2},   {kS3, kS3, kS3},
+                                  {kS3, kS3, -kS3}, {kS3, -kS3, kS3}, {-kS3, kS3, kS3}};
+struct TripleTable13 {
+    int n;
+    int8_t a[kMaxTriples], b[kMaxTriples], c[kMaxTriples];
+    float inv_det[kMaxTriples];  // float(1 / |det|), the division in FP64 rounded once
+};
+constexpr TripleTable13 MakeTripleTable13() {
+    TripleTable13 t{};
+    for (int a = 0; a < 13; a++)
+        for (int b = a + 1; b < 13; b++)
+            for (int c = b + 1; c < 13; c++) {
+                const double n1[3] = {kDirs13[a][0], kDirs13[a][1], kDirs13[a][2]};
+                const double n2[3] = {kDirs13[b][0], kDirs13[b][1], kDirs13[b][2]};
+                const double n3[3] = {kDirs13[c][0], kDirs13[c][1], kDirs13[c][2]};
+                const double cx = n2[1] * n3[2] - n2[2] * n3[1];
+                const double cy = n2[2] * n3[0] - n2[0] * n3[2];
+                const double cz = n2[0] * n3[1] - n2[1] * n3[0];
+                const double det = n1[0] * cx + n1[1] * cy + n1[2] * cz;
+                const double ad = det < 0 ? -det : det;
+                if (ad < 1e-3) continue;
+                t.a[t.n] = int8_t(a), t.b[t.n] = int8_t(b), t.c[t.n] = int8_t(c);
+                t.inv_det[t.n] = float(1.0 / ad);
+                t.n++;
+            }
+    return t;
+}
+constexpr TripleTable13 kTriples13 = MakeTripleTable13();
+int SobbTriples13() { return kTriples13.n; }
+void SobbTriple13(int t, int* a, int* b, int* c, float* inv_det) {
+    *a = kTriples13.a[t], *b = kTriples13.b[t], *c = kTriples13.c[t], *inv_det = kTriples13.inv_det[t];
+}
+
+template <int T>
+__device__ __forceinline__ void ScanTriples13(const float* w, float& best, int& bt) {
+    if constexpr (T < kTriples13.n) {
+        constexpr int a = kTriples13.a[T], b = kTriples13.b[T], c = kTriples13.c[T];
+        constexpr float inv_det = kTriples13.inv_det[T];
+        const float sa = (w[a] * w[b] + w[b] * w[c] + w[a] * w[c]) * inv_det;
+        if (T == 0 || sa < best) {
             best = sa;
-            bt = t;
+            bt = T;
+        }
+        ScanTriples13<T + 1>(w, best, bt);
+    }
+}
+
+// Selects arr[idx] for a run-time idx without indexing a private array.
+template <int N>
+__device__ __forceinline__ float SelDir(const float (&arr)[N], int idx) {
+    float r = arr[0];
+#pragma unroll
+    for (int k = 1; k < N; k++) r = idx == k ? arr[k] : r;
+    return r;
+}
+
+// Writes the SOBB (lo_a hi_a lo_b hi_b lo_c hi_c) of node `id` to f and returns a | b<<4 | c<<8.
+// ND is the size of the direction set (3, 7, 9 or 13).
+template <int ND>
+__device__ __forceinline__ uint32_t FitSobbT(const BuildBuffers& b, const int4* triples, int32_t id, float pad,
+                                             float* f) {
+    constexpr int NX = ND - 3;
+    const PNode nd = b.nodes[id];
+    float lo[ND], hi[ND], w[ND];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        lo[a] = nd.lo[a] - pad;
+        hi[a] = nd.hi[a] + pad;
+    }
+    if constexpr (NX > 0) {
+        const float4* kd = reinterpret_cast<const float4*>(b.kdop) + int64_t(id) * (NX / 2);
+        float s[2 * NX];
+#pragma unroll
+        for (int q = 0; q < NX / 2; q++) {
+            const float4 v = kd[q];
+            s[4 * q] = v.x, s[4 * q + 1] = v.y, s[4 * q + 2] = v.z, s[4 * q + 3] = v.w;
+        }
+#pragma unroll
+        for (int k = 0; k < NX; k++) {
+            lo[3 + k] = s[k];
+            hi[3 + k] = -s[NX + k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < ND; k++) w[k] = hi[k] - lo[k];
+    float best = 0.0f;
+    int bt = 0;
+    if constexpr (ND == 13) {
+        ScanTriples13<0>(w, best, bt);
+    } else {
+        for (int t = 0; t < b.ntriples; t++) {
+            const int4 T = triples[t];
+            const float wa = SelDir(w, T.x), wb = SelDir(w, T.y), wc = SelDir(w, T.z);
+            const float sa = (wa * wb + wb * wc + wa * wc) * __int_as_float(T.w);
+            if (t == 0 || sa < best) {
+                best = sa;
+                bt = t;
+            }
         }
     }
     const int4 T = triples[bt];
-    f[0] = kd[T.x], f[1] = kd[nd + T.x];
-    f[2] = kd[T.y], f[3] = kd[nd + T.y];
-    f[4] = kd[T.z], f[5] = kd[nd + T.z];
+    f[0] = SelDir(lo, T.x), f[1] = SelDir(hi, T.x);
+    f[2] = SelDir(lo, T.y), f[3] = SelDir(hi, T.y);
+    f[4] = SelDir(lo, T.z), f[5] = SelDir(hi, T.z);
     return uint32_t(T.x) | (uint32_t(T.y) << 4) | (uint32_t(T.z) << 8);
+}
+
+__device__ __forceinline__ uint32_t FitSobb(const BuildBuffers& b, const int4* triples, int32_t id, float pad,
+                                            float* f) {
+    switch (b.ndirs) {
+        case 13: return FitSobbT<13>(b, triples, id, pad, f);
+        case 9: return FitSobbT<9>(b, triples, id, pad, f);
+        case 7: return FitSobbT<7>(b, triples, id, pad, f);
+        default: return FitSobbT<3>(b, triples, id, pad, f);
+    }
 }
 
 __global__ __launch_bounds__(256) void KInitClusters(BuildBuffers b) {
@@ -408,7 +509,7 @@ __global__ __launch_bounds__(256) void KInitClusters(BuildBuffers b) {
         nd.right = -1;
         b.nodes[k] = nd;
         StoreEval(b, int32_t(k), LeafEval(nd.lo, nd.hi, int32_t(k)));
-        if (b.ndirs > 0) LeafKdop(b, v, pad, b.kdop + k * 2 * b.ndirs);
+        if (b.ndirs > 3) LeafKdop(b, v, pad, b.kdop + k * KdopWidth(b.ndirs));
         nd.left = int32_t(k);
         nd.right = 0;
         b.clusters[0][k] = nd;
@@ -617,7 +718,7 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
                 b.parent[o.left] = id;
                 b.parent[o.right] = id;
                 StoreEval(b, id, InnerEval(o.lo, o.hi, LoadEval(b, o.left), LoadEval(b, o.right), max_leaf));
-                if (b.ndirs > 0) UnionKdop(b, id, o.left, o.right);
+                if (b.ndirs > 3) UnionKdop(b, id, o.left, o.right);
                 o.left = id;
                 o.right = 0;
             } else {
@@ -732,7 +833,7 @@ __global__ __launch_bounds__(kTailThreads) void KPlocTail(BuildBuffers b, int it
                 const NodeEval ej{s_cnt[j], s_cost[j], s_ml[j], s_ht[j], 0};
                 keep_e[q] = InnerEval(o.lo, o.hi, ei, ej, max_leaf);
                 StoreEval(b, id, keep_e[q]);
-                if (b.ndirs > 0) UnionKdop(b, id, o.left, o.right);
+                if (b.ndirs > 3) UnionKdop(b, id, o.left, o.right);
                 o.left = id;
             } else {
                 o.lo[0] = s_lx[i], o.lo[1] = s_ly[i], o.lo[2] = s_lz[i];
@@ -829,7 +930,7 @@ __global__ __launch_bounds__(256) void KEmit(BuildBuffers b, int max_leaf) {
         for (int t = threadIdx.x; t < b.ntriples; t += blockDim.x) s_tri[t] = b.triples[t];
         __syncthreads();
     }
-    const int64_t kw = 2 * int64_t(b.ndirs);
+    const float pad = b.ndirs > 0 ? KdopPad(b.bounds) : 0.0f;
     for (int64_t id = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; id < nn; id += int64_t(gridDim.x) * blockDim.x) {
         const bool col = b.collapsed[id] != 0;
         const bool in = Inside(b, int32_t(id), max_leaf);
@@ -868,30 +969,40 @@ __global__ __launch_bounds__(256) void KEmit(BuildBuffers b, int max_leaf) {
                 f.right = ~0;
                 f.pad0 = f.pad1 = 0;
                 if (b.ndirs > 0) {
+                    float g[6];
+                    const uint32_t d = FitSobb(b, s_tri, int32_t(id), pad, g);
                     float* fl = &f.lx0;
-                    const uint32_t d = FitSobb(b, s_tri, b.kdop + id * kw, fl);
-                    for (int k = 0; k < 6; k++) fl[6 + k] = fl[k];
+                    for (int k = 0; k < 6; k++) fl[k] = fl[6 + k] = g[k];
                     f.pad0 = int32_t(d | (d << 12));
                 }
                 b.fnodes[0] = f;
             }
         } else if (id >= n && !col && !in) {
             const PNode nd = b.nodes[id];
-            const PNode L = b.nodes[nd.left], R = b.nodes[nd.right];
-            FNode f;
-            f.lx0 = L.lo[0], f.lx1 = L.hi[0], f.ly0 = L.lo[1], f.ly1 = L.hi[1];
-            f.rx0 = R.lo[0], f.rx1 = R.hi[0], f.ry0 = R.lo[1], f.ry1 = R.hi[1];
-            f.lz0 = L.lo[2], f.lz1 = L.hi[2], f.rz0 = R.lo[2], f.rz1 = R.hi[2];
-            f.left = kept(nd.left) ? K - 1 - rank[nd.left - n] : leaf_ref(nd.left);
-            f.right = kept(nd.right) ? K - 1 - rank[nd.right - n] : leaf_ref(nd.right);
-            f.pad0 = f.pad1 = 0;
+            const int32_t left = kept(nd.left) ? K - 1 - rank[nd.left - n] : leaf_ref(nd.left);
+            const int32_t right = kept(nd.right) ? K - 1 - rank[nd.right - n] : leaf_ref(nd.right);
+            FNode* out = b.fnodes + (K - 1 - rank[id - n]);
             if (b.ndirs > 0) {
-                float* fl = &f.lx0;
-                const uint32_t dl = FitSobb(b, s_tri, b.kdop + int64_t(nd.left) * kw, fl);
-                const uint32_t dr = FitSobb(b, s_tri, b.kdop + int64_t(nd.right) * kw, fl + 6);
-                f.pad0 = int32_t(dl | (dr << 12));
+                // one child at a time: the 26-DOP triple scan keeps its pair products in registers
+                uint32_t dirs = 0;
+#pragma nounroll
+                for (int c = 0; c < 2; c++) {
+                    float g[6];
+                    dirs |= FitSobb(b, s_tri, c ? nd.right : nd.left, pad, g) << (12 * c);
+                    float2* o = reinterpret_cast<float2*>(reinterpret_cast<float*>(out) + 6 * c);
+                    o[0] = make_float2(g[0], g[1]), o[1] = make_float2(g[2], g[3]), o[2] = make_float2(g[4], g[5]);
+                }
+                reinterpret_cast<int4*>(out)[3] = make_int4(left, right, int32_t(dirs), 0);
+            } else {
+                const PNode L = b.nodes[nd.left], R = b.nodes[nd.right];
+                FNode f;
+                f.lx0 = L.lo[0], f.lx1 = L.hi[0], f.ly0 = L.lo[1], f.ly1 = L.hi[1];
+                f.rx0 = R.lo[0], f.rx1 = R.hi[0], f.ry0 = R.lo[1], f.ry1 = R.hi[1];
+                f.lz0 = L.lo[2], f.lz1 = L.hi[2], f.rz0 = R.lo[2], f.rz1 = R.hi[2];
+                f.left = left, f.right = right;
+                f.pad0 = f.pad1 = 0;
+                *out = f;
             }
-            b.fnodes[K - 1 - rank[id - n]] = f;
         }
     }
 }

@@ -220,6 +220,9 @@ struct TraceTree {
 
 constexpr int kMaxDirs = 13;
 constexpr int kMaxTriples = 286;
+// The compile-time triple table of the 13-direction (26-DOP) set, for a host-side check.
+int SobbTriples13();
+void SobbTriple13(int t, int* a, int* b, int* c, float* inv_det);
 
 // counters (nullable, device): accumulates {inner nodes visited, triangle tests}.
 hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,

@@ -487,7 +487,7 @@ double SahCostFinal(const FinalTree& t, double ct, double ci) {
     if (!(root > 0)) return 0.0;
     auto area = [&](const FinalTree::Node& nd, int s) { return t.sobb ? SobbArea(t, nd, s) : AabbArea(nd, s); };
     const FinalTree::Node& r = t.nodes[0];
-    if (r.right == ~0 && r.left < 0) return ci * area(r, 0) * ((~r.left) & 15) / root;
+    if (r.right == ~0 && r.left < 0) return ci * ((~r.left) & 15);  // one leaf: Ci * |leaf| * SA / SA
     double sum = ct * root;
     for (const auto& nd : t.nodes) {
         for (int s = 0; s < 2; s++) {

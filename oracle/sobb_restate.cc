@@ -62,7 +62,8 @@ static inline float Proj(const float* n, const float* v) { return n[0] * v[0] + 
 float KdopPad(const Bounds& b) {
     float s = 0.0f;
     for (int a = 0; a < 3; a++) s = Max2(s, Max2(std::fabs(b.vmin[a]), std::fabs(b.vmax[a])));
-    return 1e-6f * s;
+    // never zero, so that x - pad and x + pad are exact in sign and monotone in x
+    return Max2(1e-6f * s, 1.40129846e-45f);
 }
 
 void LeafKdop(const SobbSet& S, const Tri& t, float pad, float* kd /*2*ndirs: lo..., hi...*/) {

@@ -156,10 +156,10 @@ def test_build_error_paths(ctx):
         with pytest.raises(accel.DplError) as e:
             build(ctx, tris, **kw)
         assert e.value.status == 1
-    p = accel.make_params(bv_type=1)
-    with pytest.raises(accel.DplError) as e:
-        accel.GpuBvh(ctx).build(tris, p)
-    assert e.value.status == 4
+    for kw in [dict(bv_type=2), dict(morton_type=2), dict(kdop_size=-1)]:
+        with pytest.raises(accel.DplError) as e:
+            accel.GpuBvh(ctx).build(tris, accel.make_params(**kw))
+        assert e.value.status == 1
     with pytest.raises(accel.DplError):
         accel.GpuBvh(ctx).build(np.zeros((0, 9), np.float32), accel.make_params())
     # the context still works after errors

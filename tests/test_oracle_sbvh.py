@@ -54,7 +54,7 @@ def test_fixed_child_bounds_enclose_triangles(oracle):
 
 
 def test_faithful_child_bounds_defect_is_reproduced(oracle):
-    # Appendix-A defect: bins use k1 = B(1-1e-3)/S but the partition plane is
+    # The reference's child-bounds defect: bins use k1 = B(1-1e-3)/S but the partition plane is
     # k0 + (S/B)(i+1) (sbvh.cc:213-218 vs :258,:289); faithful leaf boxes can miss triangles.
     tris = _random_soup(20000, 2)
     mm, bc, ti = oracle.SBvh(tris, fix_child_bounds=False).export()
