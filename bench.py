@@ -45,6 +45,8 @@ def main() -> int:
     ap.add_argument("--no-sweep", action="store_true", help="skip the variant x radius sweep")
     ap.add_argument("--radius", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run three plain 256 MiB device copies (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box has one GPU)",
@@ -145,6 +147,43 @@ def main() -> int:
     _, pv, pt = rb.trace_counters(cam_rays)
     log(f"primary rays: {prim_ms:.3f} ms per 1M rays = {prim_mrays:.1f} Mrays/s, {pv / cam_rays.shape[0]:.1f} "
         f"nodes and {pt / cam_rays.shape[0]:.1f} tris per ray")
+
+    # ---- path tracing on the same room: 1024x1024 primary + 1 diffuse bounce, and the
+    #      differentiable case (512x512, 4 spp, depth 4): forward and adjoint path replay ----
+    from diplodocus_amd import render
+
+    mat_id = torch.from_numpy(room.mat_id).to(dev)
+    albedo = torch.from_numpy(room.albedo).to(dev)
+    emission = torch.from_numpy(room.emission).to(dev)
+
+    def timed(fn, reps):
+        fn()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for _ in range(reps):
+            fn()
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
+
+    rp_b = render.make_render_params(1024, 1024, 1, 2, 1, room.camera, len(room.albedo))
+    img_b = torch.empty((1024, 1024, 3), dtype=torch.float32, device=dev)
+    bounce_ms = timed(lambda: render.render_forward(rb, rp_b, mat_id, albedo, emission, img_b), 10)
+    rp = render.make_render_params(512, 512, 4, 4, 1, room.camera, len(room.albedo))
+    gen = torch.Generator().manual_seed(5)
+    target_albedo = (albedo.cpu() + 0.05 * torch.randn(albedo.shape, generator=gen)).clamp(0, 1).to(dev)
+    target = render.render_forward(rb, rp, mat_id, target_albedo, emission)
+    img = torch.empty((512, 512, 3), dtype=torch.float32, device=dev)
+    fwd_ms = timed(lambda: render.render_forward(rb, rp, mat_id, albedo, emission, img), 10)
+    d_img = (2.0 * (img - target)).contiguous()
+    adj_ms = timed(lambda: render.render_adjoint(rb, rp, mat_id, albedo, emission, d_img), 10)
+    diff = {"workload": "icosphere_room 69,668 tris, 16 materials, 512x512 x 4 spp, depth 4, L2 loss vs albedo "
+                        "+ N(0, 0.05) target; gradients for 16x3 albedo and emission",
+            "forward_ms": round(fwd_ms, 4), "adjoint_ms": round(adj_ms, 4),
+            "fwd_plus_adjoint_ms": round(fwd_ms + adj_ms, 4),
+            "adjoint_over_forward": round(adj_ms / fwd_ms, 3)}
+    log(f"path tracing: 1024x1024 + 1 bounce {bounce_ms:.3f} ms; 512x512x4 depth 4 forward {fwd_ms:.3f} ms, "
+        f"adjoint {adj_ms:.3f} ms ({adj_ms / fwd_ms:.2f}x)")
     rb.close()
 
     # ---- variant x radius sweep of the 2M scene (build time, SAH, trace rate) ----
@@ -224,6 +263,8 @@ def main() -> int:
         "trace_mrays_s": round(mrays, 3),
         "primary_mrays_s": round(prim_mrays, 3),
         "primary_workload": "icosphere_room 69,668 tris, 1024x1024 camera rays, PLOC Morton30 r=16",
+        "primary_bounce_ms": round(bounce_ms, 4),
+        "differentiable": diff,
         "build_ms": round(float(build_ms.mean()), 4),
         "trace_ms": round(float(trace_ms.mean()), 4),
         "sah_cost": last["sah_cost"],
@@ -240,6 +281,13 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.calibrate_copy:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 3 x 268,435,456-byte device-to-device copies")
     bvh.close()
     ctx.close()
     return 0

@@ -253,8 +253,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     hipStream_t s = ctx->stream;
     dpl_context& C = *ctx;
     const int64_t nn = 2 * n - 1;
-    const int chunk = PlocChunk(p.block_size);
-    const int64_t tiles0 = (n + chunk - 1) / chunk;
+    const int64_t status_words = PlocStatusWords(n, p.block_size);
     const bool key64 = p.morton_type == DPL_EMC60;
     const int key_bits = key64 ? 60 : 30;
 
@@ -273,8 +272,8 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     b.clusters[0] = C.clusters0.ensure<PNode>(size_t(n) * sizeof(PNode));
     b.clusters[1] = C.clusters1.ensure<PNode>(size_t(n) * sizeof(PNode));
     b.ccount = C.ccount.ensure<int32_t>(size_t(n + 2) * 4);
-    b.status[0] = C.status0.ensure<uint64_t>(size_t(tiles0 + 1) * 8);
-    b.status[1] = C.status1.ensure<uint64_t>(size_t(tiles0 + 1) * 8);
+    b.status[0] = C.status0.ensure<uint64_t>(size_t(status_words) * 8);
+    b.status[1] = C.status1.ensure<uint64_t>(size_t(status_words) * 8);
     {
         uint8_t* sm = C.small.ensure<uint8_t>(1024);
         b.bounds = reinterpret_cast<uint32_t*>(sm);              // 13 words
@@ -343,7 +342,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     }
 
     DPL_HIP(hipEventRecord(C.ev[0], s));
-    DPL_HIP(LaunchBuildInit(b, chunk, s));
+    DPL_HIP(LaunchBuildInit(b, p.block_size, s));
     DPL_HIP(LaunchBounds(b, s));
     DPL_HIP(hipEventRecord(C.ev[1], s));
     DPL_HIP(LaunchCodes(b, p.morton_type, s));
@@ -379,11 +378,11 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     while (c_known > tail) {
         int est = int(std::ceil(std::log(double(c_known) / tail) / std::log(rate)));
         est = std::max(1, std::min(est, 96));
-        const int64_t T = (c_known + chunk - 1) / chunk;
+        const int64_t T = (c_known + p.block_size - 1) / p.block_size;  // tiles at one cluster per thread
         for (int g = 0; g < est; g++) {
             if (time_iters) DPL_HIP(hipEventRecord(C.IterEvent(2 * size_t(iter)), s));
             DPL_HIP(LaunchPlocIteration(b, iter, p.nn_search_radius, p.block_size, tail, p.max_triangles_per_leaf,
-                                        int(std::min<int64_t>(T, cap)), s));
+                                        int(std::min<int64_t>(T, cap)), cap, s));
             if (time_iters) DPL_HIP(hipEventRecord(C.IterEvent(2 * size_t(iter) + 1), s));
             iter++;
         }
@@ -429,11 +428,13 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
         if (time_iters) {
             // the first `multi` launches merged; per launch: read c_k clusters (32 B), write
             // c_k+1 survivors (32 B), per merge a node (32 B), two parents (8 B), its
-            // collapse record (17 B) and the two children's records (34 B)
+            // collapse record (17 B) and the two children's records (34 B); with SOBB bounds
+            // also the two children's stored k-DOP slabs and the new node's (3 x 8 (ndirs - 3) B)
+            const double per_merge = 91.0 + (b.ndirs > 3 ? 3.0 * 8.0 * (b.ndirs - 3) : 0.0);
             for (int k = 0; k < multi; k++) {
                 st.ploc_launch_ms += ElapsedMs(C.iter_ev[2 * k], C.iter_ev[2 * k + 1]);
                 const double ck = cc[k], ck1 = cc[k + 1];
-                st.ploc_bytes += 32.0 * ck + 32.0 * ck1 + 91.0 * (ck - ck1);
+                st.ploc_bytes += 32.0 * ck + 32.0 * ck1 + per_merge * (ck - ck1);
             }
         }
     }
@@ -726,7 +727,7 @@ dpl_status dpl_compute_codes(dpl_context* ctx, const float* tri, int64_t n, int3
         b.sah = reinterpret_cast<double*>(sm + 192);
         b.ccount = ctx->ccount.ensure<int32_t>(size_t(n + 2) * 4);
         b.parent = ctx->parent.ensure<int32_t>(size_t(2 * n - 1) * 4);
-        b.status[0] = ctx->status0.ensure<uint64_t>(size_t(n / 256 + 2) * 8);
+        b.status[0] = ctx->status0.ensure<uint64_t>(size_t(PlocStatusWords(n, 256)) * 8);
         b.leafslot = ctx->leafslot.ensure<int32_t>(size_t(n) * 4);
         DPL_HIP(LaunchBuildInit(b, 256, s));
         DPL_HIP(LaunchBounds(b, s));

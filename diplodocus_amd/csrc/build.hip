@@ -94,8 +94,8 @@ __global__ void KBuildInit(BuildBuffers b, int tiles0) {
     for (int64_t k = t; k < b.n; k += int64_t(gridDim.x) * blockDim.x) b.leafslot[k] = 0;
 }
 
-hipError_t LaunchBuildInit(const BuildBuffers& b, int chunk, hipStream_t s) {
-    int64_t tiles0 = (b.n + chunk - 1) / chunk;
+hipError_t LaunchBuildInit(const BuildBuffers& b, int block, hipStream_t s) {
+    const int64_t tiles0 = PlocStatusWords(b.n, block);
     int grid = int(std::min<int64_t>(1024, (std::max<int64_t>(b.n, tiles0) + 255) / 256));
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(KBuildInit, dim3(grid), dim3(256), 0, s, b, int(tiles0));
@@ -576,8 +576,12 @@ constexpr uint64_t kFlagAgg = 1ull << 62;
 constexpr uint64_t kFlagInc = 2ull << 62;
 constexpr uint64_t kMask31 = 0x7FFFFFFFull;
 
+// Tiles hold BLOCK * ipt clusters, ipt in {1, 2, 4} chosen per iteration so that there are at
+// least `min_tiles` tiles: small iterations spread over more workgroups, each thread then
+// computing about one nearest neighbour instead of four.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter, int R, int tail, int max_leaf) {
+__global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter, int R, int tail, int max_leaf,
+                                                        int min_tiles) {
     constexpr int CHUNK = BLOCK * kIpt;
     constexpr int WMAX = CHUNK + 4 * kMaxRadius;
     __shared__ float s_lx[WMAX], s_ly[WMAX], s_lz[WMAX], s_hx[WMAX], s_hy[WMAX], s_hz[WMAX];
@@ -603,7 +607,9 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
         }
         return;
     }
-    const int T = (c + CHUNK - 1) / CHUNK;
+    const int ipt = c >= int64_t(BLOCK) * 4 * min_tiles ? 4 : (c >= int64_t(BLOCK) * 2 * min_tiles ? 2 : 1);
+    const int chunk = BLOCK * ipt;
+    const int T = (c + chunk - 1) / chunk;
     // The grid is sized from a count the host read some iterations ago; surplus workgroups
     // leave before touching the (device-scope, cross-XCD) tile counter.
     if (int(blockIdx.x) >= T) return;
@@ -617,12 +623,11 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
         __syncthreads();
         const int tile = s_tile;
         if (tile >= T) break;
-        if (threadIdx.x == 0) {
-            status_next[tile] = 0ull;
-            if (tile == 0) b.tile_counter[(iter + 1) & 1] = 0u;
-        }
-        const int a = tile * CHUNK;
-        const int e = a + CHUNK < c ? a + CHUNK : c;
+        // the next iteration has at most ceil(c / BLOCK) tiles: clear ipt states per tile
+        if (threadIdx.x < ipt) status_next[tile * ipt + threadIdx.x] = 0ull;
+        if (threadIdx.x == 0 && tile == 0) b.tile_counter[(iter + 1) & 1] = 0u;
+        const int a = tile * chunk;
+        const int e = a + chunk < c ? a + chunk : c;
         const int w0 = a - 2 * R;
         const int wlen = (e - a) + 4 * R;
         for (int t = threadIdx.x; t < wlen; t += BLOCK) {
@@ -645,9 +650,9 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
         int32_t nnj[kIpt];
 #pragma unroll
         for (int q = 0; q < kIpt; q++) {
-            const int i = a + threadIdx.x * kIpt + q;
+            const int i = a + threadIdx.x * ipt + q;
             nnj[q] = -1;
-            if (i < e) {
+            if (q < ipt && i < e) {
                 const int j = s_nn[i - (a - R)];
                 const bool mutual = s_nn[j - (a - R)] == i;
                 nnj[q] = j;
@@ -660,32 +665,24 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
         const uint32_t before = BlockExclusiveScan<BLOCK>(mine, s_scan, total);
         const uint64_t aggS = total & 0xFFFFu, aggM = total >> 16;
 
-        // decoupled look-back over earlier tiles
-        if (threadIdx.x == 0) {
-            uint64_t excl = 0;  // S in bits 31..61, M in bits 0..30
+        // decoupled look-back over earlier tiles (S in bits 31..61, M in bits 0..30)
+        if (threadIdx.x < kWave) {
+            uint64_t excl = 0;
             if (tile == 0) {
-                __hip_atomic_store(&status[0], kFlagInc | (aggS << 31) | aggM, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(&status[0], kFlagInc | (aggS << 31) | aggM, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             } else {
-                __hip_atomic_store(&status[tile], kFlagAgg | (aggS << 31) | aggM, __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-                int p = tile - 1;
-                for (;;) {
-                    uint64_t v = __hip_atomic_load(&status[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    uint64_t flag = v & (3ull << 62);
-                    if (flag == 0) {
-                        __builtin_amdgcn_s_sleep(1);
-                        continue;
-                    }
-                    excl += v & ~(3ull << 62);
-                    if (flag == kFlagInc) break;
-                    p--;
-                }
-                __hip_atomic_store(&status[tile], kFlagInc | (excl + ((aggS << 31) | aggM)), __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(&status[tile], kFlagAgg | (aggS << 31) | aggM, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+                excl = WaveLookBack(status, tile);
+                if (threadIdx.x == 0)
+                    __hip_atomic_store(&status[tile], kFlagInc | (excl + ((aggS << 31) | aggM)), __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
             }
-            s_prefix = excl;
-            if (tile == T - 1) {
+            if (threadIdx.x == 0) s_prefix = excl;
+            if (threadIdx.x == 0 && tile == T - 1) {
                 const int32_t next = int32_t(((excl >> 31) & kMask31) + aggS);
                 b.ccount[iter + 1] = next;
                 b.results->progress = next;
@@ -699,7 +696,7 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
         int32_t mpos = node_base + int32_t(excl & kMask31) + int32_t(before >> 16);
 #pragma unroll
         for (int q = 0; q < kIpt; q++) {
-            const int i = a + threadIdx.x * kIpt + q;
+            const int i = a + threadIdx.x * ipt + q;
             if (!(sflags & (1u << q))) continue;
             const int ii = i - w0;
             PNode o;
@@ -734,20 +731,25 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
 }
 
 int PlocChunk(int block) { return block * kIpt; }
+int64_t PlocStatusWords(int64_t n, int block) { return (n + block - 1) / block + kIpt; }
 int PlocTail() { return kTail; }
 
 hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int tail, int max_leaf, int grid,
-                               hipStream_t s) {
+                               int min_tiles, hipStream_t s) {
     if (grid < 1) grid = 1;
+    if (min_tiles < 1) min_tiles = 1;
     switch (block) {
         case 128:
-            hipLaunchKernelGGL(KPlocIteration<128>, dim3(grid), dim3(128), 0, s, b, iter, radius, tail, max_leaf);
+            hipLaunchKernelGGL(KPlocIteration<128>, dim3(grid), dim3(128), 0, s, b, iter, radius, tail, max_leaf,
+                               min_tiles);
             break;
         case 512:
-            hipLaunchKernelGGL(KPlocIteration<512>, dim3(grid), dim3(512), 0, s, b, iter, radius, tail, max_leaf);
+            hipLaunchKernelGGL(KPlocIteration<512>, dim3(grid), dim3(512), 0, s, b, iter, radius, tail, max_leaf,
+                               min_tiles);
             break;
         default:
-            hipLaunchKernelGGL(KPlocIteration<256>, dim3(grid), dim3(256), 0, s, b, iter, radius, tail, max_leaf);
+            hipLaunchKernelGGL(KPlocIteration<256>, dim3(grid), dim3(256), 0, s, b, iter, radius, tail, max_leaf,
+                               min_tiles);
             break;
     }
     return hipGetLastError();

@@ -96,6 +96,46 @@ __device__ __forceinline__ uint32_t WaveInclusiveScan(uint32_t v) {
     return v;
 }
 
+__device__ __forceinline__ uint64_t WaveSumU64(uint64_t v) {
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) {
+        const uint32_t lo = __shfl_xor(uint32_t(v), d, kWave);
+        const uint32_t hi = __shfl_xor(uint32_t(v >> 32), d, kWave);
+        v += (uint64_t(hi) << 32) | lo;
+    }
+    return v;
+}
+
+// Decoupled look-back over 64-bit tile states (2 flag bits on top: 1 = aggregate, 2 =
+// inclusive prefix; values add in the low 62 bits), with one wave reading 64 predecessors
+// per round: lane l reads tile - 1 - l, the nearest inclusive state ends the walk, and a
+// round in which any tile up to that one has not published yet is retried. Called by the
+// whole first wave of the workgroup; every lane returns the exclusive prefix of `tile`.
+// Tile 0 publishes its inclusive state before it could wait, so the walk always ends.
+__device__ __forceinline__ uint64_t WaveLookBack(uint64_t* status, int tile) {
+    const int lane = int(threadIdx.x) & (kWave - 1);
+    constexpr uint64_t kFlags = 3ull << 62, kInc = 2ull << 62;
+    uint64_t excl = 0;
+    int top = tile - 1;
+    for (;;) {
+        const int q = top - lane;
+        const uint64_t v = q >= 0 ? __hip_atomic_load(&status[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : kInc;
+        const uint64_t flag = v & kFlags;
+        const uint64_t inc_mask = __ballot(flag == kInc);
+        const uint64_t zero_mask = __ballot(flag == 0);
+        const int first_inc = inc_mask ? __ffsll(static_cast<unsigned long long>(inc_mask)) - 1 : kWave;
+        const uint64_t need = first_inc >= kWave - 1 ? ~0ull : ((2ull << first_inc) - 1);  // lanes 0..first_inc
+        if (zero_mask & need) {
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        excl += WaveSumU64(lane <= first_inc ? (v & ~kFlags) : 0ull);
+        if (first_inc < kWave) break;
+        top -= kWave;
+    }
+    return excl;
+}
+
 __device__ __forceinline__ float WaveMin(float v) {
 #pragma unroll
     for (int d = kWave / 2; d > 0; d >>= 1) v = fminf(v, __shfl_xor(v, d, kWave));
@@ -187,14 +227,15 @@ struct BuildBuffers {
     const int4* triples;   // (a, b, c, bits of 1/|det|) per non-coplanar direction triple
 };
 
-hipError_t LaunchBuildInit(const BuildBuffers& b, int chunk, hipStream_t s);
+hipError_t LaunchBuildInit(const BuildBuffers& b, int block, hipStream_t s);
 hipError_t LaunchBounds(const BuildBuffers& b, hipStream_t s);
 hipError_t LaunchCodes(const BuildBuffers& b, int morton_type, hipStream_t s);
 hipError_t LaunchInitClusters(const BuildBuffers& b, hipStream_t s);
 hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int tail, int max_leaf,
-                               int grid, hipStream_t s);
+                               int grid, int min_tiles, hipStream_t s);
 hipError_t LaunchPlocTail(const BuildBuffers& b, int iter, int radius, int max_leaf, hipStream_t s);
 int PlocChunk(int block);
+int64_t PlocStatusWords(int64_t n, int block);  // look-back states per parity
 int PlocTail();
 hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s);
 hipError_t LaunchFinish(const BuildBuffers& b, hipStream_t s);

@@ -92,7 +92,8 @@ typedef struct dpl_build_stats {
     int32_t tree_height;     /* longest root-to-leaf path of the final tree */
     int32_t ploc_launches;   /* multi-workgroup PLOC iteration launches that merged */
     double ploc_launch_ms;   /* their summed device time (keep_intermediates bit 1), else 0 */
-    double ploc_bytes;       /* their algorithmic bytes: 32 c_k + 32 c_k+1 + 91 m_k (bit 1), else 0 */
+    double ploc_bytes;       /* their algorithmic bytes: 32 c_k + 32 c_k+1 + (91 + 24 x) m_k, with
+                                x = SOBB directions - 3 (10 for the 26-DOP set), 0 for AABB (bit 1) */
 } dpl_build_stats;
 
 typedef struct dpl_context dpl_context;

@@ -152,6 +152,40 @@ def test_nearest_neighbour_brute_force(oracle, radius):
         assert nn[i] == best[2]
 
 
+@pytest.mark.parametrize("n,radius,seed", [(300, 1, 1), (400, 4, 2), (500, 16, 3), (257, 8, 4)])
+def test_merge_list_replays_mutual_nearest_neighbours(oracle, n, radius, seed):
+    """Replays the recorded merge list iteration by iteration: the nodes made in iteration k
+    (ids 2n - c_k .. 2n - c_k+1 - 1) must be exactly the mutual nearest-neighbour pairs of
+    that iteration's cluster list, in list order, each replacing its lower-index member."""
+    tris = soup(n, seed, size=0.3)
+    p = oracle.Ploc(tris, 0, radius, 4)
+    nodes = p.nodes
+    clusters = list(range(n))  # node ids, in code order
+    ic = list(p.iter_counts)
+    for k, c in enumerate(ic):
+        assert len(clusters) == c
+        lohi = np.concatenate([nodes["lo"][clusters], nodes["hi"][clusters]], 1).astype(np.float32)
+        nn = oracle.ploc_nn(lohi, radius)
+        pairs = [(i, int(nn[i])) for i in range(c) if nn[nn[i]] == i and i < nn[i]]
+        assert pairs, "every iteration merges at least one pair"
+        base = 2 * n - c
+        nxt, partner = [], {j: i for i, j in pairs}
+        made = dict((i, base + r) for r, (i, _) in enumerate(pairs))
+        for r, (i, j) in enumerate(pairs):
+            nd = nodes[base + r]
+            assert nd["left"] == clusters[i] and nd["right"] == clusters[j]
+            assert (nd["lo"] == np.minimum(nodes["lo"][clusters[i]], nodes["lo"][clusters[j]])).all()
+        for i in range(c):
+            if i in made:
+                nxt.append(made[i])
+            elif i not in partner:
+                nxt.append(clusters[i])
+        clusters = nxt
+        nexpected = ic[k + 1] if k + 1 < len(ic) else 1
+        assert len(clusters) == nexpected
+    assert clusters == [2 * n - 2]
+
+
 # ---------------------------------------------------------------------------------------
 # PLOC merges, collapse, final layout
 # ---------------------------------------------------------------------------------------
