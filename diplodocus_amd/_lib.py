@@ -49,6 +49,17 @@ class RenderParams(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+class MeshInfo(C.Structure):
+    _fields_ = [("node_offset", C.c_int64), ("prim_offset", C.c_int64), ("node_count", C.c_int32),
+                ("prim_count", C.c_int32), ("tree_height", C.c_int32), ("ploc_iterations", C.c_int32),
+                ("sah_cost", C.c_double)]
+
+
+class AutotuneInfo(C.Structure):
+    _fields_ = [("build_ms", C.c_double), ("sah_cost", C.c_double), ("candidates", C.c_int32),
+                ("from_cache", C.c_int32)]
+
+
 class DplError(RuntimeError):
     def __init__(self, status: int, message: str):
         super().__init__(f"dpl error {STATUS_NAMES.get(status, status)}: {message}")
@@ -92,6 +103,12 @@ def _load() -> C.CDLL:
         "dpl_radix_sort_pairs": (I32, [P, P, P, I64, I32, I32]),
         "dpl_render_forward": (I32, [P, P, C.POINTER(RenderParams), P, P, P, P]),
         "dpl_render_adjoint": (I32, [P, P, C.POINTER(RenderParams), P, P, P, P, P, P]),
+        "dpl_bvh_get_meshes": (I32, [P, P]),
+        "dpl_trace_graph_create": (I32, [P, P, P, P, I64, U32, P, C.POINTER(P)]),
+        "dpl_trace_graph_launch": (I32, [P]),
+        "dpl_trace_graph_destroy": (None, [P]),
+        "dpl_autotune": (I32, [P, P, I64, P, I32, I32, C.POINTER(BuildParams), C.c_double, C.c_char_p,
+                               C.POINTER(BuildParams), C.POINTER(AutotuneInfo)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -109,5 +126,6 @@ EXPORTED = [
     "dpl_bvh_build_batched", "dpl_bvh_destroy", "dpl_bvh_get_stats", "dpl_trace", "dpl_trace_counters",
     "dpl_trace_batched",
     "dpl_bvh_get_layout", "dpl_bvh_download", "dpl_compute_codes", "dpl_radix_sort_pairs", "dpl_render_forward",
-    "dpl_render_adjoint",
+    "dpl_render_adjoint", "dpl_bvh_get_meshes", "dpl_trace_graph_create", "dpl_trace_graph_launch",
+    "dpl_trace_graph_destroy", "dpl_autotune",
 ]

@@ -23,6 +23,8 @@ from ._lib import BuildParams, BuildStats, BvhLayout, DplError, lib
 HIT_DTYPE = np.dtype([("t", "<f4"), ("b1", "<f4"), ("b2", "<f4"), ("prim", "<i4")])
 PNODE_DTYPE = np.dtype([("lo", "<f4", 3), ("hi", "<f4", 3), ("left", "<i4"), ("right", "<i4")])
 FNODE_DTYPE = np.dtype([("box", "<f4", 12), ("left", "<i4"), ("right", "<i4"), ("pad", "<i4", 2)])
+MESH_DTYPE = np.dtype([("node_offset", "<i8"), ("prim_offset", "<i8"), ("node_count", "<i4"), ("prim_count", "<i4"),
+                       ("tree_height", "<i4"), ("ploc_iterations", "<i4"), ("sah_cost", "<f8")])
 
 
 class AccelerationStructureType(enum.IntEnum):
@@ -182,6 +184,46 @@ class GpuBvh:
         self.n = n
         return self
 
+    def build_batched(self, tris, offsets, params: BuildParams) -> "GpuBvh":
+        """One BVH per mesh tris[offsets[m]:offsets[m+1]], all built in one persistent launch."""
+        tp, mem = _ptr(tris)
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        st = BuildStats()
+        self.close()
+        h = C.c_void_p()
+        self.ctx.check(lib.dpl_bvh_build_batched(self.ctx.h, tp, off.ctypes.data_as(C.c_void_p), len(off) - 1, mem,
+                                                 C.byref(params), C.byref(h), C.byref(st)))
+        self.h = h
+        self.stats = st
+        self.n = int(off[-1])
+        return self
+
+    def meshes(self) -> np.ndarray:
+        """Per-mesh node/triangle offsets and counts, tree height, iterations and SAH."""
+        lay = self.layout()
+        out = (_lib.MeshInfo * int(lay.mesh_count))()
+        self.ctx.check(lib.dpl_bvh_get_meshes(self.h, out))
+        return np.array([(m.node_offset, m.prim_offset, m.node_count, m.prim_count, m.tree_height, m.ploc_iterations,
+                          m.sah_cost) for m in out], MESH_DTYPE)
+
+    def trace_batched(self, rays, ray_mesh, flags: int = 0, hits=None):
+        """Ray i against mesh ray_mesh[i] of a batched BVH; hit prim ids are mesh-local."""
+        rp, mem = _ptr(rays)
+        mp, mmem = _ptr(ray_mesh)
+        n = int(rays.shape[0])
+        if hits is None:
+            if mem == _lib.DPL_MEM_DEVICE:
+                import torch
+
+                hits = torch.empty((n, 4), dtype=torch.float32, device=rays.device)
+            else:
+                hits = np.zeros(n, HIT_DTYPE)
+        hp, hmem = _ptr(hits)
+        if not (hmem == mem == mmem):
+            raise ValueError("rays, ray_mesh and hits must live in the same memory")
+        self.ctx.check(lib.dpl_trace_batched(self.ctx.h, self.h, rp, mp, n, flags, hp, mem))
+        return hits
+
     def close(self) -> None:
         if self.h is not None:
             lib.dpl_bvh_destroy(self.h)
@@ -251,3 +293,53 @@ class GpuBvh:
             ptrs += [None] * 4
         self.ctx.check(lib.dpl_bvh_download(self.h, *ptrs))
         return out
+
+
+class TraceGraph:
+    """A trace pass captured once in a hipGraph and replayed by launch(); rays, ray_mesh
+    (batched BVHs only) and hits are GPU tensors whose storage the graph keeps using."""
+
+    def __init__(self, bvh: GpuBvh, rays, hits, ray_mesh=None, flags: int = 0):
+        rp, mem = _ptr(rays)
+        hp, hmem = _ptr(hits)
+        mp = None
+        if ray_mesh is not None:
+            mp, mmem = _ptr(ray_mesh)
+            if mmem != _lib.DPL_MEM_DEVICE:
+                raise ValueError("ray_mesh must be a GPU tensor")
+        if mem != _lib.DPL_MEM_DEVICE or hmem != _lib.DPL_MEM_DEVICE:
+            raise ValueError("a captured trace needs GPU tensors")
+        self.ctx, self._keep = bvh.ctx, (bvh, rays, hits, ray_mesh)
+        h = C.c_void_p()
+        self.ctx.check(lib.dpl_trace_graph_create(self.ctx.h, bvh.h, rp, mp, int(rays.shape[0]), flags, hp,
+                                                  C.byref(h)))
+        self.h = h
+
+    def launch(self) -> None:
+        self.ctx.check(lib.dpl_trace_graph_launch(self.h))
+
+    def close(self) -> None:
+        if getattr(self, "h", None) is not None:
+            lib.dpl_trace_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def autotune(ctx: Context, tris, base: BuildParams, offsets=None, sah_tolerance: float = 0.05,
+             cache_path: str | None = None):
+    """Chooses nn_search_radius, block_size and sort_digit_bits for this workload (see
+    dpl_autotune); returns (params, info dict)."""
+    tp, mem = _ptr(tris)
+    best = BuildParams()
+    info = _lib.AutotuneInfo()
+    if offsets is not None:
+        off = np.ascontiguousarray(offsets, dtype=np.int64)
+        op, m, n = off.ctypes.data_as(C.c_void_p), len(off) - 1, int(off[-1])
+    else:
+        off, op, m, n = None, None, 0, int(tris.shape[0])
+    ctx.check(lib.dpl_autotune(ctx.h, tp, n, op, m, mem, C.byref(base), sah_tolerance,
+                               cache_path.encode() if cache_path else None, C.byref(best), C.byref(info)))
+    return best, {"build_ms": info.build_ms, "sah_cost": info.sah_cost, "candidates": info.candidates,
+                  "from_cache": bool(info.from_cache)}

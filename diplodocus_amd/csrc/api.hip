@@ -5,8 +5,12 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <map>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -153,6 +157,9 @@ struct dpl_context {
     DevBuf tri, keys, vals, keys_alt, vals_alt, nodes, parent, clusters0, clusters1, ccount, status0, status1,
         small, count, cost, collapsed, minleaf, height, leafslot, keptflag, scan_tmp, sort_tmp, rays, hits,
         trace_counter, spill, kdop, sobb_set;
+    // batched builds and traces
+    DevBuf b_off, b_foff, b_order, b_results, b_queue, ray_mesh;
+    std::string device_name;
     int sobb_kdop = -1;  // kdop_size the uploaded direction set was made for
     void* pinned = nullptr;          // host staging for small results (device-visible)
     int32_t* pinned_dev = nullptr;   // device address of `pinned`
@@ -185,6 +192,11 @@ struct dpl_bvh {
     std::vector<int32_t> iter_counts;
     bool has_intermediates = false;
     bool codes64 = false;
+    // batched BVHs: one tree per mesh
+    bool batched = false;
+    DevBuf batch_tree;                 // BatchTree per mesh
+    std::vector<dpl_mesh_info> meshes;
+    int64_t fnode_slots = 0;           // final-node slots of all meshes
 };
 
 namespace {
@@ -520,6 +532,7 @@ dpl_status dpl_context_create(int32_t device, dpl_context** out) {
         hipDeviceProp_t prop;
         DPL_HIP(hipGetDeviceProperties(&prop, device));
         ctx->cu_count = prop.multiProcessorCount;
+        ctx->device_name = std::string(prop.gcnArchName) + " " + prop.name;
         DPL_HIP(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
         ctx->stream = ctx->own_stream;
         for (auto& e : ctx->ev) DPL_HIP(hipEventCreate(&e));
@@ -584,10 +597,181 @@ dpl_status dpl_bvh_rebuild(dpl_bvh* bvh, const float* tri, int64_t n, dpl_mem me
     });
 }
 
-dpl_status dpl_bvh_build_batched(dpl_context* ctx, const float*, const int64_t*, int32_t, dpl_mem,
-                                 const dpl_build_params*, dpl_bvh** out, dpl_build_stats*) {
-    if (out) *out = nullptr;
-    return Fail(ctx, DPL_ERR_UNSUPPORTED, "batched builds are not available in this build");
+namespace {
+
+// Batched build: validates the meshes, uploads the offset tables and the work queue order
+// (meshes by decreasing size), launches the persistent per-mesh build once, and reads the
+// per-mesh results back to fill the mesh table, the per-mesh trace trees and the stats.
+void BatchImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, const int64_t* offsets, int32_t M, dpl_mem mem,
+               const dpl_build_params& p, dpl_build_stats* out_stats) {
+    Require(tri_in != nullptr && offsets != nullptr, "tri_xyz and mesh_offsets must be non-null");
+    Require(M >= 1, "at least one mesh is required");
+    Require(mem == DPL_MEM_HOST || mem == DPL_MEM_DEVICE, "mem must be DPL_MEM_HOST or DPL_MEM_DEVICE");
+    Require(offsets[0] == 0, "mesh_offsets[0] must be 0");
+    for (int32_t m = 0; m < M; m++) {
+        Require(offsets[m + 1] > offsets[m], "every mesh needs at least one triangle (offsets strictly increasing)");
+        Require(offsets[m + 1] - offsets[m] < (int64_t(1) << 27), "at most 2^27 - 1 triangles per mesh");
+    }
+    const int64_t N = offsets[M];
+    Require(N < (int64_t(1) << 31), "at most 2^31 - 1 triangles per batch");
+    if (p.bv_type != DPL_BV_AABB) throw Error(DPL_ERR_UNSUPPORTED, "batched builds support AABB bounds only");
+    Require(p.block_size == 256 || p.block_size == 512, "batched block_size must be 256 or 512");
+    hipStream_t s = ctx->stream;
+    dpl_context& C = *ctx;
+
+    std::vector<int64_t> foff(size_t(M) + 1, 0);
+    for (int32_t m = 0; m < M; m++) foff[m + 1] = foff[m] + std::max<int64_t>(1, offsets[m + 1] - offsets[m] - 1);
+    std::vector<int32_t> order(M);
+    for (int32_t m = 0; m < M; m++) order[m] = m;
+    std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return offsets[a + 1] - offsets[a] > offsets[b + 1] - offsets[b];
+    });
+    const int64_t nodes_total = 2 * N - M;
+
+    BatchArgs A{};
+    const float* tri = tri_in;
+    if (mem == DPL_MEM_HOST) tri = C.tri.ensure<float>(size_t(N) * 36);
+    A.tri = tri;
+    A.tri_off = C.b_off.ensure<int64_t>(size_t(M + 1) * 8);
+    A.fnode_off = C.b_foff.ensure<int64_t>(size_t(M + 1) * 8);
+    A.order = C.b_order.ensure<int32_t>(size_t(M) * 4);
+    A.mesh_count = M;
+    A.queue = C.b_queue.ensure<uint32_t>(64);
+    A.keys = C.keys.ensure<uint64_t>(size_t(N) * 8);
+    A.keys_alt = C.keys_alt.ensure<uint64_t>(size_t(N) * 8);
+    A.vals = C.vals.ensure<uint32_t>(size_t(N) * 4);
+    A.vals_alt = C.vals_alt.ensure<uint32_t>(size_t(N) * 4);
+    A.clusters0 = C.clusters0.ensure<PNode>(size_t(N) * sizeof(PNode));
+    A.clusters1 = C.clusters1.ensure<PNode>(size_t(N) * sizeof(PNode));
+    A.leafslot = C.leafslot.ensure<int32_t>(size_t(N) * 4);
+    A.keptflag = C.keptflag.ensure<int32_t>(size_t(N) * 4);
+    A.nodes = C.nodes.ensure<PNode>(size_t(nodes_total) * sizeof(PNode));
+    A.parent = C.parent.ensure<int32_t>(size_t(nodes_total) * 4);
+    A.count = C.count.ensure<int32_t>(size_t(nodes_total) * 4);
+    A.cost = C.cost.ensure<float>(size_t(nodes_total) * 4);
+    A.minleaf = C.minleaf.ensure<int32_t>(size_t(nodes_total) * 4);
+    A.height = C.height.ensure<int32_t>(size_t(nodes_total) * 4);
+    A.collapsed = C.collapsed.ensure<uint8_t>(size_t(nodes_total));
+    A.results = C.b_results.ensure<MeshResult>(size_t(M) * sizeof(MeshResult));
+    A.fnodes = bvh->fnodes.ensure<FNode>(size_t(foff[M]) * sizeof(FNode));
+    A.tri_ordered = bvh->tri_ordered.ensure<float4>(size_t(N) * 48);
+    A.prim_order = bvh->prim_order.ensure<int32_t>(size_t(N) * 4);
+    A.morton_type = p.morton_type;
+    A.radius = p.nn_search_radius;
+    A.max_leaf = p.max_triangles_per_leaf;
+    A.digit_bits = p.sort_digit_bits;
+    BatchTree* trees = bvh->batch_tree.ensure<BatchTree>(size_t(M) * sizeof(BatchTree));
+
+    if (mem == DPL_MEM_HOST) {
+        DPL_HIP(hipEventRecord(C.ev[8], s));
+        DPL_HIP(hipMemcpyAsync(const_cast<float*>(A.tri), tri_in, size_t(N) * 36, hipMemcpyHostToDevice, s));
+        DPL_HIP(hipEventRecord(C.ev[9], s));
+    }
+    DPL_HIP(hipMemcpyAsync(const_cast<int64_t*>(A.tri_off), offsets, size_t(M + 1) * 8, hipMemcpyHostToDevice, s));
+    DPL_HIP(hipMemcpyAsync(const_cast<int64_t*>(A.fnode_off), foff.data(), size_t(M + 1) * 8, hipMemcpyHostToDevice,
+                           s));
+    DPL_HIP(hipMemcpyAsync(const_cast<int32_t*>(A.order), order.data(), size_t(M) * 4, hipMemcpyHostToDevice, s));
+    DPL_HIP(hipMemsetAsync(A.results, 0, size_t(M) * sizeof(MeshResult), s));
+    DPL_HIP(hipMemsetAsync(A.queue, 0, 64, s));
+    const int grid = int(std::min<int64_t>(M, int64_t(C.cu_count) * BatchWgPerCu(p.block_size)));
+    DPL_HIP(hipEventRecord(C.ev[0], s));
+    DPL_HIP(LaunchBatchBuild(A, p.block_size, grid, s));
+    DPL_HIP(hipEventRecord(C.ev[7], s));
+    std::vector<MeshResult> res(M);
+    DPL_HIP(hipMemcpyAsync(res.data(), A.results, size_t(M) * sizeof(MeshResult), hipMemcpyDeviceToHost, s));
+    DPL_HIP(hipStreamSynchronize(s));
+    for (const MeshResult& r : res) {
+        if (r.flags & 1u) throw Error(DPL_ERR_INVALID_ARG, "triangle coordinates must be finite");
+        if (r.flags & 2u) throw Error(DPL_ERR_INTERNAL, "PLOC iteration without a merge");
+    }
+
+    dpl_build_stats st{};
+    std::vector<BatchTree> bt(M);
+    bvh->meshes.assign(M, dpl_mesh_info{});
+    double sah_weighted = 0.0;
+    int32_t max_height = 0, max_iters = 0;
+    int64_t inner = 0, leaves = 0;
+    for (int32_t m = 0; m < M; m++) {
+        const MeshResult& r = res[m];
+        const int64_t nm = offsets[m + 1] - offsets[m];
+        const int32_t fcount = r.kept > 0 ? r.kept : 1;
+        const double sx = double(r.root_hi[0]) - r.root_lo[0], sy = double(r.root_hi[1]) - r.root_lo[1],
+                     sz = double(r.root_hi[2]) - r.root_lo[2];
+        const double root_sa = 2.0 * (sx * sy + sy * sz + sx * sz);
+        const double sah = r.kept > 0 ? (root_sa > 0 ? (root_sa + r.sah) / root_sa : 0.0) : double(nm);
+        dpl_mesh_info& mi = bvh->meshes[m];
+        mi.node_offset = foff[m];
+        mi.prim_offset = offsets[m];
+        mi.node_count = fcount;
+        mi.prim_count = int32_t(nm);
+        mi.tree_height = r.root_height;
+        mi.ploc_iterations = r.iterations;
+        mi.sah_cost = sah;
+        bt[m] = BatchTree{foff[m], offsets[m], fcount, int32_t(nm), r.root_height, 0};
+        sah_weighted += sah * double(nm);
+        max_height = std::max(max_height, r.root_height);
+        max_iters = std::max(max_iters, r.iterations);
+        inner += fcount;
+        leaves += r.leaves;
+    }
+    DPL_HIP(hipMemcpyAsync(trees, bt.data(), size_t(M) * sizeof(BatchTree), hipMemcpyHostToDevice, s));
+    DPL_HIP(hipStreamSynchronize(s));
+    st.build_ms = ElapsedMs(C.ev[0], C.ev[7]);
+    st.upload_ms = mem == DPL_MEM_HOST ? ElapsedMs(C.ev[8], C.ev[9]) : 0.0;
+    st.stage_ms[3] = st.build_ms;  // one launch runs every stage
+    st.inner_node_count = int32_t(std::min<int64_t>(inner, INT32_MAX));
+    st.leaf_node_count = int32_t(std::min<int64_t>(leaves, INT32_MAX));
+    st.node_count = int32_t(std::min<int64_t>(inner + leaves, INT32_MAX));
+    st.ploc_iterations = max_iters;
+    st.memory_consumption = int64_t(inner) * 64 + N * 48 + N * 4;
+    st.sah_cost = sah_weighted / double(N);
+    st.tree_height = max_height;
+
+    bvh->batched = true;
+    bvh->mesh_count = M;
+    bvh->prim_count = N;
+    bvh->node_count = int32_t(std::min<int64_t>(foff[M], INT32_MAX));
+    bvh->fnode_slots = foff[M];
+    bvh->tree_height = max_height;
+    bvh->ndirs = 0;
+    bvh->params = p;
+    bvh->has_intermediates = false;
+    bvh->stats = st;
+    if (out_stats) *out_stats = st;
+}
+
+}  // namespace
+
+dpl_status dpl_bvh_build_batched(dpl_context* ctx, const float* tri, const int64_t* offsets, int32_t mesh_count,
+                                 dpl_mem mem, const dpl_build_params* params, dpl_bvh** out, dpl_build_stats* stats) {
+    if (!ctx || !out || !params) return ctx ? Fail(ctx, DPL_ERR_INVALID_ARG, "null argument") : DPL_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::unique_ptr<dpl_bvh> bvh(new (std::nothrow) dpl_bvh());
+    if (!bvh) return Fail(ctx, DPL_ERR_OOM, "host allocation failed");
+    bvh->ctx = ctx;
+    dpl_status st = Guard(ctx, [&] {
+        dpl_build_params p = Normalize(params);
+        BatchImpl(ctx, bvh.get(), tri, offsets, mesh_count, mem, p, stats);
+    });
+    if (st != DPL_OK) return st;
+    *out = bvh.release();
+    return DPL_OK;
+}
+
+dpl_status dpl_bvh_get_meshes(const dpl_bvh* bvh, dpl_mesh_info* out) {
+    if (!bvh || !out) return DPL_ERR_INVALID_ARG;
+    if (bvh->batched) {
+        std::memcpy(out, bvh->meshes.data(), bvh->meshes.size() * sizeof(dpl_mesh_info));
+    } else {
+        dpl_mesh_info mi{};
+        mi.node_count = bvh->node_count;
+        mi.prim_count = int32_t(bvh->prim_count);
+        mi.tree_height = bvh->tree_height;
+        mi.ploc_iterations = bvh->stats.ploc_iterations;
+        mi.sah_cost = bvh->stats.sah_cost;
+        out[0] = mi;
+    }
+    return DPL_OK;
 }
 
 void dpl_bvh_destroy(dpl_bvh* bvh) {
@@ -608,7 +792,7 @@ dpl_status dpl_bvh_get_stats(const dpl_bvh* bvh, dpl_build_stats* stats) {
 dpl_status dpl_bvh_get_layout(const dpl_bvh* bvh, dpl_bvh_layout* l) {
     if (!bvh || !l) return DPL_ERR_INVALID_ARG;
     l->prim_count = bvh->prim_count;
-    l->node_count = bvh->node_count;
+    l->node_count = bvh->batched ? bvh->fnode_slots : bvh->node_count;
     l->ploc_node_count = bvh->has_intermediates ? 2 * bvh->prim_count - 1 : 0;
     l->iteration_count = bvh->has_intermediates ? int64_t(bvh->iter_counts.size()) : 0;
     l->mesh_count = bvh->mesh_count;
@@ -624,7 +808,8 @@ dpl_status dpl_bvh_download(const dpl_bvh* bvh, void* nodes, void* tri_ordered, 
         hipStream_t s = ctx->stream;
         DPL_HIP(hipStreamSynchronize(s));
         const int64_t n = bvh->prim_count;
-        if (nodes) DPL_HIP(hipMemcpy(nodes, bvh->fnodes.p, size_t(bvh->node_count) * 64, hipMemcpyDeviceToHost));
+        const int64_t node_slots = bvh->batched ? bvh->fnode_slots : bvh->node_count;
+        if (nodes) DPL_HIP(hipMemcpy(nodes, bvh->fnodes.p, size_t(node_slots) * 64, hipMemcpyDeviceToHost));
         if (tri_ordered) DPL_HIP(hipMemcpy(tri_ordered, bvh->tri_ordered.p, size_t(n) * 48, hipMemcpyDeviceToHost));
         if (prim_order) DPL_HIP(hipMemcpy(prim_order, bvh->prim_order.p, size_t(n) * 4, hipMemcpyDeviceToHost));
         const bool want = codes || sorted_idx || ploc_nodes || iter_counts;
@@ -655,6 +840,7 @@ static dpl_status TraceImpl(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray*
         Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK)) == 0u, "unknown trace flags");
         Require(mem == DPL_MEM_HOST || mem == DPL_MEM_DEVICE, "mem must be DPL_MEM_HOST or DPL_MEM_DEVICE");
         Require(bvh->ctx == ctx, "bvh belongs to another context");
+        Require(!bvh->batched, "a batched BVH is traced with dpl_trace_batched");
         if (counts_out) counts_out[0] = counts_out[1] = 0;
         if (n == 0) return;
         hipStream_t s = ctx->stream;
@@ -701,9 +887,104 @@ dpl_status dpl_trace_counters(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ra
     return st;
 }
 
-dpl_status dpl_trace_batched(dpl_context* ctx, const dpl_bvh*, const dpl_ray*, const int32_t*, int64_t, uint32_t,
-                             dpl_hit*, dpl_mem) {
-    return Fail(ctx, DPL_ERR_UNSUPPORTED, "batched traces are not available in this build");
+dpl_status dpl_trace_batched(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, const int32_t* ray_mesh,
+                             int64_t n, uint32_t flags, dpl_hit* hits, dpl_mem mem) {
+    if (!ctx || !bvh) return ctx ? Fail(ctx, DPL_ERR_INVALID_ARG, "null argument") : DPL_ERR_INVALID_ARG;
+    return Guard(ctx, [&] {
+        Require(n >= 0, "n must be non-negative");
+        Require(n == 0 || (rays && hits && ray_mesh), "rays, ray_mesh and hits must be non-null");
+        Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK)) == 0u, "unknown trace flags");
+        Require(mem == DPL_MEM_HOST || mem == DPL_MEM_DEVICE, "mem must be DPL_MEM_HOST or DPL_MEM_DEVICE");
+        Require(bvh->ctx == ctx, "bvh belongs to another context");
+        Require(bvh->batched, "dpl_trace_batched needs a batched BVH");
+        if (n == 0) return;
+        hipStream_t s = ctx->stream;
+        const dpl_ray* dr = rays;
+        const int32_t* dm = ray_mesh;
+        dpl_hit* dh = hits;
+        if (mem == DPL_MEM_HOST) {
+            dr = ctx->rays.ensure<dpl_ray>(size_t(n) * sizeof(dpl_ray));
+            dm = ctx->ray_mesh.ensure<int32_t>(size_t(n) * 4);
+            dh = ctx->hits.ensure<dpl_hit>(size_t(n) * sizeof(dpl_hit));
+            DPL_HIP(hipMemcpyAsync(const_cast<dpl_ray*>(dr), rays, size_t(n) * sizeof(dpl_ray), hipMemcpyHostToDevice, s));
+            DPL_HIP(hipMemcpyAsync(const_cast<int32_t*>(dm), ray_mesh, size_t(n) * 4, hipMemcpyHostToDevice, s));
+        }
+        int32_t* spill = ctx->spill.ensure<int32_t>(TraceSpillBytes(ctx->cu_count, bvh->tree_height));
+        DPL_HIP(LaunchTraceBatched(bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(),
+                                   bvh->batch_tree.as<BatchTree>(), bvh->mesh_count, bvh->tree_height, dr, dm, n, flags,
+                                   dh, ctx->cu_count, ctx->trace_counter.as<uint32_t>(), spill, s));
+        if (mem == DPL_MEM_HOST) {
+            DPL_HIP(hipMemcpyAsync(hits, dh, size_t(n) * sizeof(dpl_hit), hipMemcpyDeviceToHost, s));
+            DPL_HIP(hipStreamSynchronize(s));
+        }
+    });
+}
+
+// ---- trace passes captured in a hipGraph ----
+struct dpl_trace_graph {
+    dpl_context* ctx = nullptr;
+    hipStream_t capture = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    DevBuf counter, spill;
+};
+
+dpl_status dpl_trace_graph_create(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, const int32_t* ray_mesh,
+                                  int64_t n, uint32_t flags, dpl_hit* hits, dpl_trace_graph** out) {
+    if (!ctx || !bvh || !out) return ctx ? Fail(ctx, DPL_ERR_INVALID_ARG, "null argument") : DPL_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::unique_ptr<dpl_trace_graph> g(new (std::nothrow) dpl_trace_graph());
+    if (!g) return Fail(ctx, DPL_ERR_OOM, "host allocation failed");
+    g->ctx = ctx;
+    dpl_status st = Guard(ctx, [&] {
+        Require(n >= 1 && rays && hits, "a captured trace needs n >= 1 and device rays / hits");
+        Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK)) == 0u, "unknown trace flags");
+        Require(bvh->ctx == ctx, "bvh belongs to another context");
+        Require(bvh->batched == (ray_mesh != nullptr), "ray_mesh is required exactly for a batched BVH");
+        uint32_t* counter = g->counter.ensure<uint32_t>(256);
+        int32_t* spill = g->spill.ensure<int32_t>(std::max<size_t>(16, TraceSpillBytes(ctx->cu_count, bvh->tree_height)));
+        DPL_HIP(hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking));
+        DPL_HIP(hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal));
+        hipError_t e;
+        if (bvh->batched) {
+            e = LaunchTraceBatched(bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(),
+                                   bvh->batch_tree.as<BatchTree>(), bvh->mesh_count, bvh->tree_height, rays, ray_mesh,
+                                   n, flags, hits, ctx->cu_count, counter, spill, g->capture);
+        } else {
+            TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
+                           int32_t(bvh->prim_count), bvh->tree_height, bvh->ndirs, bvh->sobb_dirs.as<float>()};
+            e = LaunchTrace(tree, rays, n, flags, hits, ctx->cu_count, counter, spill, nullptr, g->capture);
+        }
+        hipGraph_t graph = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(g->capture, &graph);
+        g->graph = graph;
+        DPL_HIP(e);
+        DPL_HIP(e2);
+        DPL_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    });
+    if (st != DPL_OK) {
+        dpl_trace_graph_destroy(g.release());
+        return st;
+    }
+    *out = g.release();
+    return DPL_OK;
+}
+
+dpl_status dpl_trace_graph_launch(dpl_trace_graph* g) {
+    if (!g || !g->exec) return DPL_ERR_INVALID_ARG;
+    return Guard(g->ctx, [&] { DPL_HIP(hipGraphLaunch(g->exec, g->ctx->stream)); });
+}
+
+void dpl_trace_graph_destroy(dpl_trace_graph* g) {
+    if (!g) return;
+    if (g->ctx) {
+        (void)hipSetDevice(g->ctx->device);
+        (void)hipStreamSynchronize(g->ctx->stream);
+    }
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->capture) (void)hipStreamDestroy(g->capture);
+    delete g;
 }
 
 dpl_status dpl_compute_codes(dpl_context* ctx, const float* tri, int64_t n, int32_t morton_type, uint64_t* codes,
@@ -774,6 +1055,7 @@ static dpl_status Render(bool adjoint, dpl_context* ctx, const dpl_bvh* bvh, con
     if (!ctx || !bvh || !rp) return ctx ? Fail(ctx, DPL_ERR_INVALID_ARG, "null argument") : DPL_ERR_INVALID_ARG;
     return Guard(ctx, [&] {
         Require(bvh->ctx == ctx, "bvh belongs to another context");
+        Require(!bvh->batched, "path tracing needs a single-scene BVH");
         Require(rp->width >= 2 && rp->height >= 2, "width and height must be at least 2");
         Require(rp->spp >= 1, "spp must be at least 1");
         Require(rp->max_depth >= 1 && rp->max_depth <= (adjoint ? 16 : 64), "max_depth out of range");
@@ -807,3 +1089,148 @@ dpl_status dpl_render_adjoint(dpl_context* ctx, const dpl_bvh* bvh, const dpl_re
 }
 
 }  // extern "C"
+
+// ---- autotuner ----
+namespace {
+
+struct TuneEntry {
+    int radius = 0, block = 0, digit = 0;
+    double build_ms = 0.0, sah = 0.0;
+};
+
+// The cache is a JSON object written one entry per line by WriteTuneCache:
+// {"version": 1, "entries": {
+//   "<key>": {"nn_search_radius": 16, "block_size": 256, "sort_digit_bits": 8, "build_ms": 4.5, "sah_cost": 284.5},
+// ...}}
+std::map<std::string, TuneEntry> ReadTuneCache(const char* path) {
+    std::map<std::string, TuneEntry> m;
+    std::ifstream f(path);
+    std::string line;
+    while (std::getline(f, line)) {
+        const size_t q0 = line.find('"');
+        const size_t q1 = q0 == std::string::npos ? q0 : line.find('"', q0 + 1);
+        const size_t br = q1 == std::string::npos ? q1 : line.find('{', q1);
+        if (br == std::string::npos) continue;
+        const std::string key = line.substr(q0 + 1, q1 - q0 - 1);
+        if (key == "version" || key == "entries") continue;
+        TuneEntry e;
+        if (std::sscanf(line.c_str() + br,
+                        "{\"nn_search_radius\": %d, \"block_size\": %d, \"sort_digit_bits\": %d, \"build_ms\": %lf, "
+                        "\"sah_cost\": %lf}",
+                        &e.radius, &e.block, &e.digit, &e.build_ms, &e.sah) == 5)
+            m[key] = e;
+    }
+    return m;
+}
+
+void WriteTuneCache(const char* path, const std::map<std::string, TuneEntry>& m) {
+    const std::string tmp = std::string(path) + ".tmp";
+    {
+        std::ofstream f(tmp, std::ios::trunc);
+        if (!f) throw Error(DPL_ERR_INVALID_ARG, std::string("cannot write the autotune cache ") + path);
+        f << "{\"version\": 1, \"entries\": {\n";
+        size_t k = 0;
+        for (const auto& [key, e] : m) {
+            char buf[512];
+            std::snprintf(buf, sizeof(buf),
+                          "  \"%s\": {\"nn_search_radius\": %d, \"block_size\": %d, \"sort_digit_bits\": %d, "
+                          "\"build_ms\": %.6f, \"sah_cost\": %.9g}%s\n",
+                          key.c_str(), e.radius, e.block, e.digit, e.build_ms, e.sah, ++k < m.size() ? "," : "");
+            f << buf;
+        }
+        f << "}}\n";
+    }
+    if (std::rename(tmp.c_str(), path) != 0) throw Error(DPL_ERR_INVALID_ARG, "cannot replace the autotune cache");
+}
+
+int FloorLog2(int64_t v) {
+    int r = 0;
+    while (v > 1) {
+        v >>= 1;
+        r++;
+    }
+    return r;
+}
+
+}  // namespace
+
+dpl_status dpl_autotune(dpl_context* ctx, const float* tri, int64_t n, const int64_t* offsets, int32_t mesh_count,
+                        dpl_mem mem, const dpl_build_params* base, double sah_tolerance, const char* cache_path,
+                        dpl_build_params* best, dpl_autotune_info* info) {
+    if (!ctx || !base || !best) return ctx ? Fail(ctx, DPL_ERR_INVALID_ARG, "null argument") : DPL_ERR_INVALID_ARG;
+    return Guard(ctx, [&] {
+        const dpl_build_params b0 = Normalize(base);
+        const bool batched = offsets != nullptr;
+        Require(tri != nullptr, "tri_xyz is null");
+        Require(sah_tolerance >= 0.0, "sah_tolerance must be non-negative");
+        const int64_t N = batched ? (mesh_count >= 1 ? offsets[mesh_count] : 0) : n;
+        Require(N >= 1, "at least one triangle is required");
+        std::ostringstream key;
+        key << (batched ? "batched" : "single") << "/n2^" << FloorLog2(N) << "/m2^"
+            << FloorLog2(batched ? mesh_count : 1) << "/" << (b0.morton_type == DPL_EMC60 ? "emc60" : "morton30") << "/"
+            << (b0.bv_type == DPL_BV_SOBB ? "sobb" + std::to_string(b0.kdop_size) : std::string("aabb")) << "/leaf"
+            << b0.max_triangles_per_leaf << "/" << ctx->device_name;
+        std::map<std::string, TuneEntry> cache;
+        if (cache_path) cache = ReadTuneCache(cache_path);
+        dpl_autotune_info inf{};
+        auto hit = cache.find(key.str());
+        if (hit != cache.end()) {
+            *best = b0;
+            best->nn_search_radius = hit->second.radius;
+            best->block_size = hit->second.block;
+            best->sort_digit_bits = hit->second.digit;
+            inf.build_ms = hit->second.build_ms;
+            inf.sah_cost = hit->second.sah;
+            inf.from_cache = 1;
+            if (info) *info = inf;
+            return;
+        }
+        // stage host input once so that the timed builds read device memory
+        const float* dtri = tri;
+        DevBuf staged;
+        if (mem == DPL_MEM_HOST) {
+            float* d = staged.ensure<float>(size_t(N) * 36);
+            DPL_HIP(hipMemcpy(d, tri, size_t(N) * 36, hipMemcpyHostToDevice));
+            dtri = d;
+        }
+        dpl_bvh scratch;
+        scratch.ctx = ctx;
+        std::vector<std::pair<dpl_build_params, TuneEntry>> runs;
+        const std::vector<int> blocks = batched ? std::vector<int>{256, 512} : std::vector<int>{128, 256, 512};
+        for (int r : {8, 16, 32})
+            for (int blk : blocks)
+                for (int dg : {4, 8}) {
+                    dpl_build_params q = b0;
+                    q.nn_search_radius = r;
+                    q.block_size = blk;
+                    q.sort_digit_bits = dg;
+                    q.keep_intermediates = 0;
+                    double ms = 1e30, sah = 0.0;
+                    for (int rep = 0; rep < 3; rep++) {  // the first build warms up
+                        dpl_build_stats st{};
+                        if (batched) BatchImpl(ctx, &scratch, dtri, offsets, mesh_count, DPL_MEM_DEVICE, q, &st);
+                        else BuildImpl(ctx, &scratch, dtri, n, DPL_MEM_DEVICE, q, &st);
+                        if (rep > 0) ms = std::min(ms, st.build_ms);
+                        sah = st.sah_cost;
+                    }
+                    runs.push_back({q, TuneEntry{r, blk, dg, ms, sah}});
+                }
+        double min_sah = 1e300;
+        for (const auto& r : runs) min_sah = std::min(min_sah, r.second.sah);
+        const double limit = min_sah * (1.0 + sah_tolerance);
+        size_t pick = runs.size();
+        for (size_t k = 0; k < runs.size(); k++)
+            if (runs[k].second.sah <= limit && (pick == runs.size() || runs[k].second.build_ms < runs[pick].second.build_ms))
+                pick = k;
+        *best = runs[pick].first;
+        best->keep_intermediates = b0.keep_intermediates;
+        inf.build_ms = runs[pick].second.build_ms;
+        inf.sah_cost = runs[pick].second.sah;
+        inf.candidates = int32_t(runs.size());
+        if (cache_path) {
+            cache[key.str()] = runs[pick].second;
+            WriteTuneCache(cache_path, cache);
+        }
+        if (info) *info = inf;
+    });
+}

@@ -259,6 +259,62 @@ struct TraceTree {
     const float* dirs;    // SOBB trees: ndirs x 3 unit normals (device)
 };
 
+// ---- batched build: many meshes, each built by one workgroup of one persistent launch ----
+struct MeshResult {
+    int32_t kept;         // kept inner nodes (0: the mesh is one leaf)
+    int32_t leaves;
+    int32_t root_height;
+    int32_t iterations;   // PLOC iterations
+    uint32_t flags;       // bit 0: non-finite coordinates; bit 1: an iteration without a merge
+    int32_t pad;
+    double sah;           // sum of the final nodes' SAH terms (before the root normalisation)
+    float root_lo[3], root_hi[3];
+};
+
+struct BatchArgs {
+    const float* tri;          // 9 floats per triangle, meshes back to back
+    const int64_t* tri_off;    // mesh_count + 1: first triangle of each mesh
+    const int64_t* fnode_off;  // mesh_count + 1: first final node (max(1, n - 1) per mesh)
+    const int32_t* order;      // mesh ids by decreasing triangle count (the work queue)
+    int32_t mesh_count;
+    uint32_t* queue;           // next job (zero before the launch)
+    // per triangle
+    uint64_t* keys;
+    uint64_t* keys_alt;
+    uint32_t* vals;
+    uint32_t* vals_alt;
+    PNode* clusters0;
+    PNode* clusters1;
+    int32_t* leafslot;
+    int32_t* keptflag;
+    float4* tri_ordered;
+    int32_t* prim_order;
+    // per PLOC node: mesh m's 2 n_m - 1 nodes start at 2 tri_off[m] - m
+    PNode* nodes;
+    int32_t* parent;
+    int32_t* count;
+    float* cost;
+    int32_t* minleaf;
+    int32_t* height;
+    uint8_t* collapsed;
+    FNode* fnodes;
+    MeshResult* results;       // per mesh (zeroed before the launch)
+    int morton_type, radius, max_leaf, digit_bits;
+};
+
+hipError_t LaunchBatchBuild(const BatchArgs& a, int block, int grid, hipStream_t s);
+int BatchWgPerCu(int block);
+
+// Per-mesh tree of a batched BVH, for the batched trace.
+struct BatchTree {
+    int64_t node_off;  // first final node
+    int64_t tri_off;   // first ordered triangle
+    int32_t node_count;
+    int32_t prim_count;
+    int32_t height;
+    int32_t pad;
+};
+
 constexpr int kMaxDirs = 13;
 constexpr int kMaxTriples = 286;
 // The compile-time triple table of the 13-direction (26-DOP) set, for a host-side check.
@@ -268,6 +324,12 @@ void SobbTriple13(int t, int* a, int* b, int* c, float* inv_det);
 // counters (nullable, device): accumulates {inner nodes visited, triangle tests}.
 hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,
                        int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters, hipStream_t s);
+// Batched trace: ray i against mesh ray_mesh[i] of a batched BVH (per-mesh trees in mt; a
+// mesh id out of range gives a miss); max_height sizes the spill stacks.
+hipError_t LaunchTraceBatched(const FNode* nodes, const float4* tris, const BatchTree* mt, int32_t mesh_count,
+                              int32_t max_height, const dpl_ray* rays, const int32_t* ray_mesh, int64_t n,
+                              uint32_t flags, dpl_hit* hits, int cu_count, uint32_t* counter, int32_t* spill,
+                              hipStream_t s);
 // Bytes of the per-lane stack spill region the trace and render kernels need for a tree.
 size_t TraceSpillBytes(int cu_count, int height);
 hipError_t LaunchRender(bool adjoint, const TraceTree& tree, const dpl_render_params& rp, const float* basis,

@@ -15,11 +15,14 @@ constexpr int kTraceBlock = 256;
 constexpr int kTraceWgPerCu = 8;
 }  // namespace
 
-template <int BLOCK, bool COUNT, bool SOBB>
+// BATCH: ray i is traced against mesh ray_mesh[i], whose tree starts at the per-mesh offsets
+// of mt (tree.nodes / tree.tris are then the bases of all meshes' nodes and triangles).
+template <int BLOCK, bool COUNT, bool SOBB, bool BATCH>
 __global__ __launch_bounds__(BLOCK) void KTrace(TraceTree tree, const float4* __restrict__ rays, int64_t n,
                                                 uint32_t flags, dpl_hit* __restrict__ hits, uint32_t* counter,
                                                 int32_t* spill, int32_t spill_stride,
-                                                unsigned long long* counters) {
+                                                unsigned long long* counters, const int32_t* __restrict__ ray_mesh,
+                                                const BatchTree* __restrict__ mt, int32_t mesh_count) {
     __shared__ int32_t s_stack[kStackLds][BLOCK];
     __shared__ uint32_t s_base[BLOCK / kWave];
     __shared__ float s_on[SOBB ? kMaxDirs : 1][BLOCK];
@@ -46,8 +49,23 @@ __global__ __launch_bounds__(BLOCK) void KTrace(TraceTree tree, const float4* __
             const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
             const WtRay w = PrepareRay(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w);
             if (SOBB) PrepareSobb<BLOCK>(sr, s_dirs, tree.ndirs, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z);
-            const HitResult h = any ? Traverse<true, SOBB>(tree, w, sr, cull, st, cnt)
-                                    : Traverse<false, SOBB>(tree, w, sr, cull, st, cnt);
+            TraceTree t = tree;
+            bool valid = true;
+            if (BATCH) {
+                const int32_t m = ray_mesh[i];
+                valid = m >= 0 && m < mesh_count;
+                if (valid) {
+                    const BatchTree bt = mt[m];
+                    t.nodes = tree.nodes + bt.node_off;
+                    t.tris = tree.tris + 3 * bt.tri_off;
+                    t.node_count = bt.node_count;
+                    t.prim_count = bt.prim_count;
+                    t.height = bt.height;
+                }
+            }
+            HitResult h{0.0f, 0.0f, 0.0f, -1, -1};
+            if (valid)
+                h = any ? Traverse<true, SOBB>(t, w, sr, cull, st, cnt) : Traverse<false, SOBB>(t, w, sr, cull, st, cnt);
             dpl_hit o;
             o.t = h.t;
             o.b1 = h.b1;
@@ -101,8 +119,8 @@ hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, ui
     const int grid = TraceGrid(cu_count, n);
     const int32_t stride = TraceSpillStride(tree.height);
 #define DPL_TRACE(COUNT_, SOBB_)                                                                                   \
-    hipLaunchKernelGGL((KTrace<kTraceBlock, COUNT_, SOBB_>), dim3(grid), dim3(kTraceBlock), 0, s, tree, r4, n, flags, \
-                       hits, counter, spill, stride, counters)
+    hipLaunchKernelGGL((KTrace<kTraceBlock, COUNT_, SOBB_, false>), dim3(grid), dim3(kTraceBlock), 0, s, tree, r4, n, \
+                       flags, hits, counter, spill, stride, counters, nullptr, nullptr, 0)
     const bool sobb = tree.ndirs > 0;
     if (counters) {
         if (sobb) DPL_TRACE(true, true);
@@ -112,6 +130,21 @@ hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, ui
         else DPL_TRACE(false, false);
     }
 #undef DPL_TRACE
+    return hipGetLastError();
+}
+
+hipError_t LaunchTraceBatched(const FNode* nodes, const float4* tris, const BatchTree* mt, int32_t mesh_count,
+                              int32_t max_height, const dpl_ray* rays, const int32_t* ray_mesh, int64_t n,
+                              uint32_t flags, dpl_hit* hits, int cu_count, uint32_t* counter, int32_t* spill,
+                              hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (n > int64_t(0xFFFFFFFFu) - 64) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(KResetCounter, dim3(1), dim3(1), 0, s, counter, nullptr);
+    TraceTree tree{nodes, tris, 0, 0, max_height, 0, nullptr};
+    const int grid = TraceGrid(cu_count, n);
+    hipLaunchKernelGGL((KTrace<kTraceBlock, false, false, true>), dim3(grid), dim3(kTraceBlock), 0, s, tree,
+                       reinterpret_cast<const float4*>(rays), n, flags, hits, counter, spill,
+                       TraceSpillStride(max_height), nullptr, ray_mesh, mt, mesh_count);
     return hipGetLastError();
 }
 

@@ -140,10 +140,32 @@ dpl_status dpl_bvh_rebuild(dpl_bvh* bvh, const float* tri_xyz, int64_t n, dpl_me
                            const dpl_build_params* params, dpl_build_stats* stats);
 
 /* Builds mesh_count independent BVHs over tri_xyz[offsets[m]..offsets[m+1]) into one handle
- * (all meshes in one pass per stage). offsets has mesh_count+1 entries in host memory. */
+ * in ONE persistent launch: each workgroup takes whole meshes from a queue (largest first)
+ * and runs every stage for its mesh. Each mesh's tree equals dpl_bvh_build of that mesh alone
+ * (codes, order, merges, final nodes, triangle order; primitive ids local to the mesh).
+ * mesh_offsets: mesh_count+1 host int64, offsets[0] = 0, strictly increasing. AABB bounds
+ * only (SOBB: DPL_ERR_UNSUPPORTED); block_size 256 or 512. There is no per-mesh counterpart
+ * in the reference, whose factory builds one structure per scene
+ * (src/acceleration/acceleration_structure_factory.cc:9-19). stats: sums over meshes, with
+ * sah_cost the triangle-weighted mean of the per-mesh SAH costs. */
 dpl_status dpl_bvh_build_batched(dpl_context* ctx, const float* tri_xyz, const int64_t* mesh_offsets,
                                  int32_t mesh_count, dpl_mem mem, const dpl_build_params* params, dpl_bvh** out,
                                  dpl_build_stats* stats);
+
+/* Per-mesh view of a batched BVH (a single-scene BVH reports one mesh). Final nodes of mesh m
+ * are nodes[node_offset .. node_offset + node_count) of dpl_bvh_download, its triangles
+ * tri_ordered / prim_order[prim_offset .. prim_offset + prim_count). */
+typedef struct dpl_mesh_info {
+    int64_t node_offset;
+    int64_t prim_offset;
+    int32_t node_count;
+    int32_t prim_count;
+    int32_t tree_height;
+    int32_t ploc_iterations;
+    double sah_cost;
+} dpl_mesh_info;
+
+dpl_status dpl_bvh_get_meshes(const dpl_bvh* bvh, dpl_mesh_info* out /* mesh_count entries */);
 
 void dpl_bvh_destroy(dpl_bvh* bvh);
 dpl_status dpl_bvh_get_stats(const dpl_bvh* bvh, dpl_build_stats* stats);
@@ -161,9 +183,39 @@ dpl_status dpl_trace(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, 
 dpl_status dpl_trace_counters(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
                               dpl_hit* hits, dpl_mem mem, int64_t* node_visits, int64_t* triangle_tests);
 
-/* Batched BVH: ray i is traced against mesh ray_mesh[i] (device or host as `mem`). */
+/* Batched BVH: ray i is traced against mesh ray_mesh[i] (device or host as `mem`); hit prim
+ * ids are local to that mesh; a mesh id out of range gives a miss. */
 dpl_status dpl_trace_batched(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, const int32_t* ray_mesh,
                              int64_t n, uint32_t flags, dpl_hit* hits, dpl_mem mem);
+
+/* A trace pass captured once in a hipGraph and replayed with one launch: rays, ray_mesh
+ * (NULL for a single-scene BVH) and hits are device buffers whose addresses the graph keeps;
+ * their contents may change between launches. The graph owns its ray counter and spill
+ * stacks, so the context's other calls may run between launches. Replaces the per-frame
+ * LaunchRaytracingKernel call of GpuRenderer::StartRender (src/renderer/gpu_renderer.cc:53). */
+typedef struct dpl_trace_graph dpl_trace_graph;
+dpl_status dpl_trace_graph_create(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, const int32_t* ray_mesh,
+                                  int64_t n, uint32_t flags, dpl_hit* hits, dpl_trace_graph** out);
+dpl_status dpl_trace_graph_launch(dpl_trace_graph* graph); /* on the context's stream */
+void dpl_trace_graph_destroy(dpl_trace_graph* graph);
+
+/* Autotuner: times builds of the given workload over nn_search_radius {8, 16, 32}, block_size
+ * {128, 256, 512} ({256, 512} batched) and sort_digit_bits {4, 8}, and returns in *best the
+ * fastest whose SAH cost is at most (1 + sah_tolerance) x the lowest SAH among them (the
+ * other fields as in *base). Results are cached in the JSON file cache_path (NULL: no cache)
+ * under a key of workload kind, log2 triangle and mesh counts, code and bounds types, leaf
+ * size and device; a cached entry is returned without building. The reference has no
+ * counterpart (its SBvh parameters are fixed in acceleration_structure_config.h:22-29). */
+typedef struct dpl_autotune_info {
+    double build_ms;   /* device build time of the chosen parameters */
+    double sah_cost;
+    int32_t candidates; /* configurations timed (0 when served from the cache) */
+    int32_t from_cache;
+} dpl_autotune_info;
+
+dpl_status dpl_autotune(dpl_context* ctx, const float* tri_xyz, int64_t n, const int64_t* mesh_offsets,
+                        int32_t mesh_count, dpl_mem mem, const dpl_build_params* base, double sah_tolerance,
+                        const char* cache_path, dpl_build_params* best, dpl_autotune_info* info);
 
 /* Sizes (in elements) of the arrays dpl_bvh_download fills; any count may be zero when the
  * BVH was built without keep_intermediates. */
