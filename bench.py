@@ -45,6 +45,7 @@ def main() -> int:
     ap.add_argument("--no-sweep", action="store_true", help="skip the variant x radius sweep")
     ap.add_argument("--radius", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 4096-mesh batched build and trace")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run three plain 256 MiB device copies (a known-size kernel for HBM counter scaling)")
     args = ap.parse_args()
@@ -186,6 +187,49 @@ def main() -> int:
         f"adjoint {adj_ms:.3f} ms ({adj_ms / fwd_ms:.2f}x)")
     rb.close()
 
+    # ---- batched builds: 4096 meshes (log-uniform 64..65536 tris) in one persistent
+    #      PLOC+EMC launch under autotuned radius / block / digit width, then a
+    #      hipGraph-captured trace of 256 rays per mesh ----
+    batch = None
+    if not args.no_batch:
+        t0 = time.time()
+        btris_np, boff = scenes.batched_meshes()
+        brays_np, bids_np = scenes.batched_rays(btris_np, boff, 256)
+        log(f"batch: {len(boff) - 1} meshes, {boff[-1]} tris generated in {time.time() - t0:.1f}s")
+        btris = torch.from_numpy(btris_np).to(dev)
+        del btris_np
+        bparams, tune = accel.autotune(ctx, btris, accel.make_params(morton_type=1, radius=16, max_leaf=4),
+                                       offsets=boff, sah_tolerance=0.05)
+        gb = accel.GpuBvh(ctx)
+        bms = []
+        for _ in range(4):
+            gb.build_batched(btris, boff, bparams)
+            bms.append(gb.stats.build_ms)
+        bms = bms[1:]
+        brays = torch.from_numpy(brays_np).to(dev)
+        bids = torch.from_numpy(bids_np).to(dev)
+        bhits = torch.empty((len(bids_np), 4), dtype=torch.float32, device=dev)
+        graph = accel.TraceGraph(gb, brays, bhits, ray_mesh=bids)
+        graph_ms = timed(graph.launch, 10)
+        eager_ms = timed(lambda: gb.trace_batched(brays, bids, 0, bhits), 10)
+        nb = int(boff[-1])
+        batch = {"workload": f"{len(boff) - 1} meshes, log-uniform 64..65536 tris (noise spheres / soups), {nb:,} tris; "
+                             f"256 rays per mesh", "meshes": len(boff) - 1, "triangles": nb,
+                 "autotuned": {"nn_search_radius": bparams.nn_search_radius, "block_size": bparams.block_size,
+                               "sort_digit_bits": bparams.sort_digit_bits, "candidates": tune["candidates"]},
+                 "build_ms": round(float(np.median(bms)), 4),
+                 "build_mtris_s": round(nb / (float(np.median(bms)) * 1e-3) / 1e6, 2),
+                 "sah_cost_mean": gb.stats.sah_cost,
+                 "trace_rays": len(bids_np), "trace_graph_ms": round(graph_ms, 4),
+                 "trace_eager_ms": round(eager_ms, 4),
+                 "trace_mrays_s": round(len(bids_np) / (graph_ms * 1e-3) / 1e6, 2)}
+        log(f"batch: build {batch['build_ms']:.2f} ms ({batch['build_mtris_s']:.0f} Mtris/s, r="
+            f"{bparams.nn_search_radius} block={bparams.block_size} digit={bparams.sort_digit_bits}), trace graph "
+            f"{graph_ms:.3f} ms / eager {eager_ms:.3f} ms for {len(bids_np)} rays")
+        graph.close()
+        gb.close()
+        del btris, brays, bids, bhits
+
     # ---- variant x radius sweep of the 2M scene (build time, SAH, trace rate) ----
     sweep = []
     if not args.no_sweep:
@@ -265,6 +309,7 @@ def main() -> int:
         "primary_workload": "icosphere_room 69,668 tris, 1024x1024 camera rays, PLOC Morton30 r=16",
         "primary_bounce_ms": round(bounce_ms, 4),
         "differentiable": diff,
+        "batched": batch,
         "build_ms": round(float(build_ms.mean()), 4),
         "trace_ms": round(float(trace_ms.mean()), 4),
         "sah_cost": last["sah_cost"],

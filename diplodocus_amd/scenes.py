@@ -321,6 +321,30 @@ def random_rays_towards(tris: np.ndarray, n: int, seed: int = 11, jitter: float 
     return r
 
 
+def batched_rays(tris: np.ndarray, offsets: np.ndarray, per_mesh: int, seed: int = 13, jitter: float = 0.2):
+    """random_rays_towards for every mesh of a batch at once: per_mesh rays per mesh from a
+    sphere around that mesh, aimed at its centre. Returns (rays (M*per_mesh, 8) float32,
+    ray_mesh (M*per_mesh,) int32), mesh-major."""
+    rng = np.random.default_rng(seed)
+    m = len(offsets) - 1
+    v = tris.reshape(-1, 9)
+    lo = np.minimum.reduceat(np.minimum(np.minimum(v[:, 0:3], v[:, 3:6]), v[:, 6:9]), offsets[:-1]).astype(np.float64)
+    hi = np.maximum.reduceat(np.maximum(np.maximum(v[:, 0:3], v[:, 3:6]), v[:, 6:9]), offsets[:-1]).astype(np.float64)
+    ids = np.repeat(np.arange(m, dtype=np.int32), per_mesh)
+    c = ((lo + hi) / 2)[ids]
+    ext = (hi - lo)[ids]
+    rad = 1.2 * np.linalg.norm(ext, axis=1, keepdims=True) / 2
+    o = c + _unit_vectors(rng, len(ids)) * rad
+    d = c + rng.uniform(-1, 1, size=(len(ids), 3)) * ext * jitter - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    r = np.empty((len(ids), 8), np.float32)
+    r[:, 0:3] = o
+    r[:, 3] = 1e-6
+    r[:, 4:7] = d
+    r[:, 7] = np.finfo(np.float32).max
+    return r, ids
+
+
 def camera_rays(cam: Camera, w: int, h: int) -> np.ndarray:
     """One primary ray per pixel, row-major, following src/renderer/cpu_raytracer.cc:62-73
     and :99-106 (no pixel-centre offset), evaluated in float32 on the host."""
