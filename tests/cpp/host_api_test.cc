@@ -1,0 +1,231 @@
+// Drives the C++ host interface the way the reference's renderer drives an
+// AccelerationStructure (src/renderer/cpu_raytracer.cc:56-57,112,236): build through the
+// factory, per-ray Intersect / IntersectAny, plus the batched calls, and the path tracer.
+//   host_api_test --cpu   interface checks that need no GPU
+//   host_api_test         the GPU checks; hits are compared with the oracle's brute force
+// Prints one line per check group and "host_api_test: OK" at the end; exits non-zero on the
+// first failure.
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+#include "dpl_host.h"
+
+extern "C" void orc_trace_brute(const float* tri, int64_t ntri, const void* rays, int64_t n, uint32_t flags, void* hits,
+                                int threads);
+
+using namespace diplodocus_amd;
+
+namespace {
+
+int g_failures = 0;
+
+void Expect(bool ok, const std::string& what) {
+    if (!ok) {
+        std::printf("FAILED: %s\n", what.c_str());
+        std::fflush(stdout);
+        g_failures++;
+    }
+}
+
+struct Lcg {
+    uint64_t s;
+    float Next() {
+        s = s * 6364136223846793005ull + 1442695040888963407ull;
+        return float((s >> 40) & 0xFFFFFF) / 16777216.0f;
+    }
+};
+
+// A closed box (12 triangles, material 0..5 by face) around a seeded soup of small
+// triangles (material 6), some with vertex normals.
+std::vector<Triangle> MakeScene(int n_soup, uint64_t seed) {
+    std::vector<Triangle> tris;
+    auto quad = [&](Vec3 a, Vec3 b, Vec3 c, Vec3 d, int mat) {
+        for (int k = 0; k < 2; k++) {
+            Triangle t;
+            t.v0.pos = a;
+            t.v1.pos = k ? c : b;
+            t.v2.pos = k ? d : c;
+            t.geom_normal = Normalize(Cross(t.v1.pos - t.v0.pos, t.v2.pos - t.v0.pos));
+            t.material_id = mat;
+            tris.push_back(t);
+        }
+    };
+    const float L = 0.0f, H = 1.0f;
+    quad({L, L, L}, {H, L, L}, {H, L, H}, {L, L, H}, 0);  // floor
+    quad({L, H, L}, {L, H, H}, {H, H, H}, {H, H, L}, 1);  // ceiling
+    quad({L, L, H}, {H, L, H}, {H, H, H}, {L, H, H}, 2);  // back
+    quad({L, L, L}, {L, L, H}, {L, H, H}, {L, H, L}, 3);  // left
+    quad({H, L, L}, {H, H, L}, {H, H, H}, {H, L, H}, 4);  // right
+    quad({L, L, L}, {L, H, L}, {H, H, L}, {H, L, L}, 5);  // front (the camera is inside)
+    Lcg r{seed};
+    for (int i = 0; i < n_soup; i++) {
+        const Vec3 c{0.2f + 0.6f * r.Next(), 0.2f + 0.6f * r.Next(), 0.2f + 0.6f * r.Next()};
+        Triangle t;
+        Vertex* v[3] = {&t.v0, &t.v1, &t.v2};
+        for (int k = 0; k < 3; k++)
+            v[k]->pos = c + Vec3{0.04f * (r.Next() - 0.5f), 0.04f * (r.Next() - 0.5f), 0.04f * (r.Next() - 0.5f)};
+        t.geom_normal = Normalize(Cross(t.v1.pos - t.v0.pos, t.v2.pos - t.v0.pos));
+        t.material_id = 6;
+        if (i % 3 == 0) {
+            t.has_vertex_normals = true;
+            for (int k = 0; k < 3; k++) v[k]->normal = Normalize(v[k]->pos - Vec3{0.5f, 0.5f, 0.5f});
+        }
+        tris.push_back(t);
+    }
+    return tris;
+}
+
+std::vector<Ray> CameraRays(int w, int h) {
+    // src/renderer/cpu_raytracer.cc:62-73 with pos (0.5, 0.5, 0.05) looking along +z
+    const Vec3 pos{0.5f, 0.5f, 0.05f}, dir{0, 0, 1}, up{0, 1, 0};
+    const float fov = 1.2f;
+    const Vec3 right = Normalize(Cross(dir, up));
+    const float gw = float(2.0 * std::tan(fov / 2.0f)), gh = gw * (float(h) / float(w));
+    const Vec3 qw = right * (gw / float(w - 1)), qh = -up * (gh / float(h - 1));
+    const Vec3 p00 = dir - right * (gw / 2) + up * (gh / 2);
+    std::vector<Ray> rays;
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) rays.push_back(Ray{pos, Normalize(p00 + qw * float(x) + qh * float(y)), 100.0f});
+    return rays;
+}
+
+void CpuChecks() {
+    Expect(ParseAccelerationStructureType("ploc_emc_sobb") == AccelerationStructureType::kPlocEmcSobb, "parse");
+    Expect(ParseAccelerationStructureType("sbvh") == AccelerationStructureType::kSbvh, "parse sbvh");
+    bool threw = false;
+    try {
+        ParseAccelerationStructureType("bvh8");
+    } catch (const std::runtime_error&) {
+        threw = true;
+    }
+    Expect(threw, "parse rejects unknown names");
+    const AccelerationStructureConfig cfg;
+    Expect(cfg.nn_search_radius == 10 && cfg.kdop_size == 32 && cfg.max_triangles_per_leaf == 4 &&
+               cfg.bin_count == 16 && cfg.max_depth == 32,
+           "config defaults");
+    // RayEpsilon: clamp(max(1e-6 max(|p|inf, 1), 1e-7 max(t, 1)), 1e-7, 1e-3)
+    Expect(RayEpsilon({0, 0, 0}, 0.5f) == 1e-6f, "epsilon at the origin");
+    Expect(RayEpsilon({-300.0f, 2, 1}, 1.0f) == std::clamp(1e-6f * 300.0f, 1e-7f, 1e-3f), "epsilon scales with |p|");
+    Expect(RayEpsilon({1e6f, 0, 0}, 1.0f) == 1e-3f, "epsilon upper clamp");
+    Expect(RayEpsilon({0, 0, 0}, 5e3f) == 1e-7f * 5e3f, "epsilon scales with t");
+    threw = false;
+    try {
+        AccelerationStructureConfig c;
+        c.acceleration_structure_type = AccelerationStructureType::kSbvh;
+        CreateGpuAccelerationStructure(c, nullptr);
+    } catch (const std::invalid_argument&) {
+        threw = true;
+    }
+    Expect(threw, "factory rejects the CPU structure types");
+    std::printf("cpu checks: %s\n", g_failures ? "failed" : "ok");
+}
+
+void GpuChecks() {
+    auto ctx = std::make_shared<Context>(0);
+    const std::vector<Triangle> tris = MakeScene(20000, 7);
+    std::vector<float> xyz;
+    for (const Triangle& t : tris)
+        for (const Vec3* v : {&t.v0.pos, &t.v1.pos, &t.v2.pos}) xyz.insert(xyz.end(), {v->x, v->y, v->z});
+    const std::vector<Ray> rays = CameraRays(160, 120);
+    std::vector<dpl_ray> rr;
+    for (const Ray& r : rays) rr.push_back({r.origin.x, r.origin.y, r.origin.z, kEpsilon, r.dir.x, r.dir.y, r.dir.z, r.t_max});
+    std::vector<dpl_hit> brute(rays.size());
+    orc_trace_brute(xyz.data(), int64_t(tris.size()), rr.data(), int64_t(rr.size()), 0, brute.data(), 8);
+
+    for (const char* name : {"ploc", "ploc_emc", "ploc_sobb", "ploc_emc_sobb"}) {
+        AccelerationStructureConfig cfg;
+        cfg.acceleration_structure_type = ParseAccelerationStructureType(name);
+        cfg.nn_search_radius = 16;
+        std::unique_ptr<AccelerationStructure> as = CreateGpuAccelerationStructure(cfg, ctx);
+        Stats stats;
+        as->Build(cfg, stats, tris);
+        Expect(stats.accel_stats.build_time > 0.0 && stats.accel_stats.node_count ==
+                   stats.accel_stats.inner_node_count + stats.accel_stats.leaf_node_count,
+               std::string(name) + ": build stats");
+        auto* gpu = dynamic_cast<GpuBvh*>(as.get());
+        std::vector<RayHit> hits(rays.size());
+        const int64_t nhit = gpu->IntersectBatch(stats, rays, hits, false);
+        Expect(stats.accel_stats.query_count == int(rays.size()) && stats.accel_stats.traversal_count > 0,
+               std::string(name) + ": query stats");
+        int64_t mismatch = 0, expect_hits = 0;
+        for (size_t i = 0; i < rays.size(); i++) {
+            expect_hits += brute[i].prim >= 0;
+            if (hits[i].triangle_id != brute[i].prim) mismatch++;
+            if (hits[i].triangle_id >= 0) {
+                const Triangle& t = tris[size_t(hits[i].triangle_id)];
+                if (hits[i].t != brute[i].t || hits[i].material_id != t.material_id ||
+                    std::fabs(hits[i].b0 + hits[i].b1 + hits[i].b2 - 1.0f) > 1e-5f)
+                    mismatch++;
+            }
+        }
+        Expect(mismatch == 0 && nhit == expect_hits, std::string(name) + ": batched hits equal brute force");
+        // the per-ray slow path agrees with the batch
+        int64_t bad = 0;
+        for (size_t i = 0; i < rays.size(); i += 97) {
+            RayHit h;
+            const bool hit = as->Intersect(stats, rays[i], h, false);
+            if (hit != (hits[i].triangle_id >= 0) || (hit && (h.triangle_id != hits[i].triangle_id || h.t != hits[i].t)))
+                bad++;
+            if (as->IntersectAny(stats, rays[i], false) != hit) bad++;
+        }
+        std::vector<uint8_t> occ(rays.size());
+        gpu->IntersectAnyBatch(stats, rays, occ, false);
+        for (size_t i = 0; i < rays.size(); i++) bad += occ[i] != (hits[i].triangle_id >= 0);
+        Expect(bad == 0, std::string(name) + ": Intersect / IntersectAny agree with the batch");
+        std::printf("%s: %lld hits of %zu rays, build %.3f ms, %d nodes\n", name, (long long)nhit, rays.size(),
+                    stats.accel_stats.build_time, stats.accel_stats.node_count);
+        std::fflush(stdout);
+    }
+
+    // path tracing and its adjoint through the same BVH
+    AccelerationStructureConfig cfg;
+    cfg.acceleration_structure_type = AccelerationStructureType::kPlocEmc;
+    GpuBvh bvh(ctx);
+    Stats stats;
+    bvh.Build(cfg, stats, tris);
+    const int M = 7;
+    std::vector<float> albedo(3 * M, 0.5f), emission(3 * M, 0.0f);
+    emission[3 * 1] = emission[3 * 1 + 1] = emission[3 * 1 + 2] = 5.0f;  // the ceiling glows
+    GpuPathTracer pt(bvh, tris, M);
+    PathTraceConfig pc{64, 48, 4, 4, 3};
+    Camera cam{{0.5f, 0.5f, 0.05f}, {0, 0, 1}, {0, 1, 0}, 1.2f, 100.0f};
+    const std::vector<float> img = pt.Render(pc, cam, albedo, emission);
+    const std::vector<float> img2 = pt.Render(pc, cam, albedo, emission);
+    double sum = 0.0;
+    bool finite = true;
+    for (float v : img) {
+        sum += v;
+        finite &= std::isfinite(v);
+    }
+    Expect(finite && sum > 0.0 && img == img2, "path tracer: finite, lit and deterministic");
+    // L = sum(image): dL/dimage = 1; every albedo gradient is non-negative, the emitter's is positive
+    std::vector<float> ones(img.size(), 1.0f), ga(3 * M), ge(3 * M);
+    pt.Adjoint(pc, cam, albedo, emission, ones, ga, ge);
+    bool nonneg = true;
+    for (float g : ga) nonneg &= g >= 0.0f;
+    Expect(nonneg && ge[3] > 0.0f, "path tracer adjoint: gradient signs");
+    // emission enters linearly: L(emission) = sum(image) = dL/de . e for the one emitter
+    const double lin = double(ge[3]) * 5.0 + double(ge[4]) * 5.0 + double(ge[5]) * 5.0;
+    Expect(std::fabs(lin - sum) <= 1e-3 * std::fabs(sum), "path tracer adjoint: linearity in emission");
+    std::printf("path tracer: image sum %.4f, dL/de %.4f %.4f %.4f\n", sum, ge[3], ge[4], ge[5]);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const bool cpu_only = argc > 1 && std::string(argv[1]) == "--cpu";
+    CpuChecks();
+    if (!cpu_only) {
+        try {
+            GpuChecks();
+        } catch (const std::exception& e) {
+            std::printf("FAILED: exception %s\n", e.what());
+            g_failures++;
+        }
+    }
+    std::printf("host_api_test: %s\n", g_failures ? "FAILED" : "OK");
+    return g_failures ? 1 : 0;
+}
