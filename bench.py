@@ -32,6 +32,22 @@ def cpu_model() -> str:
     return platform.processor()
 
 
+def measured_traffic(variant: str, radius: int):
+    """HBM bytes per merging PLOC-iteration launch for this workload, as measured by the two
+    rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) summarised in the profile below, or None
+    when no measurement of this variant and radius is recorded there."""
+    path = os.path.join(ROOT, "profiles", "kploc_iteration_mixed2m_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                key, _, val = line.split(":", 1)[1].partition("=")
+                if key.strip() == f"{variant} r{radius}":
+                    return int(val.strip())
+    except OSError:
+        pass
+    return None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -122,7 +138,7 @@ def main() -> int:
         achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
         roof = {"kernel": "KPlocIteration<256> (one PLOC iteration: NN search, merge, compaction)",
                 "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.variant, args.radius),
                 "launches": ps.ploc_launches, "avg_launch_us": round(per_launch_ms * 1e3, 3),
                 "algorithmic_bytes_per_launch": round(per_launch_bytes)}
         log(f"ploc iteration kernel: {ps.ploc_launches} launches, {per_launch_ms * 1e3:.2f} us avg, "
