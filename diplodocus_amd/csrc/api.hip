@@ -155,7 +155,7 @@ struct dpl_context {
     std::string last_error;
     // scratch
     DevBuf tri, keys, vals, keys_alt, vals_alt, nodes, parent, clusters0, clusters1, ccount, status0, status1,
-        small, count, cost, collapsed, minleaf, height, leafslot, keptflag, scan_tmp, sort_tmp, rays, hits,
+        small, eval, state, kept_id, leafslot, keptflag, scan_tmp, sort_tmp, rays, hits,
         trace_counter, spill, kdop, sobb_set;
     // batched builds and traces
     DevBuf b_off, b_foff, b_order, b_results, b_queue, ray_mesh;
@@ -294,11 +294,8 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
         b.counters = reinterpret_cast<int32_t*>(sm + 128);       // 8
         b.sah = reinterpret_cast<double*>(sm + 192);             // 1
     }
-    b.count = C.count.ensure<int32_t>(size_t(nn) * 4);
-    b.cost = C.cost.ensure<float>(size_t(nn) * 4);
-    b.collapsed = C.collapsed.ensure<uint8_t>(size_t(nn));
-    b.minleaf = C.minleaf.ensure<int32_t>(size_t(nn) * 4);
-    b.height = C.height.ensure<int32_t>(size_t(nn) * 4);
+    b.eval = C.eval.ensure<EvalRec>(size_t(nn) * sizeof(EvalRec));
+    b.state = C.state.ensure<uint8_t>(size_t(nn));
     b.results = reinterpret_cast<BuildResults*>(C.pinned_dev);
     b.leafslot = C.leafslot.ensure<int32_t>(size_t(n) * 4);
     b.keptflag = C.keptflag.ensure<int32_t>(size_t(n) * 4);
@@ -306,6 +303,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     const size_t sort_bytes = RadixSortTempBytes(n, key_bits, p.sort_digit_bits, key64);
     void* sort_tmp = C.sort_tmp.ensure(sort_bytes);
     b.fnodes = bvh->fnodes.ensure<FNode>(size_t(std::max<int64_t>(1, n - 1)) * sizeof(FNode));
+    b.kept_id = p.bv_type == DPL_BV_SOBB ? C.kept_id.ensure<int32_t>(size_t(n) * 4) : nullptr;
     b.tri_ordered = bvh->tri_ordered.ensure<float4>(size_t(n) * 48);
     b.prim_order = bvh->prim_order.ensure<int32_t>(size_t(n) * 4);
     if (p.bv_type == DPL_BV_SOBB) {
@@ -647,11 +645,8 @@ void BatchImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, const int64_
     A.keptflag = C.keptflag.ensure<int32_t>(size_t(N) * 4);
     A.nodes = C.nodes.ensure<PNode>(size_t(nodes_total) * sizeof(PNode));
     A.parent = C.parent.ensure<int32_t>(size_t(nodes_total) * 4);
-    A.count = C.count.ensure<int32_t>(size_t(nodes_total) * 4);
-    A.cost = C.cost.ensure<float>(size_t(nodes_total) * 4);
-    A.minleaf = C.minleaf.ensure<int32_t>(size_t(nodes_total) * 4);
-    A.height = C.height.ensure<int32_t>(size_t(nodes_total) * 4);
-    A.collapsed = C.collapsed.ensure<uint8_t>(size_t(nodes_total));
+    A.eval = C.eval.ensure<EvalRec>(size_t(nodes_total) * sizeof(EvalRec));
+    A.state = C.state.ensure<uint8_t>(size_t(nodes_total));
     A.results = C.b_results.ensure<MeshResult>(size_t(M) * sizeof(MeshResult));
     A.fnodes = bvh->fnodes.ensure<FNode>(size_t(foff[M]) * sizeof(FNode));
     A.tri_ordered = bvh->tri_ordered.ensure<float4>(size_t(N) * 48);

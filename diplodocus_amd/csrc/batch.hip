@@ -62,11 +62,8 @@ __device__ __forceinline__ BuildBuffers MeshView(const BatchArgs& A, int m, int6
     v.vals_alt = A.vals_alt + off;
     v.nodes = A.nodes + nb;
     v.parent = A.parent + nb;
-    v.count = A.count + nb;
-    v.cost = A.cost + nb;
-    v.minleaf = A.minleaf + nb;
-    v.height = A.height + nb;
-    v.collapsed = A.collapsed + nb;
+    v.eval = A.eval + nb;
+    v.state = A.state + nb;
     v.clusters[0] = A.clusters0 + off;
     v.clusters[1] = A.clusters1 + off;
     v.leafslot = A.leafslot + off;
@@ -398,7 +395,7 @@ __global__ __launch_bounds__(kBT) void KBatchBuild(BatchArgs A) {
         uint32_t lt;
         BlockExclusiveScan<kBT>(uint32_t(leaves), S.scan, lt);
         __syncthreads();
-        for (int64_t id = threadIdx.x; id < nn; id += kBT) EmitNode(v, int32_t(id), A.max_leaf, K, nullptr, 0.0f);
+        for (int64_t id = threadIdx.x; id < nn; id += kBT) EmitNode(v, int32_t(id), K);
         __syncthreads();
 
         // ---- SAH terms and the mesh's results ----
@@ -414,7 +411,7 @@ __global__ __launch_bounds__(kBT) void KBatchBuild(BatchArgs A) {
             MeshResult& r = A.results[m];
             r.kept = K;
             r.leaves = int32_t(lt);
-            r.root_height = v.height[nn - 1];
+            r.root_height = v.eval[nn - 1].height;
             r.iterations = iters;
             r.sah = S.u.sah[0];
             if (bad) r.flags |= 1u;

@@ -149,7 +149,8 @@ hipError_t LaunchCodes(const BuildBuffers& b, int morton_type, hipStream_t s) {
 
 int SobbTriples13() { return kTriples13.n; }
 void SobbTriple13(int t, int* a, int* b, int* c, float* inv_det) {
-    *a = kTriples13.a[t], *b = kTriples13.b[t], *c = kTriples13.c[t], *inv_det = kTriples13.inv_det[t];
+    const Triple13& T = kTriples13.t[t];
+    *a = T.a, *b = T.b, *c = T.c, *inv_det = T.inv_det;
 }
 
 
@@ -403,17 +404,24 @@ __global__ __launch_bounds__(256) void KClassify(BuildBuffers b, int max_leaf) {
 
 // Gathers the primitives of every final leaf, and writes the node of every kept inner node.
 // keptflag has been scanned in place into exclusive ranks and leafslot into begin offsets.
-__global__ __launch_bounds__(256) void KEmit(BuildBuffers b, int max_leaf) {
+__global__ __launch_bounds__(256) void KEmit(BuildBuffers b) {
     const int64_t nn = 2 * b.n - 1;
     const int32_t K = b.counters[0];
-    __shared__ int4 s_tri[kMaxTriples];
-    if (b.ndirs > 0) {
-        for (int t = threadIdx.x; t < b.ntriples; t += blockDim.x) s_tri[t] = b.triples[t];
-        __syncthreads();
-    }
-    const float pad = b.ndirs > 0 ? KdopPad(b.bounds) : 0.0f;
     for (int64_t id = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; id < nn; id += int64_t(gridDim.x) * blockDim.x)
-        EmitNode(b, int32_t(id), max_leaf, K, s_tri, pad);
+        EmitNode(b, int32_t(id), K);
+}
+
+// SOBB trees: the child slabs of every final inner node, one thread per child.
+__global__ __launch_bounds__(256) void KFitSobb(BuildBuffers b) {
+    const int32_t K = b.counters[0];
+    __shared__ int4 s_tri[kMaxTriples];
+    __shared__ float s_w[kMaxDirs * 256];
+    for (int t = threadIdx.x; t < b.ntriples; t += blockDim.x) s_tri[t] = b.triples[t];
+    __syncthreads();
+    const float pad = KdopPad(b.bounds);
+    const int64_t m = K > 0 ? 2 * int64_t(K) : 1;
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < m; i += int64_t(gridDim.x) * blockDim.x)
+        FitChild(b, i, K, s_tri, pad, s_w + threadIdx.x, 256);
 }
 
 // SAH of the final tree in FP64 (Ct = Ci = 1, divided by SA(root) on the host); the node
@@ -447,7 +455,13 @@ hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s) {
         e = LaunchExclusiveScan(b.keptflag, b.keptflag, b.n - 1, b.scan_tmp, b.counters + 5, s);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL(KEmit, dim3(grid), dim3(256), 0, s, b, max_leaf);
+    hipLaunchKernelGGL(KEmit, dim3(grid), dim3(256), 0, s, b);
+    if (b.ndirs > 0) {
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        const int fgrid = int(std::min<int64_t>(4096, (2 * b.n + 255) / 256));
+        hipLaunchKernelGGL(KFitSobb, dim3(fgrid), dim3(256), 0, s, b);
+    }
     return hipGetLastError();
 }
 
@@ -457,7 +471,7 @@ __global__ void KFinish(BuildBuffers b) {
     const int64_t root = 2 * b.n - 2;
     r->kept = b.counters[0];
     r->leaves = b.counters[1];
-    r->root_height = b.height[root];
+    r->root_height = b.eval[root].height;
     r->sah = b.sah[0];
     r->flags = b.bounds[12];
     const PNode nd = b.nodes[root];

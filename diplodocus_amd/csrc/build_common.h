@@ -133,15 +133,12 @@ __device__ __forceinline__ NodeEval InnerEval(const float lo[3], const float hi[
 }
 
 __device__ __forceinline__ NodeEval LoadEval(const BuildBuffers& b, int32_t id) {
-    return NodeEval{b.count[id], b.cost[id], b.minleaf[id], b.height[id], b.collapsed[id]};
+    const EvalRec r = b.eval[id];
+    return NodeEval{r.count_col & 0x7FFFFFFF, r.cost, r.minleaf, r.height, uint8_t(uint32_t(r.count_col) >> 31)};
 }
 
 __device__ __forceinline__ void StoreEval(const BuildBuffers& b, int32_t id, const NodeEval& e) {
-    b.count[id] = e.count;
-    b.cost[id] = e.cost;
-    b.minleaf[id] = e.minleaf;
-    b.height[id] = e.height;
-    b.collapsed[id] = e.collapsed;
+    b.eval[id] = EvalRec{int32_t(uint32_t(e.count) | (uint32_t(e.collapsed) << 31)), e.cost, e.minleaf, e.height};
 }
 
 // ---------------------------------------------------------------------------------------
@@ -163,16 +160,30 @@ __device__ __forceinline__ float KdopPad(const uint32_t* bounds) {
 
 __device__ __forceinline__ int KdopWidth(int ndirs) { return 2 * (ndirs - 3); }
 
-__device__ __forceinline__ void LeafKdop(const BuildBuffers& b, const float v[9], float pad, float* kd) {
-    const int nx = b.ndirs - 3;
-    for (int k = 0; k < nx; k++) {
+template <int NX>
+__device__ __forceinline__ void LeafKdopT(const BuildBuffers& b, const float v[9], float pad, float* kd) {
+    float o[2 * NX];
+#pragma unroll
+    for (int k = 0; k < NX; k++) {
         const float* n = b.dirs + 3 * (3 + k);
         const float nx_ = n[0], ny = n[1], nz = n[2];
         const float p0 = nx_ * v[0] + ny * v[1] + nz * v[2];
         const float p1 = nx_ * v[3] + ny * v[4] + nz * v[5];
         const float p2 = nx_ * v[6] + ny * v[7] + nz * v[8];
-        kd[k] = Min2(Min2(p0, p1), p2) - pad;
-        kd[nx + k] = -(Max2(Max2(p0, p1), p2) + pad);
+        o[k] = Min2(Min2(p0, p1), p2) - pad;
+        o[NX + k] = -(Max2(Max2(p0, p1), p2) + pad);
+    }
+    float4* d = reinterpret_cast<float4*>(kd);  // 2 NX is a multiple of 4: whole 16-byte stores
+#pragma unroll
+    for (int q = 0; q < NX / 2; q++) d[q] = make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+}
+
+__device__ __forceinline__ void LeafKdop(const BuildBuffers& b, const float v[9], float pad, float* kd) {
+    switch (b.ndirs - 3) {
+        case 10: LeafKdopT<10>(b, v, pad, kd); break;
+        case 6: LeafKdopT<6>(b, v, pad, kd); break;
+        case 4: LeafKdopT<4>(b, v, pad, kd); break;
+        default: break;
     }
 }
 
@@ -194,10 +205,13 @@ constexpr float kS2 = 0.70710677f, kS3 = 0.57735026f;
 constexpr float kDirs13[13][3] = {{1, 0, 0},     {0, 1, 0},       {0, 0, 1},       {kS2, kS2, 0},    {kS2, -kS2, 0},
                                   {kS2, 0, kS2}, {kS2, 0, -kS2},  {0, kS2, kS2},   {0, kS2, -kS2},   {kS3, kS3, kS3},
                                   {kS3, kS3, -kS3}, {kS3, -kS3, kS3}, {-kS3, kS3, kS3}};
+struct alignas(16) Triple13 {
+    int32_t a, b, c;
+    float inv_det;  // float(1 / |det|), the division in FP64 rounded once
+};
 struct TripleTable13 {
     int n;
-    int8_t a[kMaxTriples], b[kMaxTriples], c[kMaxTriples];
-    float inv_det[kMaxTriples];  // float(1 / |det|), the division in FP64 rounded once
+    Triple13 t[kMaxTriples];
 };
 constexpr TripleTable13 MakeTripleTable13() {
     TripleTable13 t{};
@@ -213,50 +227,75 @@ constexpr TripleTable13 MakeTripleTable13() {
                 const double det = n1[0] * cx + n1[1] * cy + n1[2] * cz;
                 const double ad = det < 0 ? -det : det;
                 if (ad < 1e-3) continue;
-                t.a[t.n] = int8_t(a), t.b[t.n] = int8_t(b), t.c[t.n] = int8_t(c);
-                t.inv_det[t.n] = float(1.0 / ad);
+                t.t[t.n] = Triple13{a, b, c, float(1.0 / ad)};
                 t.n++;
             }
     return t;
 }
 constexpr TripleTable13 kTriples13 = MakeTripleTable13();
 
-template <int T>
-__device__ __forceinline__ void ScanTriples13(const float* w, float& best, int& bt) {
-    if constexpr (T < kTriples13.n) {
-        constexpr int a = kTriples13.a[T], b = kTriples13.b[T], c = kTriples13.c[T];
-        constexpr float inv_det = kTriples13.inv_det[T];
-        const float sa = (w[a] * w[b] + w[b] * w[c] + w[a] * w[c]) * inv_det;
-        if (T == 0 || sa < best) {
-            best = sa;
-            bt = T;
+// The table in constant memory: the scan reads it with uniform 16-byte scalar loads.
+__constant__ const TripleTable13 c_triples13 = kTriples13;
+
+// wl: this thread's widths in LDS, width k at wl[k * stride].
+__device__ __forceinline__ void ScanTriplesUniform(const float* wl, int stride, float& best, int& bt) {
+    // triples come in lexicographic order: w_a and w_a w_b change only when a or b does
+    int pa = -1, pb = -1;
+    float wa = 0.0f, wb = 0.0f, wab = 0.0f;
+#pragma unroll 4
+    for (int t = 0; t < kTriples13.n; t++) {
+        const Triple13 T = c_triples13.t[t];
+        if (T.a != pa) {
+            pa = T.a;
+            wa = wl[T.a * stride];
+            pb = -1;
         }
-        ScanTriples13<T + 1>(w, best, bt);
+        if (T.b != pb) {
+            pb = T.b;
+            wb = wl[T.b * stride];
+            wab = wa * wb;
+        }
+        const float wc = wl[T.c * stride];
+        const float sa = (wab + wb * wc + wa * wc) * T.inv_det;
+        if (t == 0 || sa < best) {
+            best = sa;
+            bt = t;
+        }
     }
 }
 
-// Selects arr[idx] for a run-time idx without indexing a private array.
-template <int N>
-__device__ __forceinline__ float SelDir(const float (&arr)[N], int idx) {
-    float r = arr[0];
-#pragma unroll
-    for (int k = 1; k < N; k++) r = idx == k ? arr[k] : r;
-    return r;
-}
+
 
 // Writes the SOBB (lo_a hi_a lo_b hi_b lo_c hi_c) of node `id` to f and returns a | b<<4 | c<<8.
 // ND is the size of the direction set (3, 7, 9 or 13).
+// Slab s of direction k of node id: axis directions from the node box -/+ pad, the others
+// from the stored k-DOP slabs (lo, -hi).
+__device__ __forceinline__ void SlabOf(const BuildBuffers& b, const PNode& nd, int32_t id, int k, float pad, float& lo,
+                                       float& hi) {
+    if (k < 3) {
+        const float l = k == 0 ? nd.lo[0] : (k == 1 ? nd.lo[1] : nd.lo[2]);
+        const float h = k == 0 ? nd.hi[0] : (k == 1 ? nd.hi[1] : nd.hi[2]);
+        lo = l - pad;
+        hi = h + pad;
+    } else {
+        const int nx = b.ndirs - 3;
+        const float* kd = b.kdop + int64_t(id) * (2 * nx);
+        lo = kd[k - 3];
+        hi = -kd[nx + k - 3];
+    }
+}
+
+// Writes the SOBB (lo_a hi_a lo_b hi_b lo_c hi_c) of node `id` to f and returns a | b<<4 | c<<8.
+// ND is the size of the direction set (3, 7, 9 or 13). The widths go to this thread's LDS
+// column wl (stride apart) and the triple scan reads them there: register arrays indexed at
+// run time would spill.
 template <int ND>
 __device__ __forceinline__ uint32_t FitSobbT(const BuildBuffers& b, const int4* triples, int32_t id, float pad,
-                                             float* f) {
+                                             float* f, float* wl, int stride) {
     constexpr int NX = ND - 3;
     const PNode nd = b.nodes[id];
-    float lo[ND], hi[ND], w[ND];
 #pragma unroll
-    for (int a = 0; a < 3; a++) {
-        lo[a] = nd.lo[a] - pad;
-        hi[a] = nd.hi[a] + pad;
-    }
+    for (int a = 0; a < 3; a++) wl[a * stride] = (nd.hi[a] + pad) - (nd.lo[a] - pad);
     if constexpr (NX > 0) {
         const float4* kd = reinterpret_cast<const float4*>(b.kdop) + int64_t(id) * (NX / 2);
         float s[2 * NX];
@@ -266,21 +305,16 @@ __device__ __forceinline__ uint32_t FitSobbT(const BuildBuffers& b, const int4* 
             s[4 * q] = v.x, s[4 * q + 1] = v.y, s[4 * q + 2] = v.z, s[4 * q + 3] = v.w;
         }
 #pragma unroll
-        for (int k = 0; k < NX; k++) {
-            lo[3 + k] = s[k];
-            hi[3 + k] = -s[NX + k];
-        }
+        for (int k = 0; k < NX; k++) wl[(3 + k) * stride] = -s[NX + k] - s[k];
     }
-#pragma unroll
-    for (int k = 0; k < ND; k++) w[k] = hi[k] - lo[k];
     float best = 0.0f;
     int bt = 0;
     if constexpr (ND == 13) {
-        ScanTriples13<0>(w, best, bt);
+        ScanTriplesUniform(wl, stride, best, bt);
     } else {
         for (int t = 0; t < b.ntriples; t++) {
             const int4 T = triples[t];
-            const float wa = SelDir(w, T.x), wb = SelDir(w, T.y), wc = SelDir(w, T.z);
+            const float wa = wl[T.x * stride], wb = wl[T.y * stride], wc = wl[T.z * stride];
             const float sa = (wa * wb + wb * wc + wa * wc) * __int_as_float(T.w);
             if (t == 0 || sa < best) {
                 best = sa;
@@ -289,19 +323,20 @@ __device__ __forceinline__ uint32_t FitSobbT(const BuildBuffers& b, const int4* 
         }
     }
     const int4 T = triples[bt];
-    f[0] = SelDir(lo, T.x), f[1] = SelDir(hi, T.x);
-    f[2] = SelDir(lo, T.y), f[3] = SelDir(hi, T.y);
-    f[4] = SelDir(lo, T.z), f[5] = SelDir(hi, T.z);
+    SlabOf(b, nd, id, T.x, pad, f[0], f[1]);
+    SlabOf(b, nd, id, T.y, pad, f[2], f[3]);
+    SlabOf(b, nd, id, T.z, pad, f[4], f[5]);
     return uint32_t(T.x) | (uint32_t(T.y) << 4) | (uint32_t(T.z) << 8);
 }
 
+// wl / stride: kMaxDirs LDS floats of this thread (the 26-DOP scan reads its widths there).
 __device__ __forceinline__ uint32_t FitSobb(const BuildBuffers& b, const int4* triples, int32_t id, float pad,
-                                            float* f) {
+                                            float* f, float* wl, int stride) {
     switch (b.ndirs) {
-        case 13: return FitSobbT<13>(b, triples, id, pad, f);
-        case 9: return FitSobbT<9>(b, triples, id, pad, f);
-        case 7: return FitSobbT<7>(b, triples, id, pad, f);
-        default: return FitSobbT<3>(b, triples, id, pad, f);
+        case 13: return FitSobbT<13>(b, triples, id, pad, f, wl, stride);
+        case 9: return FitSobbT<9>(b, triples, id, pad, f, wl, stride);
+        case 7: return FitSobbT<7>(b, triples, id, pad, f, wl, stride);
+        default: return FitSobbT<3>(b, triples, id, pad, f, wl, stride);
     }
 }
 
@@ -355,8 +390,10 @@ __device__ __forceinline__ int32_t NearestNeighbour(const WindowLds& w, int i, i
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ bool Inside(const BuildBuffers& b, int32_t id, int max_leaf) {
     int32_t q = b.parent[id];
-    while (q >= 0 && b.count[q] <= max_leaf) {
-        if (b.collapsed[q]) return true;
+    while (q >= 0) {
+        const int32_t cc = b.eval[q].count_col;
+        if ((cc & 0x7FFFFFFF) > max_leaf) break;
+        if (cc < 0) return true;  // collapsed
         q = b.parent[q];
     }
     return false;
@@ -478,34 +515,40 @@ __device__ __forceinline__ void ClassifyNode(const BuildBuffers& b, int32_t id, 
                                              int& leaves) {
     const int64_t n = b.n;
     const bool in = Inside(b, id, max_leaf);
-    const bool col = b.collapsed[id] != 0;
+    const EvalRec e = b.eval[id];
+    const bool col = e.count_col < 0;
+    uint8_t st = 0;
     if (col && !in) {
-        b.leafslot[b.minleaf[id]] = b.count[id];
+        b.leafslot[e.minleaf] = e.count_col & 0x7FFFFFFF;
         leaves++;
+        st = 1;
     }
     if (id >= n) {
         const bool k = !col && !in;
         b.keptflag[id - n] = k ? 1 : 0;
         kept += k;
+        if (k) st = 2;
     }
+    b.state[id] = st;
 }
 
 // keptflag holds exclusive ranks and leafslot begin offsets (both scanned); K kept nodes.
-__device__ __forceinline__ int32_t ChildRef(const BuildBuffers& b, int32_t id, int max_leaf, int32_t K) {
-    const int64_t n = b.n;
-    if (id >= n && !b.collapsed[id] && !Inside(b, id, max_leaf)) return K - 1 - b.keptflag[id - n];
-    return ~((b.leafslot[b.minleaf[id]] << 4) | b.count[id]);
+// A child of a kept node is itself kept (state 2) or a final leaf (state 1).
+__device__ __forceinline__ int32_t ChildRef(const BuildBuffers& b, int32_t id, int32_t K) {
+    if (b.state[id] == 2) return K - 1 - b.keptflag[id - b.n];
+    const EvalRec e = b.eval[id];
+    return ~((b.leafslot[e.minleaf] << 4) | (e.count_col & 0x7FFFFFFF));
 }
 
 // Gathers the primitives of a final leaf (left-first DFS of its PLOC subtree) and writes the
-// node of a kept inner node; s_tri is the SOBB triple table (LDS), pad the k-DOP pad.
-__device__ __forceinline__ void EmitNode(const BuildBuffers& b, int32_t id, int max_leaf, int32_t K,
-                                         const int4* s_tri, float pad) {
+// node of a kept inner node (for SOBB trees its child references only; FitChild adds the
+// child slabs).
+__device__ __forceinline__ void EmitNode(const BuildBuffers& b, int32_t id, int32_t K) {
     const int64_t n = b.n, nn = 2 * n - 1;
-    const bool col = b.collapsed[id] != 0;
-    const bool in = Inside(b, id, max_leaf);
-    if (col && !in) {
-        int32_t pos = b.leafslot[b.minleaf[id]];
+    const uint8_t st = b.state[id];
+    if (st == 1) {
+        const EvalRec e = b.eval[id];
+        int32_t pos = b.leafslot[e.minleaf];
         int32_t st[kMaxLeaf + 1];
         int sp = 0;
         st[sp++] = id;
@@ -534,34 +577,20 @@ __device__ __forceinline__ void EmitNode(const BuildBuffers& b, int32_t id, int 
             f.ly1 = f.ry1 = r.hi[1];
             f.lz0 = f.rz0 = r.lo[2];
             f.lz1 = f.rz1 = r.hi[2];
-            f.left = ~((b.leafslot[b.minleaf[id]] << 4) | b.count[id]);
+            f.left = ~((b.leafslot[e.minleaf] << 4) | (e.count_col & 0x7FFFFFFF));
             f.right = ~0;
-            f.pad0 = f.pad1 = 0;
-            if (b.ndirs > 0) {
-                float g[6];
-                const uint32_t d = FitSobb(b, s_tri, id, pad, g);
-                float* fl = &f.lx0;
-                for (int k = 0; k < 6; k++) fl[k] = fl[6 + k] = g[k];
-                f.pad0 = int32_t(d | (d << 12));
-            }
+            f.pad0 = f.pad1 = 0;  // SOBB trees: FitChild writes the slabs and directions
             b.fnodes[0] = f;
         }
-    } else if (id >= n && !col && !in) {
+    } else if (st == 2) {
         const PNode nd = b.nodes[id];
-        const int32_t left = ChildRef(b, nd.left, max_leaf, K);
-        const int32_t right = ChildRef(b, nd.right, max_leaf, K);
-        FNode* out = b.fnodes + (K - 1 - b.keptflag[id - n]);
-        if (b.ndirs > 0) {
-            // one child at a time: the 26-DOP triple scan keeps its pair products in registers
-            uint32_t dirs = 0;
-#pragma nounroll
-            for (int c = 0; c < 2; c++) {
-                float g[6];
-                dirs |= FitSobb(b, s_tri, c ? nd.right : nd.left, pad, g) << (12 * c);
-                float2* o = reinterpret_cast<float2*>(reinterpret_cast<float*>(out) + 6 * c);
-                o[0] = make_float2(g[0], g[1]), o[1] = make_float2(g[2], g[3]), o[2] = make_float2(g[4], g[5]);
-            }
-            reinterpret_cast<int4*>(out)[3] = make_int4(left, right, int32_t(dirs), 0);
+        const int32_t left = ChildRef(b, nd.left, K);
+        const int32_t right = ChildRef(b, nd.right, K);
+        const int32_t slot = K - 1 - b.keptflag[id - n];
+        FNode* out = b.fnodes + slot;
+        if (b.ndirs > 0) {  // child slabs and directions: FitChild, one thread per child
+            b.kept_id[slot] = id;
+            reinterpret_cast<int4*>(out)[3] = make_int4(left, right, 0, 0);
         } else {
             const PNode L = b.nodes[nd.left], R = b.nodes[nd.right];
             FNode f;
@@ -573,6 +602,29 @@ __device__ __forceinline__ void EmitNode(const BuildBuffers& b, int32_t id, int 
             *out = f;
         }
     }
+}
+
+// SOBB of child c = i & 1 of final inner node i >> 1 (or, when the whole scene is one leaf,
+// i == 0 fits the root into both halves): slabs to the node's float slots, directions OR-ed
+// into its pad word (zeroed by EmitNode).
+__device__ __forceinline__ void FitChild(const BuildBuffers& b, int64_t i, int32_t K, const int4* s_tri, float pad,
+                                         float* wl, int stride) {
+    float g[6];
+    if (K == 0) {
+        if (i != 0) return;
+        const uint32_t d = FitSobb(b, s_tri, int32_t(2 * b.n - 2), pad, g, wl, stride);
+        float* fl = &b.fnodes[0].lx0;
+        for (int k = 0; k < 6; k++) fl[k] = fl[6 + k] = g[k];
+        atomicOr(&b.fnodes[0].pad0, int32_t(d | (d << 12)));
+        return;
+    }
+    const int64_t slot = i >> 1;
+    const int c = int(i & 1);
+    const PNode nd = b.nodes[b.kept_id[slot]];
+    const uint32_t d = FitSobb(b, s_tri, c ? nd.right : nd.left, pad, g, wl, stride);
+    float2* o = reinterpret_cast<float2*>(reinterpret_cast<float*>(b.fnodes + slot) + 6 * c);
+    o[0] = make_float2(g[0], g[1]), o[1] = make_float2(g[2], g[3]), o[2] = make_float2(g[4], g[5]);
+    atomicOr(&b.fnodes[slot].pad0, int32_t(d << (12 * c)));
 }
 
 // The SAH terms (Ct = Ci = 1) one final node contributes: its children's areas, a leaf

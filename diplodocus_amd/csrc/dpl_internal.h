@@ -190,6 +190,15 @@ struct BuildResults {
     float root_lo[3], root_hi[3];
 };
 
+// Fused collapse record of a PLOC node, one 16-byte load or store: triangle count with the
+// collapsed flag in the sign bit, SAH cost, smallest sorted position below it, height.
+struct alignas(16) EvalRec {
+    int32_t count_col;
+    float cost;
+    int32_t minleaf;
+    int32_t height;
+};
+
 struct BuildBuffers {
     int64_t n;
     const float* tri;      // n x 9 (device)
@@ -205,11 +214,8 @@ struct BuildBuffers {
     uint64_t* status[2];   // look-back words per tile
     uint32_t* tile_counter;  // 2
     int32_t* iter_done;    // 1: total iterations
-    int32_t* count;        // 2n-1
-    float* cost;           // 2n-1
-    uint8_t* collapsed;    // 2n-1
-    int32_t* minleaf;      // 2n-1
-    int32_t* height;       // 2n-1
+    EvalRec* eval;         // 2n-1: fused collapse record
+    uint8_t* state;        // 2n-1: final-layout class (0 inside a leaf, 1 final leaf, 2 kept inner)
     BuildResults* results;   // host-mapped
     int32_t* leafslot;     // n: count of the final leaf whose minimum sorted position is k
     int32_t* keptflag;     // n: internal node kept as a final inner node
@@ -217,6 +223,7 @@ struct BuildBuffers {
     int32_t* counters;     // 8: [0] kept inner, [1] final leaves, [2] max height
     double* sah;           // 1
     FNode* fnodes;         // n (capacity)
+    int32_t* kept_id;      // n: PLOC id of each final inner node (SOBB trees; else null)
     float4* tri_ordered;   // 3n
     int32_t* prim_order;   // n
     // SOBB builds (ndirs > 0): k-DOP of every node, 2*ndirs floats (lo..., hi...)
@@ -292,11 +299,8 @@ struct BatchArgs {
     // per PLOC node: mesh m's 2 n_m - 1 nodes start at 2 tri_off[m] - m
     PNode* nodes;
     int32_t* parent;
-    int32_t* count;
-    float* cost;
-    int32_t* minleaf;
-    int32_t* height;
-    uint8_t* collapsed;
+    EvalRec* eval;
+    uint8_t* state;
     FNode* fnodes;
     MeshResult* results;       // per mesh (zeroed before the launch)
     int morton_type, radius, max_leaf, digit_bits;
