@@ -49,6 +49,12 @@ class RenderParams(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+class WhittedParams(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("max_depth", C.c_int32),
+                ("area_light_sample_cnt", C.c_int32), ("backface_culling", C.c_int32), ("reserved", C.c_int32 * 3),
+                ("background", C.c_float * 3), ("camera", C.c_float * 11)]
+
+
 class MeshInfo(C.Structure):
     _fields_ = [("node_offset", C.c_int64), ("prim_offset", C.c_int64), ("node_count", C.c_int32),
                 ("prim_count", C.c_int32), ("tree_height", C.c_int32), ("ploc_iterations", C.c_int32),
@@ -104,6 +110,7 @@ def _load() -> C.CDLL:
         "dpl_render_forward": (I32, [P, P, C.POINTER(RenderParams), P, P, P, P]),
         "dpl_render_adjoint": (I32, [P, P, C.POINTER(RenderParams), P, P, P, P, P, P]),
         "dpl_bvh_get_meshes": (I32, [P, P]),
+        "dpl_render_whitted": (I32, [P, P, C.POINTER(WhittedParams), P, P, P, P, I32, P, I32, P]),
         "dpl_trace_graph_create": (I32, [P, P, P, P, I64, U32, P, C.POINTER(P)]),
         "dpl_trace_graph_launch": (I32, [P]),
         "dpl_trace_graph_destroy": (None, [P]),
@@ -127,5 +134,5 @@ EXPORTED = [
     "dpl_trace_batched",
     "dpl_bvh_get_layout", "dpl_bvh_download", "dpl_compute_codes", "dpl_radix_sort_pairs", "dpl_render_forward",
     "dpl_render_adjoint", "dpl_bvh_get_meshes", "dpl_trace_graph_create", "dpl_trace_graph_launch",
-    "dpl_trace_graph_destroy", "dpl_autotune",
+    "dpl_trace_graph_destroy", "dpl_autotune", "dpl_render_whitted",
 ]

@@ -1072,6 +1072,31 @@ static dpl_status Render(bool adjoint, dpl_context* ctx, const dpl_bvh* bvh, con
     });
 }
 
+dpl_status dpl_render_whitted(dpl_context* ctx, const dpl_bvh* bvh, const dpl_whitted_params* wp, const float* tri,
+                              const float* vn, const int32_t* mat_id, const dpl_material* mats, int32_t material_count,
+                              const int32_t* lights, int32_t light_count, float* image) {
+    if (!ctx || !bvh || !wp) return ctx ? Fail(ctx, DPL_ERR_INVALID_ARG, "null argument") : DPL_ERR_INVALID_ARG;
+    return Guard(ctx, [&] {
+        Require(bvh->ctx == ctx, "bvh belongs to another context");
+        Require(!bvh->batched, "Whitted rendering needs a single-scene BVH");
+        Require(wp->width >= 2 && wp->height >= 2, "width and height must be at least 2");
+        Require(wp->max_depth >= 0 && wp->max_depth <= kDplWhittedMaxDepth, "max_depth must be in [0, 8]");
+        Require(wp->area_light_sample_cnt >= 0, "area_light_sample_cnt must be non-negative");
+        Require(tri && mat_id && mats && image, "triangle, material and image arrays must be non-null");
+        Require(material_count >= 1, "at least one material is required");
+        Require(light_count >= 0 && (light_count == 0 || lights), "light_tris must be non-null when light_count > 0");
+        float basis[12];
+        CameraBasisHost(wp->camera, wp->width, wp->height, basis);
+        TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
+                       int32_t(bvh->prim_count), bvh->tree_height, bvh->ndirs, bvh->sobb_dirs.as<float>()};
+        int32_t* spill = ctx->spill.ensure<int32_t>(std::max<size_t>(16, TraceSpillBytes(ctx->cu_count, bvh->tree_height)));
+        hipError_t e = LaunchWhitted(tree, *wp, basis, tri, vn, mat_id, mats, lights, light_count, image, ctx->cu_count,
+                                     ctx->trace_counter.as<uint32_t>(), spill, ctx->stream);
+        if (e == hipErrorNotSupported) throw Error(DPL_ERR_UNSUPPORTED, "tree height exceeds the traversal stack");
+        DPL_HIP(e);
+    });
+}
+
 dpl_status dpl_render_forward(dpl_context* ctx, const dpl_bvh* bvh, const dpl_render_params* rp, const int32_t* mat_id,
                               const float* albedo, const float* emission, float* image) {
     return Render(false, ctx, bvh, rp, mat_id, albedo, emission, nullptr, image, nullptr, nullptr);

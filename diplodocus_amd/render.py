@@ -11,7 +11,7 @@ from __future__ import annotations
 import ctypes as C
 
 from . import _lib
-from ._lib import RenderParams, lib
+from ._lib import RenderParams, WhittedParams, lib
 
 
 def make_render_params(width: int, height: int, spp: int, max_depth: int, seed: int, camera,
@@ -75,3 +75,43 @@ def render_function():
             return ga, ge, None, None, None
 
     return RenderFunction
+
+
+# ---- Whitted ray tracing (dpl_render_whitted) -----------------------------------------------
+def make_whitted_params(width: int, height: int, max_depth: int, samples: int, camera, background=(0.0, 0.0, 0.0),
+                        backface_culling: bool = False) -> WhittedParams:
+    wp = WhittedParams()
+    wp.width, wp.height, wp.max_depth, wp.area_light_sample_cnt = width, height, max_depth, samples
+    wp.backface_culling = 1 if backface_culling else 0
+    for k in range(3):
+        wp.background[k] = float(background[k])
+    cam = camera.as_array() if hasattr(camera, "as_array") else camera
+    for k in range(11):
+        wp.camera[k] = float(cam[k])
+    return wp
+
+
+def emissive_triangles(mat_id, materials) -> "np.ndarray":
+    """The reference's area lights: triangles whose material emits, in triangle order
+    (src/io/scene/obj_scene_loader.cc:95-105)."""
+    import numpy as np
+
+    mats = np.asarray(materials, np.float32).reshape(-1, 16)
+    emits = (mats[:, 9:12] != 0).any(axis=1)
+    return np.nonzero(emits[np.asarray(mat_id)])[0].astype(np.int32)
+
+
+def render_whitted(bvh, wp: WhittedParams, tris, mat_id, materials, lights, vertex_normals=None, image=None):
+    """GPU Whitted image (H, W, 3). tris / mat_id / materials (M, 16) / lights / vertex_normals
+    are GPU tensors (materials rows: diffuse3 specular3 transmittance3 emission3 shininess ior
+    r_ior pad)."""
+    import torch
+
+    if image is None:
+        image = torch.empty((wp.height, wp.width, 3), dtype=torch.float32, device=tris.device)
+    m = int(materials.shape[0])
+    nl = int(lights.shape[0])
+    bvh.ctx.check(lib.dpl_render_whitted(bvh.ctx.h, bvh.h, C.byref(wp), _dev(tris),
+                                         _dev(vertex_normals) if vertex_normals is not None else None, _dev(mat_id),
+                                         _dev(materials), m, _dev(lights) if nl else None, nl, _dev(image)))
+    return image

@@ -119,6 +119,44 @@ def cornell_room() -> SceneData:
     return SceneData(tris.astype(np.float32), grp, albedo, emission, Camera())
 
 
+def whitted_material(diffuse, specular=(0.0, 0.0, 0.0), transmittance=(0.0, 0.0, 0.0), emission=(0.0, 0.0, 0.0),
+                     shininess=1.0, ior=1.0) -> np.ndarray:
+    """One Whitted material row (16 float32): diffuse3 specular3 transmittance3 emission3
+    shininess ior r_ior pad, r_ior = 1 / ior (src/io/scene/obj_scene_loader.cc:118)."""
+    m = np.zeros(16, np.float32)
+    m[0:3], m[3:6], m[6:9], m[9:12] = diffuse, specular, transmittance, emission
+    m[12], m[13] = shininess, ior
+    m[14] = np.float32(1.0) / np.float32(ior)
+    return m
+
+
+def cornell_whitted():
+    """The 36-triangle room with Whitted materials: white floor / ceiling / back wall, red
+    left and green right walls, a glossy short box (Phong + mirror), a glass tall box
+    (refraction, ior 1.5) and the emissive light quad. Returns (scene, materials (8, 16))."""
+    s = cornell_room()
+    mats = np.stack([
+        whitted_material((0.73, 0.73, 0.73)), whitted_material((0.73, 0.73, 0.73)),
+        whitted_material((0.73, 0.73, 0.73)), whitted_material((0.63, 0.065, 0.05)),
+        whitted_material((0.14, 0.45, 0.091)),
+        whitted_material((0.3, 0.3, 0.35), specular=(0.6, 0.6, 0.6), shininess=64.0),
+        whitted_material((0.05, 0.05, 0.05), specular=(0.1, 0.1, 0.1), transmittance=(0.85, 0.85, 0.85),
+                         shininess=128.0, ior=1.5),
+        whitted_material((0.0, 0.0, 0.0), emission=(17.0, 12.0, 4.0)),
+    ])
+    return s, mats
+
+
+def sphere_vertex_normals(tris: np.ndarray, first: int, center) -> np.ndarray:
+    """Per-vertex normals (n, 9): unit (vertex - center) for triangles first.., zero (none)
+    before."""
+    vn = np.zeros_like(tris, dtype=np.float32)
+    v = tris[first:].reshape(-1, 3, 3).astype(np.float64) - np.asarray(center, np.float64)
+    v /= np.linalg.norm(v, axis=2, keepdims=True)
+    vn[first:] = v.reshape(-1, 9).astype(np.float32)
+    return vn
+
+
 # --------------------------------------------------------------------------------------
 # Value noise and the displaced geodesic icosphere
 # --------------------------------------------------------------------------------------

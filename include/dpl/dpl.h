@@ -199,6 +199,42 @@ dpl_status dpl_trace_graph_create(dpl_context* ctx, const dpl_bvh* bvh, const dp
 dpl_status dpl_trace_graph_launch(dpl_trace_graph* graph); /* on the context's stream */
 void dpl_trace_graph_destroy(dpl_trace_graph* graph);
 
+/* Whitted ray tracing: the reference's CpuRaytracer (src/renderer/cpu_raytracer.cc:50-288)
+ * over the BVH — area-light next-event estimation with hash-sampled light points (seed 42),
+ * Phong, mirror reflection and refraction to max_depth, emissive hits return their emission.
+ * Replaces the brute-force RaytracingBounceKernel path of GpuRenderer
+ * (src/gpu/renderer/gpu_renderer_api.h:10). Material as src/scene/material.h:7-18
+ * (r_ior = 1 / ior as the OBJ loader sets it, obj_scene_loader.cc:118). */
+typedef struct dpl_material {
+    float diffuse[3];
+    float specular[3];
+    float transmittance[3];
+    float emission[3];
+    float shininess;
+    float ior;
+    float r_ior;
+    float pad;
+} dpl_material;
+
+typedef struct dpl_whitted_params {
+    int32_t width, height;
+    int32_t max_depth;              /* 0..8 */
+    int32_t area_light_sample_cnt;  /* per light per hit */
+    int32_t backface_culling;
+    int32_t reserved[3];
+    float background[3];
+    float camera[11];               /* pos3 dir3 up3 fov far (src/scene/camera.h:7-13) */
+} dpl_whitted_params;
+
+/* All arrays are device memory. tri_xyz: the n triangles the BVH was built from, in their
+ * original order; vertex_normals: n x 9 floats or NULL (a triangle whose three normals are
+ * zero uses its geometric normal); mat_id: n; materials: material_count; light_tris: the
+ * emissive triangles in triangle order (the reference's area lights); image: H x W x 3. */
+dpl_status dpl_render_whitted(dpl_context* ctx, const dpl_bvh* bvh, const dpl_whitted_params* params,
+                              const float* tri_xyz, const float* vertex_normals, const int32_t* mat_id,
+                              const dpl_material* materials, int32_t material_count, const int32_t* light_tris,
+                              int32_t light_count, float* image);
+
 /* Autotuner: times builds of the given workload over nn_search_radius {8, 16, 32}, block_size
  * {128, 256, 512} ({256, 512} batched) and sort_digit_bits {4, 8}, and returns in *best the
  * fastest whose SAH cost is at most (1 + sah_tolerance) x the lowest SAH among them (the

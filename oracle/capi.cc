@@ -272,6 +272,22 @@ void orc_trace_brute(const float* tri, int64_t ntri, const RayRec* rays, int64_t
     TraceBrute(tri, ntri, rays, n, flags, hits, threads);
 }
 
+// Whitted ray tracing over the PLOC tree of `ploc` (closest hits are tree-independent).
+// mats: M x 16 floats (diffuse3 specular3 transmittance3 emission3 shininess ior r_ior pad).
+void orc_whitted(void* ploc, const float* tri_xyz, const float* vertex_normals, int64_t n, const int32_t* mat_id,
+                 const float* mats, int W, int H, int max_depth, int samples, int cull, const float* background,
+                 const float* cam, int recursive, float* image, int threads) {
+    const FinalTree& T = static_cast<PlocHandle*>(ploc)->r.tree;
+    WParams p{W, H, max_depth, samples, cull, {background[0], background[1], background[2]}, {}};
+    for (int k = 0; k < 11; k++) p.cam[k] = cam[k];
+    WhittedRender(T, tri_xyz, vertex_normals, n, mat_id, reinterpret_cast<const WMaterial*>(mats), p, recursive != 0,
+                  image, threads);
+}
+float orc_pow_pos(float x, float y) { return PowPos(x, y); }
+float orc_area_light_sample01(uint32_t seed, uint32_t px, uint32_t py, uint32_t light_id, uint32_t s, uint32_t dim) {
+    return AreaLightSample01(seed, px, py, light_id, s, dim);
+}
+
 // PCG32 / sincos KATs
 void orc_pcg_stream(uint64_t seed, uint64_t stream, int32_t count, uint32_t* out) {
     Pcg32 r;

@@ -297,6 +297,7 @@ bool IntersectWatertight(const WtRay& w, const float* v0, const float* v1, const
 bool IntersectBoxRobust(const WtRay& w, float tmin, float tmax, const float lo[3], const float hi[3], float& tnear);
 
 void TraceFinal(const FinalTree& t, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads);
+void TraceFinalOne(const FinalTree& t, const RayRec& r, uint32_t flags, HitRec& h);
 void TraceBrute(const float* tri_xyz, int64_t ntri, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits,
                 int threads);
 double IntersectFp64(const float* tri_xyz9, const RayRec& r, double& b1, double& b2);
@@ -316,6 +317,23 @@ void SinCos2Pi(float u, float& s, float& c);  // polynomial sin/cos of 2*pi*u, u
 
 namespace orc {
 void CameraBasis(const float* cam, int w, int h, float* out12);
+
+// Whitted ray tracing (cpu_raytracer.cc). Materials as src/scene/material.h:7-18.
+constexpr int kWhittedMaxDepth = 8;
+struct WMaterial {
+    float diffuse[3], specular[3], transmittance[3], emission[3];
+    float shininess, ior, r_ior, pad;
+};
+struct WParams {
+    int W, H, max_depth, samples, cull;
+    float background[3];
+    float cam[11];
+};
+float PowPos(float x, float y);
+float AreaLightSample01(uint32_t seed, uint32_t px, uint32_t py, uint32_t light_id, uint32_t sample_idx,
+                        uint32_t dimension);
+void WhittedRender(const FinalTree& tree, const float* tri, const float* vn, int64_t n, const int32_t* mat_id,
+                   const WMaterial* mats, const WParams& p, bool recursive, float* image, int threads);
 int TracePathVertices(const FinalTree& T, const float* tri_xyz, const int32_t* mat_id, const float* albedo,
                       const float* basis, float far, int x, int y, int W, int spp, int s, int depth, uint64_t seed,
                       int32_t* mats, Vec3* betas);

@@ -342,3 +342,44 @@ def path_grad_fp64(pm, pixels, spp, depth, albedo64, emission64, target):
     ge = np.zeros_like(e)
     lib.orc_path_grad_fp64(_p(pm), pixels, spp, depth, a.shape[0], _p(a), _p(e), _p(t), _p(ga), _p(ge))
     return ga, ge
+
+
+# ---- Whitted ray tracing --------------------------------------------------------------------
+def _sig_whitted():
+    P, I32, I64, U32, F = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_float
+    for name, res, args in [
+        ("orc_whitted", None, [P, P, P, I64, P, P, I32, I32, I32, I32, I32, P, P, I32, P, I32]),
+        ("orc_pow_pos", F, [F, F]),
+        ("orc_area_light_sample01", F, [U32, U32, U32, U32, U32, U32]),
+    ]:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+_sig_whitted()
+
+
+def pow_pos(x: float, y: float) -> float:
+    return lib.orc_pow_pos(x, y)
+
+
+def area_light_sample01(seed, px, py, light, s, dim) -> float:
+    return lib.orc_area_light_sample01(seed, px, py, light, s, dim)
+
+
+def whitted(ploc: "Ploc", mat_id, materials, W, H, max_depth, samples, camera, background=(0.0, 0.0, 0.0),
+            cull=False, vertex_normals=None, recursive=False, threads=None):
+    """CPU restatement of the reference's Whitted CpuRaytracer over the oracle's PLOC tree
+    (watertight hits). recursive=True evaluates the reference's recursion as written;
+    otherwise the weighted depth-first sum the GPU kernel uses. Returns (H, W, 3) float32."""
+    mats = _f32(materials).reshape(-1, 16)
+    mat = np.ascontiguousarray(mat_id, np.int32)
+    cam = _f32(camera.as_array() if hasattr(camera, "as_array") else camera)
+    bg = _f32(background)
+    vn = _f32(vertex_normals) if vertex_normals is not None else None
+    img = np.zeros((H, W, 3), np.float32)
+    lib.orc_whitted(ploc.h, _p(ploc.tris), _p(vn) if vn is not None else None, ploc.tris.shape[0], _p(mat), _p(mats),
+                    W, H, max_depth, samples, 1 if cull else 0, _p(bg), _p(cam), 1 if recursive else 0, _p(img),
+                    threads or nthreads())
+    return img

@@ -115,7 +115,7 @@ static inline bool Better(float t, int32_t prim, float best_t, int32_t best_prim
     return t < best_t || (t == best_t && best_prim >= 0 && prim < best_prim);
 }
 
-static void TraceFinalOne(const FinalTree& T, const RayRec& r, uint32_t flags, HitRec& h) {
+void TraceFinalOne(const FinalTree& T, const RayRec& r, uint32_t flags, HitRec& h) {
     const bool any = flags & 1u, cull = flags & 2u;
     WtRay w = PrepareRay(r);
     SobbRay sr{};
