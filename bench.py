@@ -203,6 +203,22 @@ def main() -> int:
         f"adjoint {adj_ms:.3f} ms ({adj_ms / fwd_ms:.2f}x)")
     rb.close()
 
+    # ---- Whitted ray tracing of the 36-triangle Cornell room (256x256, depth 3, 8 light
+    #      samples per hit), the reference's CPU renderer's own case ----
+    cs, cmats = scenes.cornell_whitted()
+    cb = accel.GpuBvh(ctx).build(torch.from_numpy(cs.tris).to(dev), accel.make_params(0, 16, 4))
+    c_tris, c_mat = torch.from_numpy(cs.tris).to(dev), torch.from_numpy(cs.mat_id).to(dev)
+    c_mats = torch.from_numpy(cmats).to(dev)
+    c_lights = torch.from_numpy(render.emissive_triangles(cs.mat_id, cmats)).to(dev)
+    whitted = {}
+    for side in (256, 1024):
+        wp = render.make_whitted_params(side, side, 3, 8, cs.camera)
+        wimg = torch.empty((side, side, 3), dtype=torch.float32, device=dev)
+        ms = timed(lambda: render.render_whitted(cb, wp, c_tris, c_mat, c_mats, c_lights, image=wimg), 10)
+        whitted[f"{side}x{side}_ms"] = round(ms, 4)
+    cb.close()
+    log(f"whitted cornell: 256x256 {whitted['256x256_ms']:.3f} ms, 1024x1024 {whitted['1024x1024_ms']:.3f} ms")
+
     # ---- batched builds: 4096 meshes (log-uniform 64..65536 tris) in one persistent
     #      PLOC+EMC launch under autotuned radius / block / digit width, then a
     #      hipGraph-captured trace of 256 rays per mesh ----
@@ -287,6 +303,12 @@ def main() -> int:
         threads = pyoracle.nthreads()
         sb.trace(sample, flags=0, mode=0, threads=threads)
         cpu_trace_s = sb.last_trace_s
+        t0 = time.perf_counter()
+        pyoracle.whitted(pyoracle.Ploc(cs.tris, 0, 16, 4), cs.mat_id, cmats, 256, 256, 3, 8, cs.camera,
+                         threads=threads)
+        whitted_cpu_ms = (time.perf_counter() - t0) * 1e3
+        whitted["cpu_256x256_ms"] = round(whitted_cpu_ms, 2)
+        whitted["cpu_threads"] = threads
         cpu = {
             "value": round(n / (sb.build_ms * 1e-3) / 1e6, 4),
             "unit": "Mtris/s",
@@ -325,6 +347,8 @@ def main() -> int:
         "primary_workload": "icosphere_room 69,668 tris, 1024x1024 camera rays, PLOC Morton30 r=16",
         "primary_bounce_ms": round(bounce_ms, 4),
         "differentiable": diff,
+        "whitted_cornell": dict(whitted, workload="cornell room 36 tris, Whitted depth 3, 8 area-light samples "
+                                                  "per hit (the reference CPU renderer's case)"),
         "batched": batch,
         "build_ms": round(float(build_ms.mean()), 4),
         "trace_ms": round(float(trace_ms.mean()), 4),
