@@ -81,6 +81,28 @@ struct RayHit {
     float epsilon = kEpsilon;
 };
 
+// src/scene/material.h:7-18
+struct Material {
+    std::string name;
+    Vec3 ambient, diffuse, specular, transmittance, emission;
+    float shininess = 0.0f;
+    float ior = 1.0f;
+    float r_ior = 1.0f;  // 1 / ior, as the reference's OBJ loader sets it
+    float dissolve = 1.0f;
+};
+
+// src/config/render_config.h:17-27 (the fields the Whitted renderer reads)
+struct RenderConfig {
+    int width = 800;
+    int height = 600;
+    Vec3 background_color;
+    bool backface_culling = false;
+    int max_depth = 4;
+    int area_light_sample_cnt = 8;
+    int pixel_sample_cnt = 4;
+    int seed = 42;
+};
+
 struct Camera {
     Vec3 pos{0.0f, 0.0f, 0.0f};
     Vec3 dir{0.0f, 0.0f, 1.0f};
@@ -197,6 +219,20 @@ class GpuBvh final : public AccelerationStructure {
 // kPlocEmcSobb); throws std::invalid_argument for the others.
 std::unique_ptr<AccelerationStructure> CreateGpuAccelerationStructure(const AccelerationStructureConfig& config,
                                                                       std::shared_ptr<Context> ctx = nullptr);
+
+// The reference's Whitted CpuRaytracer (src/renderer/cpu_raytracer.cc:50-288) on the GPU over
+// a GpuBvh built from the same triangles (dpl_render_whitted): area lights are the emissive
+// triangles in triangle order, as the reference's loader makes them. Returns the framebuffer
+// (row-major, height x width x 3, src/framebuffer/framebuffer.cc:47-52).
+class GpuWhittedRenderer {
+   public:
+    explicit GpuWhittedRenderer(const GpuBvh& bvh) : bvh_(bvh) {}
+    std::vector<float> Render(const RenderConfig& cfg, const Camera& cam, std::span<const Triangle> triangles,
+                              std::span<const Material> materials) const;
+
+   private:
+    const GpuBvh& bvh_;
+};
 
 // Diffuse path tracing over a GpuBvh with per-material albedo and emission
 // (dpl_render_forward), and the adjoint path replay that gives the image-loss gradient with

@@ -181,6 +181,81 @@ struct DeviceFloats {
 
 }  // namespace
 
+namespace {
+
+// Device copies of host arrays for one call.
+template <typename T>
+struct DeviceArray {
+    T* p = nullptr;
+    explicit DeviceArray(const std::vector<T>& h) {
+        HipCheck(hipMalloc(&p, std::max<size_t>(h.size(), 1) * sizeof(T)), "hipMalloc");
+        if (!h.empty()) HipCheck(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+    }
+    ~DeviceArray() { (void)hipFree(p); }
+    DeviceArray(const DeviceArray&) = delete;
+    DeviceArray& operator=(const DeviceArray&) = delete;
+};
+
+}  // namespace
+
+std::vector<float> GpuWhittedRenderer::Render(const RenderConfig& cfg, const Camera& cam,
+                                              std::span<const Triangle> triangles,
+                                              std::span<const Material> materials) const {
+    const size_t n = triangles.size();
+    std::vector<float> xyz(n * 9), vn(n * 9, 0.0f);
+    std::vector<int32_t> mat(n), lights;
+    bool any_vn = false;
+    for (size_t i = 0; i < n; i++) {
+        const Triangle& t = triangles[i];
+        const Vertex* v[3] = {&t.v0, &t.v1, &t.v2};
+        for (int k = 0; k < 3; k++) {
+            xyz[9 * i + 3 * k] = v[k]->pos.x, xyz[9 * i + 3 * k + 1] = v[k]->pos.y, xyz[9 * i + 3 * k + 2] = v[k]->pos.z;
+            if (t.has_vertex_normals) {
+                vn[9 * i + 3 * k] = v[k]->normal.x, vn[9 * i + 3 * k + 1] = v[k]->normal.y;
+                vn[9 * i + 3 * k + 2] = v[k]->normal.z;
+            }
+        }
+        any_vn |= t.has_vertex_normals;
+        if (t.material_id < 0 || size_t(t.material_id) >= materials.size())
+            throw std::invalid_argument("material_id out of range");
+        mat[i] = t.material_id;
+        if (!(materials[size_t(t.material_id)].emission == Vec3{})) lights.push_back(int32_t(i));
+    }
+    std::vector<dpl_material> dm(materials.size());
+    for (size_t m = 0; m < materials.size(); m++) {
+        const Material& a = materials[m];
+        dm[m] = dpl_material{{a.diffuse.x, a.diffuse.y, a.diffuse.z},
+                             {a.specular.x, a.specular.y, a.specular.z},
+                             {a.transmittance.x, a.transmittance.y, a.transmittance.z},
+                             {a.emission.x, a.emission.y, a.emission.z},
+                             a.shininess,
+                             a.ior,
+                             a.r_ior,
+                             0.0f};
+    }
+    dpl_whitted_params wp{};
+    wp.width = cfg.width, wp.height = cfg.height, wp.max_depth = cfg.max_depth;
+    wp.area_light_sample_cnt = cfg.area_light_sample_cnt;
+    wp.backface_culling = cfg.backface_culling ? 1 : 0;
+    wp.background[0] = cfg.background_color.x, wp.background[1] = cfg.background_color.y;
+    wp.background[2] = cfg.background_color.z;
+    const float c[11] = {cam.pos.x, cam.pos.y, cam.pos.z, cam.dir.x, cam.dir.y, cam.dir.z,
+                         cam.up.x,  cam.up.y,  cam.up.z,  cam.fov,   cam.far};
+    std::memcpy(wp.camera, c, sizeof(c));
+    DeviceArray<float> d_xyz(xyz), d_vn(any_vn ? vn : std::vector<float>{});
+    DeviceArray<int32_t> d_mat(mat), d_lights(lights);
+    DeviceArray<dpl_material> d_mats(dm);
+    DeviceFloats img(size_t(cfg.width) * cfg.height * 3);
+    const auto& ctx = bvh_.context();
+    ctx->Check(dpl_render_whitted(ctx->get(), bvh_.handle(), &wp, d_xyz.p, any_vn ? d_vn.p : nullptr, d_mat.p,
+                                  d_mats.p, int32_t(dm.size()), lights.empty() ? nullptr : d_lights.p,
+                                  int32_t(lights.size()), img.p));
+    ctx->Check(dpl_context_synchronize(ctx->get()));
+    std::vector<float> out(size_t(cfg.width) * cfg.height * 3);
+    img.Download(out);
+    return out;
+}
+
 GpuPathTracer::GpuPathTracer(const GpuBvh& bvh, std::span<const Triangle> triangles, int material_count)
     : bvh_(bvh), material_count_(material_count) {
     std::vector<int32_t> ids(triangles.size());

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -15,6 +16,12 @@
 
 extern "C" void orc_trace_brute(const float* tri, int64_t ntri, const void* rays, int64_t n, uint32_t flags, void* hits,
                                 int threads);
+extern "C" void* orc_ploc_build(const float* tri, int64_t n, int morton_type, int radius, int max_leaf, int bv_type,
+                                int kdop_size, double* build_ms);
+extern "C" void orc_ploc_free(void* h);
+extern "C" void orc_whitted(void* ploc, const float* tri_xyz, const float* vertex_normals, int64_t n,
+                            const int32_t* mat_id, const float* mats, int W, int H, int max_depth, int samples, int cull,
+                            const float* background, const float* cam, int recursive, float* image, int threads);
 
 using namespace diplodocus_amd;
 
@@ -211,6 +218,50 @@ void GpuChecks() {
     const double lin = double(ge[3]) * 5.0 + double(ge[4]) * 5.0 + double(ge[5]) * 5.0;
     Expect(std::fabs(lin - sum) <= 1e-3 * std::fabs(sum), "path tracer adjoint: linearity in emission");
     std::printf("path tracer: image sum %.4f, dL/de %.4f %.4f %.4f\n", sum, ge[3], ge[4], ge[5]);
+
+    // Whitted renderer (the reference's CpuRaytracer) against the oracle's restatement
+    std::vector<Material> wm(7);
+    for (int m = 0; m < 7; m++) wm[m].diffuse = {0.7f, 0.6f - 0.05f * m, 0.5f};
+    wm[1].diffuse = {};
+    wm[1].emission = {6.0f, 6.0f, 5.0f};  // the ceiling
+    wm[2].specular = {0.5f, 0.5f, 0.5f}, wm[2].shininess = 32.0f;
+    wm[6].specular = {0.2f, 0.2f, 0.2f}, wm[6].transmittance = {0.6f, 0.6f, 0.6f};
+    wm[6].ior = 1.5f, wm[6].r_ior = 1.0f / 1.5f, wm[6].shininess = 16.0f;
+    RenderConfig rc;
+    rc.width = 96, rc.height = 72, rc.max_depth = 3, rc.area_light_sample_cnt = 2;
+    rc.background_color = {0.05f, 0.1f, 0.2f};
+    const std::vector<float> wimg = GpuWhittedRenderer(bvh).Render(rc, cam, tris, wm);
+    std::vector<float> mrows(16 * 7, 0.0f), vn(tris.size() * 9, 0.0f);
+    std::vector<int32_t> mid(tris.size());
+    for (int m = 0; m < 7; m++) {
+        const Vec3 f[4] = {wm[m].diffuse, wm[m].specular, wm[m].transmittance, wm[m].emission};
+        for (int k = 0; k < 4; k++) mrows[16 * m + 3 * k] = f[k].x, mrows[16 * m + 3 * k + 1] = f[k].y, mrows[16 * m + 3 * k + 2] = f[k].z;
+        mrows[16 * m + 12] = wm[m].shininess, mrows[16 * m + 13] = wm[m].ior, mrows[16 * m + 14] = wm[m].r_ior;
+    }
+    for (size_t i = 0; i < tris.size(); i++) {
+        mid[i] = tris[i].material_id;
+        if (tris[i].has_vertex_normals)
+            for (int k = 0; k < 3; k++) {
+                const Vec3& nv = k == 0 ? tris[i].v0.normal : (k == 1 ? tris[i].v1.normal : tris[i].v2.normal);
+                vn[9 * i + 3 * k] = nv.x, vn[9 * i + 3 * k + 1] = nv.y, vn[9 * i + 3 * k + 2] = nv.z;
+            }
+    }
+    void* ploc = orc_ploc_build(xyz.data(), int64_t(tris.size()), 0, 16, 4, 0, 32, nullptr);
+    std::vector<float> ref(wimg.size());
+    const float bg[3] = {0.05f, 0.1f, 0.2f};
+    const float camv[11] = {cam.pos.x, cam.pos.y, cam.pos.z, cam.dir.x, cam.dir.y, cam.dir.z,
+                            cam.up.x,  cam.up.y,  cam.up.z,  cam.fov,   cam.far};
+    orc_whitted(ploc, xyz.data(), vn.data(), int64_t(tris.size()), mid.data(), mrows.data(), rc.width, rc.height,
+                rc.max_depth, rc.area_light_sample_cnt, 0, bg, camv, 0, ref.data(), 8);
+    orc_ploc_free(ploc);
+    size_t diff = 0;
+    double wsum = 0.0;
+    for (size_t i = 0; i < ref.size(); i++) {
+        diff += std::memcmp(&wimg[i], &ref[i], 4) != 0;
+        wsum += wimg[i];
+    }
+    Expect(diff == 0 && wsum > 0.0, "whitted renderer: bit-identical to the oracle");
+    std::printf("whitted: %zu of %zu values differ, image sum %.4f\n", diff, ref.size(), wsum);
 }
 
 }  // namespace
