@@ -384,7 +384,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     int64_t c_known = n;
     double rate = 1.15;  // clusters shrink by about this factor per iteration
     const bool time_iters = (p.keep_intermediates & 2) != 0;
-    const int cap = std::max(1, C.cu_count * (p.block_size == 512 ? 2 : (p.block_size == 128 ? 8 : 4)));
+    const int cap = std::max(1, C.cu_count * (p.block_size == 512 ? 4 : (p.block_size == 128 ? 16 : 8)));
     while (c_known > tail) {
         int est = int(std::ceil(std::log(double(c_known) / tail) / std::log(rate)));
         est = std::max(1, std::min(est, 96));

@@ -16,7 +16,7 @@ namespace {
 
 constexpr int kTail = 2048;        // clusters finished by the single-workgroup tail
 constexpr int kTailThreads = 1024;
-constexpr int kIpt = 4;            // clusters per thread in an iteration tile
+constexpr int kIpt = 2;            // at most this many clusters per thread in an iteration tile
 
 }  // namespace
 
@@ -177,7 +177,7 @@ constexpr uint64_t kFlagAgg = 1ull << 62;
 constexpr uint64_t kFlagInc = 2ull << 62;
 constexpr uint64_t kMask31 = 0x7FFFFFFFull;
 
-// Tiles hold BLOCK * ipt clusters, ipt in {1, 2, 4} chosen per iteration so that there are at
+// Tiles hold BLOCK * ipt clusters, ipt in {1, .., kIpt} chosen per iteration so that there are at
 // least `min_tiles` tiles: small iterations spread over more workgroups, each thread then
 // computing about one nearest neighbour instead of four.
 template <int BLOCK>
@@ -208,7 +208,8 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
         }
         return;
     }
-    const int ipt = c >= int64_t(BLOCK) * 4 * min_tiles ? 4 : (c >= int64_t(BLOCK) * 2 * min_tiles ? 2 : 1);
+    int ipt = 1;
+    while (ipt < kIpt && c >= int64_t(BLOCK) * 2 * ipt * min_tiles) ipt *= 2;
     const int chunk = BLOCK * ipt;
     const int T = (c + chunk - 1) / chunk;
     // The grid is sized from a count the host read some iterations ago; surplus workgroups
