@@ -1,8 +1,11 @@
 // Drives the C++ host interface the way the reference's renderer drives an
 // AccelerationStructure (src/renderer/cpu_raytracer.cc:56-57,112,236): build through the
 // factory, per-ray Intersect / IntersectAny, plus the batched calls, and the path tracer.
-//   host_api_test --cpu   interface checks that need no GPU
-//   host_api_test         the GPU checks; hits are compared with the oracle's brute force
+//   host_api_test --cpu [--scenes DIR]   interface checks that need no GPU
+//   host_api_test [--scenes DIR]         the GPU checks; hits are compared with the oracle's
+//                                        brute force, images with the oracle's renderer
+// DIR holds the OBJ scene fixtures (CornellBox-Original / CornellBox-Sphere); the scene
+// checks are skipped without it.
 // Prints one line per check group and "host_api_test: OK" at the end; exits non-zero on the
 // first failure.
 #include <cmath>
@@ -13,6 +16,7 @@
 #include <vector>
 
 #include "dpl_host.h"
+#include "scene_loader.h"
 
 extern "C" void orc_trace_brute(const float* tri, int64_t ntri, const void* rays, int64_t n, uint32_t flags, void* hits,
                                 int threads);
@@ -28,6 +32,7 @@ using namespace diplodocus_amd;
 namespace {
 
 int g_failures = 0;
+std::string g_scene_dir;
 
 void Expect(bool ok, const std::string& what) {
     if (!ok) {
@@ -127,7 +132,69 @@ void CpuChecks() {
         threw = true;
     }
     Expect(threw, "factory rejects the CPU structure types");
+    if (!g_scene_dir.empty()) {
+        // the reference's scene path: factory, Load(<dir>/<name>.meta + .obj)
+        SceneLoadConfig sc;
+        sc.dirpath = g_scene_dir;
+        sc.name = "CornellBox-Original";
+        const auto scene = CreateSceneLoader(sc)->Load(sc);
+        Expect(scene.has_value(), "scene loader: " + ObjSceneLoader::LastError());
+        if (scene) {
+            Expect(scene->Triangles().size() == 34 && scene->Materials().size() == 8 &&
+                       scene->AreaLights().size() == 2 && scene->PointLights().size() == 1,
+                   "scene loader: Cornell box counts");
+            const Camera& c = scene->GetCamera();
+            Expect(c.pos == Vec3{278.0f, 273.0f, -1000.0f} && c.fov == 0.6f && c.far == 2000.0f,
+                   "scene loader: camera from the .meta file");
+            const AreaLight& al = scene->AreaLights()[0];
+            Expect(scene->Materials()[size_t(scene->Triangles()[size_t(al.triangle_id)].material_id)].name == "light" &&
+                       al.color == Vec3{17.0f, 12.0f, 4.0f} && std::fabs(al.surface_area - 6825.0f) < 1e-2f,
+                   "scene loader: area light");
+        }
+        sc.name = "no-such-scene";
+        Expect(!ObjSceneLoader().Load(sc).has_value() && !ObjSceneLoader::LastError().empty(),
+               "scene loader: a missing scene fails with a reason");
+        Expect(ParseSceneLoadFormat("obj") == SceneLoadFormat::kObj, "scene load format");
+    }
     std::printf("cpu checks: %s\n", g_failures ? "failed" : "ok");
+}
+
+// Renders scene with the GPU Whitted renderer and with the oracle's restatement; true when
+// every value is bit-identical.
+bool WhittedMatchesOracle(const GpuBvh& bvh, const RenderConfig& rc, const Camera& cam, std::span<const Triangle> tris,
+                          std::span<const Material> mats, double* sum) {
+    const std::vector<float> img = GpuWhittedRenderer(bvh).Render(rc, cam, tris, mats);
+    std::vector<float> mrows(16 * mats.size(), 0.0f), vn(tris.size() * 9, 0.0f), xyz(tris.size() * 9);
+    std::vector<int32_t> mid(tris.size());
+    for (size_t m = 0; m < mats.size(); m++) {
+        const Vec3 f[4] = {mats[m].diffuse, mats[m].specular, mats[m].transmittance, mats[m].emission};
+        for (int k = 0; k < 4; k++) mrows[16 * m + 3 * k] = f[k].x, mrows[16 * m + 3 * k + 1] = f[k].y, mrows[16 * m + 3 * k + 2] = f[k].z;
+        mrows[16 * m + 12] = mats[m].shininess, mrows[16 * m + 13] = mats[m].ior, mrows[16 * m + 14] = mats[m].r_ior;
+    }
+    for (size_t i = 0; i < tris.size(); i++) {
+        mid[i] = tris[i].material_id;
+        const Vertex* v[3] = {&tris[i].v0, &tris[i].v1, &tris[i].v2};
+        for (int k = 0; k < 3; k++) {
+            xyz[9 * i + 3 * k] = v[k]->pos.x, xyz[9 * i + 3 * k + 1] = v[k]->pos.y, xyz[9 * i + 3 * k + 2] = v[k]->pos.z;
+            if (tris[i].has_vertex_normals)
+                vn[9 * i + 3 * k] = v[k]->normal.x, vn[9 * i + 3 * k + 1] = v[k]->normal.y, vn[9 * i + 3 * k + 2] = v[k]->normal.z;
+        }
+    }
+    void* ploc = orc_ploc_build(xyz.data(), int64_t(tris.size()), 0, 16, 4, 0, 32, nullptr);
+    std::vector<float> ref(img.size());
+    const float bg[3] = {rc.background_color.x, rc.background_color.y, rc.background_color.z};
+    const float camv[11] = {cam.pos.x, cam.pos.y, cam.pos.z, cam.dir.x, cam.dir.y, cam.dir.z,
+                            cam.up.x,  cam.up.y,  cam.up.z,  cam.fov,   cam.far};
+    orc_whitted(ploc, xyz.data(), vn.data(), int64_t(tris.size()), mid.data(), mrows.data(), rc.width, rc.height,
+                rc.max_depth, rc.area_light_sample_cnt, rc.backface_culling ? 1 : 0, bg, camv, 0, ref.data(), 8);
+    orc_ploc_free(ploc);
+    size_t diff = 0;
+    *sum = 0.0;
+    for (size_t i = 0; i < ref.size(); i++) {
+        diff += std::memcmp(&img[i], &ref[i], 4) != 0;
+        *sum += img[i];
+    }
+    return diff == 0;
 }
 
 void GpuChecks() {
@@ -262,12 +329,42 @@ void GpuChecks() {
     }
     Expect(diff == 0 && wsum > 0.0, "whitted renderer: bit-identical to the oracle");
     std::printf("whitted: %zu of %zu values differ, image sum %.4f\n", diff, ref.size(), wsum);
+
+    if (!g_scene_dir.empty()) {
+        // load -> build through the factory -> render, the reference's renderer path with its
+        // sample configuration (max depth 8, 8 area-light samples)
+        SceneLoadConfig sc;
+        sc.dirpath = g_scene_dir;
+        sc.name = "CornellBox-Sphere";
+        const auto scene = CreateSceneLoader(sc)->Load(sc);
+        Expect(scene.has_value(), "scene loader (GPU path): " + ObjSceneLoader::LastError());
+        if (scene) {
+            AccelerationStructureConfig ac;
+            ac.acceleration_structure_type = AccelerationStructureType::kPlocEmcSobb;
+            auto as = CreateGpuAccelerationStructure(ac, ctx);
+            Stats st;
+            as->Build(ac, st, scene->Triangles());
+            RenderConfig src;
+            src.width = 96, src.height = 96, src.max_depth = 8, src.area_light_sample_cnt = 8;
+            double ssum = 0.0;
+            const bool same = WhittedMatchesOracle(static_cast<const GpuBvh&>(*as), src, scene->GetCamera(),
+                                                   scene->Triangles(), scene->Materials(), &ssum);
+            Expect(same && ssum > 0.0, "loaded CornellBox-Sphere: Whitted bit-identical to the oracle");
+            std::printf("loaded scene: %zu triangles, %d nodes, whitted image sum %.4f\n", scene->Triangles().size(),
+                        st.accel_stats.node_count, ssum);
+        }
+    }
 }
 
 }  // namespace
 
 int main(int argc, char** argv) {
-    const bool cpu_only = argc > 1 && std::string(argv[1]) == "--cpu";
+    bool cpu_only = false;
+    for (int i = 1; i < argc; i++) {
+        const std::string a = argv[i];
+        if (a == "--cpu") cpu_only = true;
+        if (a == "--scenes" && i + 1 < argc) g_scene_dir = argv[++i];
+    }
     CpuChecks();
     if (!cpu_only) {
         try {

@@ -1,5 +1,6 @@
-"""The C-ABI library loads and exports every symbol include/dpl/dpl.h declares; host-side
-logic that needs no GPU (parameter bridge, error paths)."""
+"""The C-ABI libraries load and export every symbol their headers declare (include/dpl/dpl.h:
+libdpl.so; include/dpl/dpl_scene.h: libdpl_scene.so); host-side logic that needs no GPU
+(parameter bridge, error paths)."""
 import ctypes as C
 import os
 import re
@@ -9,8 +10,8 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def header_symbols():
-    text = open(os.path.join(ROOT, "include", "dpl", "dpl.h")).read()
+def header_symbols(header="dpl.h"):
+    text = open(os.path.join(ROOT, "include", "dpl", header)).read()
     decl = r"^(?:dpl_status|void|const char\s*\*|int32_t)\s+(dpl_[a-z_0-9]+)\s*\("
     return sorted(set(re.findall(decl, text, flags=re.M)))
 
@@ -23,6 +24,29 @@ def test_library_exports_every_declared_symbol():
     for s in syms:
         assert hasattr(_lib.lib, s), s
     assert sorted(_lib.EXPORTED) == syms
+
+
+def test_scene_library_exports_every_declared_symbol():
+    from diplodocus_amd import scene_io
+
+    syms = header_symbols("dpl_scene.h")
+    assert len(syms) == 10
+    for s in syms:
+        assert hasattr(scene_io.lib(), s), s
+    assert sorted(scene_io.EXPORTED) == syms
+
+
+def test_scene_library_null_arguments():
+    from diplodocus_amd import scene_io
+
+    L = scene_io.lib()
+    assert L.dpl_scene_load(None, b"x", None) == 1
+    assert b"NULL" in L.dpl_scene_last_error()
+    assert L.dpl_scene_load_obj(None, None) == 1
+    assert L.dpl_scene_get_info(None, None) == 1
+    assert L.dpl_scene_get_triangles(None, None, None, None, None, None) == 1
+    assert L.dpl_scene_get_material_name(None, 0, None, 0) == 1
+    L.dpl_scene_destroy(None)
 
 
 def test_params_bridge_mirrors_reference_switch():

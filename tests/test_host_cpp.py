@@ -1,5 +1,6 @@
 """The C++20 host interface (diplodocus_amd/host: AccelerationStructure-shaped GpuBvh,
-factory, path tracer) driven by its C++ test program tests/cpp/host_api_test.cc."""
+factory, path tracer, Whitted renderer, OBJ scene loader) driven by its C++ test program
+tests/cpp/host_api_test.cc."""
 import os
 import subprocess
 
@@ -7,6 +8,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 EXE = os.path.join(ROOT, "diplodocus_amd", "host", "_build", "host_api_test")
+SCENES = ["--scenes", os.path.join(ROOT, "tests", "golden", "obj")]
 
 
 def run(args, timeout):
@@ -18,9 +20,9 @@ def run(args, timeout):
 
 
 def test_host_api_cpu_checks():
-    run(["--cpu"], 60)
+    run(["--cpu"] + SCENES, 60)
 
 
 @pytest.mark.gpu
 def test_host_api_gpu():
-    run([], 240)
+    run(SCENES, 240)
