@@ -55,6 +55,15 @@ class WhittedParams(C.Structure):
                 ("background", C.c_float * 3), ("camera", C.c_float * 11)]
 
 
+class WhittedStats(C.Structure):
+    _fields_ = [("primary_ray_count", C.c_int64), ("secondary_ray_count", C.c_int64),
+                ("shadow_ray_count", C.c_int64), ("query_count", C.c_int64), ("traversal_count", C.c_int64),
+                ("intersection_count", C.c_int64), ("raytracing_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class MeshInfo(C.Structure):
     _fields_ = [("node_offset", C.c_int64), ("prim_offset", C.c_int64), ("node_count", C.c_int32),
                 ("prim_count", C.c_int32), ("tree_height", C.c_int32), ("ploc_iterations", C.c_int32),
@@ -111,6 +120,8 @@ def _load() -> C.CDLL:
         "dpl_render_adjoint": (I32, [P, P, C.POINTER(RenderParams), P, P, P, P, P, P]),
         "dpl_bvh_get_meshes": (I32, [P, P]),
         "dpl_render_whitted": (I32, [P, P, C.POINTER(WhittedParams), P, P, P, P, I32, P, I32, P]),
+        "dpl_render_whitted_stats": (I32, [P, P, C.POINTER(WhittedParams), P, P, P, P, I32, P, I32, P,
+                                           C.POINTER(WhittedStats)]),
         "dpl_trace_graph_create": (I32, [P, P, P, P, I64, U32, P, C.POINTER(P)]),
         "dpl_trace_graph_launch": (I32, [P]),
         "dpl_trace_graph_destroy": (None, [P]),
@@ -134,5 +145,5 @@ EXPORTED = [
     "dpl_trace_batched",
     "dpl_bvh_get_layout", "dpl_bvh_download", "dpl_compute_codes", "dpl_radix_sort_pairs", "dpl_render_forward",
     "dpl_render_adjoint", "dpl_bvh_get_meshes", "dpl_trace_graph_create", "dpl_trace_graph_launch",
-    "dpl_trace_graph_destroy", "dpl_autotune", "dpl_render_whitted",
+    "dpl_trace_graph_destroy", "dpl_autotune", "dpl_render_whitted", "dpl_render_whitted_stats",
 ]

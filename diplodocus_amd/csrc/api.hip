@@ -1072,9 +1072,11 @@ static dpl_status Render(bool adjoint, dpl_context* ctx, const dpl_bvh* bvh, con
     });
 }
 
-dpl_status dpl_render_whitted(dpl_context* ctx, const dpl_bvh* bvh, const dpl_whitted_params* wp, const float* tri,
-                              const float* vn, const int32_t* mat_id, const dpl_material* mats, int32_t material_count,
-                              const int32_t* lights, int32_t light_count, float* image) {
+namespace {
+
+dpl_status WhittedImpl(dpl_context* ctx, const dpl_bvh* bvh, const dpl_whitted_params* wp, const float* tri,
+                       const float* vn, const int32_t* mat_id, const dpl_material* mats, int32_t material_count,
+                       const int32_t* lights, int32_t light_count, float* image, dpl_whitted_stats* stats) {
     if (!ctx || !bvh || !wp) return ctx ? Fail(ctx, DPL_ERR_INVALID_ARG, "null argument") : DPL_ERR_INVALID_ARG;
     return Guard(ctx, [&] {
         Require(bvh->ctx == ctx, "bvh belongs to another context");
@@ -1090,11 +1092,49 @@ dpl_status dpl_render_whitted(dpl_context* ctx, const dpl_bvh* bvh, const dpl_wh
         TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
                        int32_t(bvh->prim_count), bvh->tree_height, bvh->ndirs, bvh->sobb_dirs.as<float>()};
         int32_t* spill = ctx->spill.ensure<int32_t>(std::max<size_t>(16, TraceSpillBytes(ctx->cu_count, bvh->tree_height)));
+        uint8_t* tc = ctx->trace_counter.ensure<uint8_t>(256);
+        // the six totals sit past the pixel counter; zeroed by the launch
+        unsigned long long* totals = stats ? reinterpret_cast<unsigned long long*>(tc + 64) : nullptr;
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        if (stats) {
+            DPL_HIP(hipEventCreate(&ev[0]));
+            DPL_HIP(hipEventCreate(&ev[1]));
+            DPL_HIP(hipEventRecord(ev[0], ctx->stream));
+        }
         hipError_t e = LaunchWhitted(tree, *wp, basis, tri, vn, mat_id, mats, lights, light_count, image, ctx->cu_count,
-                                     ctx->trace_counter.as<uint32_t>(), spill, ctx->stream);
+                                     reinterpret_cast<uint32_t*>(tc), spill, totals, ctx->stream);
+        if (stats) {
+            unsigned long long v[6] = {};
+            hipError_t e2 = hipEventRecord(ev[1], ctx->stream);
+            if (e2 == hipSuccess) e2 = hipMemcpyAsync(v, totals, sizeof(v), hipMemcpyDeviceToHost, ctx->stream);
+            if (e2 == hipSuccess) e2 = hipStreamSynchronize(ctx->stream);
+            float ms = 0.0f;
+            if (e2 == hipSuccess) e2 = hipEventElapsedTime(&ms, ev[0], ev[1]);
+            (void)hipEventDestroy(ev[0]);
+            (void)hipEventDestroy(ev[1]);
+            if (e == hipSuccess) DPL_HIP(e2);
+            *stats = dpl_whitted_stats{int64_t(v[0]), int64_t(v[1]), int64_t(v[2]), int64_t(v[3]), int64_t(v[4]),
+                                       int64_t(v[5]), double(ms)};
+        }
         if (e == hipErrorNotSupported) throw Error(DPL_ERR_UNSUPPORTED, "tree height exceeds the traversal stack");
         DPL_HIP(e);
     });
+}
+
+}  // namespace
+
+dpl_status dpl_render_whitted(dpl_context* ctx, const dpl_bvh* bvh, const dpl_whitted_params* wp, const float* tri,
+                              const float* vn, const int32_t* mat_id, const dpl_material* mats, int32_t material_count,
+                              const int32_t* lights, int32_t light_count, float* image) {
+    return WhittedImpl(ctx, bvh, wp, tri, vn, mat_id, mats, material_count, lights, light_count, image, nullptr);
+}
+
+dpl_status dpl_render_whitted_stats(dpl_context* ctx, const dpl_bvh* bvh, const dpl_whitted_params* wp,
+                                    const float* tri, const float* vn, const int32_t* mat_id, const dpl_material* mats,
+                                    int32_t material_count, const int32_t* lights, int32_t light_count, float* image,
+                                    dpl_whitted_stats* stats) {
+    if (!stats) return ctx ? Fail(ctx, DPL_ERR_INVALID_ARG, "null stats") : DPL_ERR_INVALID_ARG;
+    return WhittedImpl(ctx, bvh, wp, tri, vn, mat_id, mats, material_count, lights, light_count, image, stats);
 }
 
 dpl_status dpl_render_forward(dpl_context* ctx, const dpl_bvh* bvh, const dpl_render_params* rp, const int32_t* mat_id,

@@ -323,7 +323,7 @@ constexpr int kDplWhittedMaxDepth = 8;
 hipError_t LaunchWhitted(const TraceTree& tree, const dpl_whitted_params& wp, const float* basis, const float* tri,
                          const float* vn, const int32_t* mat_id, const dpl_material* mats, const int32_t* lights,
                          int32_t light_count, float* image, int cu_count, uint32_t* counter, int32_t* spill,
-                         hipStream_t s);
+                         unsigned long long* totals, hipStream_t s);
 
 constexpr int kMaxDirs = 13;
 constexpr int kMaxTriples = 286;

@@ -9,7 +9,9 @@
 // sum over the ray tree, depth first, reflection subtree first: a per-lane stack of
 // (ray, weight, depth) items. Persistent grid: waves take 64 pixels at a time from a counter.
 // Every float operation follows the oracle's restatement in order (no FMA contraction), so
-// the image is bit-identical to it.
+// the image is bit-identical to it. With COUNT the kernel also keeps the reference's ray
+// counters (primary, secondary, shadow: cpu_raytracer.cc:103,131,147,223) and the traversal
+// work of every query, summed per wave and added to six 64-bit totals.
 #include <hip/hip_runtime.h>
 
 #include "dpl_internal.h"
@@ -125,8 +127,11 @@ struct Tracer {
     SobbRay& sr;
     const float* s_dirs;
     TraceCounters cnt{0u, 0u};
+    uint32_t queries = 0;  // closest-hit and any-hit traces
+    uint32_t shadow = 0;   // IsShadowed calls
 
     __device__ HitResult Trace(V3 o, V3 d, float tmax, bool any, bool cull) {
+        queries++;
         const WtRay w = PrepareRay(o.x, o.y, o.z, kEps, d.x, d.y, d.z, tmax);
         if (SOBB) PrepareSobb<BLOCK>(sr, s_dirs, a.tree.ndirs, o.x, o.y, o.z, d.x, d.y, d.z);
         return any ? Traverse<true, SOBB>(a.tree, w, sr, cull, st, cnt)
@@ -163,7 +168,8 @@ __device__ V3 LocalAreaLights(const WhittedArgs& a, Tracer<SOBB, BLOCK>& T, int 
             const float u = r1 + r2 > 1.0f ? 1.0f - r1 : r1;
             const float v = r1 + r2 > 1.0f ? 1.0f - r2 : r2;
             const V3 lpos = Add(Add(v0, Mul(Sub(v1, v0), u)), Mul(Sub(v2, v0), v));
-            // IsShadowed (cpu_raytracer.cc:214-229)
+            // IsShadowed (cpu_raytracer.cc:222-236), counted on entry
+            T.shadow++;
             const V3 to_l = Sub(lpos, h.pos);
             const float dist = Len(to_l);
             if (!(dist < kEps)) {
@@ -191,8 +197,8 @@ __device__ V3 LocalAreaLights(const WhittedArgs& a, Tracer<SOBB, BLOCK>& T, int 
 
 }  // namespace
 
-template <bool SOBB, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* counter) {
+template <bool SOBB, int BLOCK, bool COUNT>
+__global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* counter, unsigned long long* totals) {
     __shared__ int32_t s_stack[kWhittedStack][BLOCK];
     __shared__ uint32_t s_base[BLOCK / kWave];
     __shared__ float s_on[SOBB ? kMaxDirs : 1][BLOCK];
@@ -211,6 +217,7 @@ __global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* count
     const V3 cam{a.basis[0], a.basis[1], a.basis[2]}, p00{a.basis[3], a.basis[4], a.basis[5]};
     const V3 qw{a.basis[6], a.basis[7], a.basis[8]}, qh{a.basis[9], a.basis[10], a.basis[11]};
     const V3 bg{a.background[0], a.background[1], a.background[2]};
+    uint32_t primary = 0, secondary = 0;
     for (;;) {
         if (lane == 0) s_base[wid] = atomicAdd(counter, uint32_t(kWave));
         __builtin_amdgcn_wave_barrier();
@@ -219,6 +226,7 @@ __global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* count
         if (base >= pixels) break;
         const int64_t p = base + lane;
         if (p < pixels) {
+            primary++;
             const int x = int(p % a.W), y = int(p / a.W);
             Item stack[kDplWhittedMaxDepth + 2];
             int sp = 0;
@@ -273,12 +281,14 @@ __global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* count
                         }
                         stack[sp++] = Item{OffsetRayOrigin(h.pos, h.eps, h.gn, dir), dir, Mul(it.w, Ld(m.transmittance)),
                                            it.tmax, it.depth + 1};
+                        secondary++;
                     }
                     if (NonZero(m.specular)) {  // cpu_raytracer.cc:257-262
                         const V3 dv = Neg(it.d);
                         const V3 dir = Add(Neg(dv), Mul(h.normal, 2.0f * Dot(dv, h.normal)));
                         stack[sp++] = Item{OffsetRayOrigin(h.pos, h.eps, h.gn, dir), dir, Mul(it.w, Ld(m.specular)),
                                            it.tmax, it.depth + 1};
+                        secondary++;
                     }
                 }
             }
@@ -288,16 +298,33 @@ __global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* count
         }
         __builtin_amdgcn_wave_barrier();
     }
+    if (COUNT) {
+        unsigned long long v[6] = {primary, secondary, T.shadow, T.queries, T.cnt.nodes, T.cnt.tris};
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+#pragma unroll
+            for (int d = kWave / 2; d > 0; d >>= 1) v[k] += __shfl_xor(v[k], d, kWave);
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < 6; k++) atomicAdd(&totals[k], v[k]);
+        }
+    }
 }
 
-hipError_t LaunchResetCounter(uint32_t* counter, hipStream_t s);
+__global__ void KResetWhitted(uint32_t* counter, unsigned long long* totals) {
+    counter[0] = 0u;
+    if (totals)
+        for (int k = 0; k < 6; k++) totals[k] = 0ull;
+}
+
 int TraceGrid(int cu_count, int64_t n);
 int32_t TraceSpillStride(int height);
 
 hipError_t LaunchWhitted(const TraceTree& tree, const dpl_whitted_params& wp, const float* basis, const float* tri,
                          const float* vn, const int32_t* mat_id, const dpl_material* mats, const int32_t* lights,
                          int32_t light_count, float* image, int cu_count, uint32_t* counter, int32_t* spill,
-                         hipStream_t s) {
+                         unsigned long long* totals, hipStream_t s) {
     WhittedArgs a{};
     a.tree = tree;
     a.tri = tri;
@@ -315,14 +342,22 @@ hipError_t LaunchWhitted(const TraceTree& tree, const dpl_whitted_params& wp, co
     a.spill = spill;
     a.spill_stride = TraceSpillStride(tree.height);
     if (a.max_depth < 0 || a.max_depth > kDplWhittedMaxDepth) return hipErrorInvalidValue;
-    hipError_t e = LaunchResetCounter(counter, s);
+    hipLaunchKernelGGL(KResetWhitted, dim3(1), dim3(1), 0, s, counter, totals);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     constexpr int B = 256;
     const int grid = TraceGrid(cu_count, int64_t(a.W) * a.H);
-    if (tree.ndirs > 0)
-        hipLaunchKernelGGL((KWhitted<true, B>), dim3(grid), dim3(B), 0, s, a, counter);
-    else
-        hipLaunchKernelGGL((KWhitted<false, B>), dim3(grid), dim3(B), 0, s, a, counter);
+    if (tree.ndirs > 0) {
+        if (totals)
+            hipLaunchKernelGGL((KWhitted<true, B, true>), dim3(grid), dim3(B), 0, s, a, counter, totals);
+        else
+            hipLaunchKernelGGL((KWhitted<true, B, false>), dim3(grid), dim3(B), 0, s, a, counter, totals);
+    } else {
+        if (totals)
+            hipLaunchKernelGGL((KWhitted<false, B, true>), dim3(grid), dim3(B), 0, s, a, counter, totals);
+        else
+            hipLaunchKernelGGL((KWhitted<false, B, false>), dim3(grid), dim3(B), 0, s, a, counter, totals);
+    }
     return hipGetLastError();
 }
 

@@ -223,12 +223,15 @@ std::unique_ptr<AccelerationStructure> CreateGpuAccelerationStructure(const Acce
 // The reference's Whitted CpuRaytracer (src/renderer/cpu_raytracer.cc:50-288) on the GPU over
 // a GpuBvh built from the same triangles (dpl_render_whitted): area lights are the emissive
 // triangles in triangle order, as the reference's loader makes them. Returns the framebuffer
-// (row-major, height x width x 3, src/framebuffer/framebuffer.cc:47-52).
+// (row-major, height x width x 3, src/framebuffer/framebuffer.cc:47-52). With stats, adds the
+// render's counters as the reference's renderer does (cpu_raytracer.cc:84-88,103,131,147,223):
+// primary / secondary / shadow rays, raytracing_time (s, device time), and the
+// acceleration-structure queries with their traversal work (dpl_render_whitted_stats).
 class GpuWhittedRenderer {
    public:
     explicit GpuWhittedRenderer(const GpuBvh& bvh) : bvh_(bvh) {}
     std::vector<float> Render(const RenderConfig& cfg, const Camera& cam, std::span<const Triangle> triangles,
-                              std::span<const Material> materials) const;
+                              std::span<const Material> materials, Stats* stats = nullptr) const;
 
    private:
     const GpuBvh& bvh_;

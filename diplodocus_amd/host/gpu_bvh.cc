@@ -199,8 +199,8 @@ struct DeviceArray {
 }  // namespace
 
 std::vector<float> GpuWhittedRenderer::Render(const RenderConfig& cfg, const Camera& cam,
-                                              std::span<const Triangle> triangles,
-                                              std::span<const Material> materials) const {
+                                              std::span<const Triangle> triangles, std::span<const Material> materials,
+                                              Stats* stats) const {
     const size_t n = triangles.size();
     std::vector<float> xyz(n * 9), vn(n * 9, 0.0f);
     std::vector<int32_t> mat(n), lights;
@@ -247,9 +247,26 @@ std::vector<float> GpuWhittedRenderer::Render(const RenderConfig& cfg, const Cam
     DeviceArray<dpl_material> d_mats(dm);
     DeviceFloats img(size_t(cfg.width) * cfg.height * 3);
     const auto& ctx = bvh_.context();
-    ctx->Check(dpl_render_whitted(ctx->get(), bvh_.handle(), &wp, d_xyz.p, any_vn ? d_vn.p : nullptr, d_mat.p,
-                                  d_mats.p, int32_t(dm.size()), lights.empty() ? nullptr : d_lights.p,
-                                  int32_t(lights.size()), img.p));
+    if (stats) {
+        dpl_whitted_stats ws{};
+        ctx->Check(dpl_render_whitted_stats(ctx->get(), bvh_.handle(), &wp, d_xyz.p, any_vn ? d_vn.p : nullptr,
+                                            d_mat.p, d_mats.p, int32_t(dm.size()),
+                                            lights.empty() ? nullptr : d_lights.p, int32_t(lights.size()), img.p, &ws));
+        auto add = [](int& dst, int64_t v) { dst = int(std::min<int64_t>(int64_t(dst) + v, INT32_MAX)); };
+        RaytracingStats& r = stats->rt_stats;
+        r.raytracing_time = ws.raytracing_ms * 1e-3;
+        add(r.primary_ray_count, ws.primary_ray_count);
+        add(r.secondary_ray_count, ws.secondary_ray_count);
+        add(r.shadow_ray_count, ws.shadow_ray_count);
+        AccelerationStats& a = stats->accel_stats;
+        add(a.query_count, ws.query_count);
+        add(a.traversal_count, ws.traversal_count);
+        add(a.intersection_count, ws.intersection_count);
+    } else {
+        ctx->Check(dpl_render_whitted(ctx->get(), bvh_.handle(), &wp, d_xyz.p, any_vn ? d_vn.p : nullptr, d_mat.p,
+                                      d_mats.p, int32_t(dm.size()), lights.empty() ? nullptr : d_lights.p,
+                                      int32_t(lights.size()), img.p));
+    }
     ctx->Check(dpl_context_synchronize(ctx->get()));
     std::vector<float> out(size_t(cfg.width) * cfg.height * 3);
     img.Download(out);

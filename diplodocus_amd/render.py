@@ -101,17 +101,23 @@ def emissive_triangles(mat_id, materials) -> "np.ndarray":
     return np.nonzero(emits[np.asarray(mat_id)])[0].astype(np.int32)
 
 
-def render_whitted(bvh, wp: WhittedParams, tris, mat_id, materials, lights, vertex_normals=None, image=None):
+def render_whitted(bvh, wp: WhittedParams, tris, mat_id, materials, lights, vertex_normals=None, image=None,
+                   stats=False):
     """GPU Whitted image (H, W, 3). tris / mat_id / materials (M, 16) / lights / vertex_normals
     are GPU tensors (materials rows: diffuse3 specular3 transmittance3 emission3 shininess ior
-    r_ior pad)."""
+    r_ior pad). stats=True (dpl_render_whitted_stats) returns (image, counters dict): the
+    reference's ray counters, the queries with their traversal work, and the kernel time."""
     import torch
 
     if image is None:
         image = torch.empty((wp.height, wp.width, 3), dtype=torch.float32, device=tris.device)
     m = int(materials.shape[0])
     nl = int(lights.shape[0])
-    bvh.ctx.check(lib.dpl_render_whitted(bvh.ctx.h, bvh.h, C.byref(wp), _dev(tris),
-                                         _dev(vertex_normals) if vertex_normals is not None else None, _dev(mat_id),
-                                         _dev(materials), m, _dev(lights) if nl else None, nl, _dev(image)))
+    args = (bvh.ctx.h, bvh.h, C.byref(wp), _dev(tris), _dev(vertex_normals) if vertex_normals is not None else None,
+            _dev(mat_id), _dev(materials), m, _dev(lights) if nl else None, nl, _dev(image))
+    if stats:
+        st = _lib.WhittedStats()
+        bvh.ctx.check(lib.dpl_render_whitted_stats(*args, C.byref(st)))
+        return image, st.as_dict()
+    bvh.ctx.check(lib.dpl_render_whitted(*args))
     return image

@@ -235,6 +235,30 @@ dpl_status dpl_render_whitted(dpl_context* ctx, const dpl_bvh* bvh, const dpl_wh
                               const dpl_material* materials, int32_t material_count, const int32_t* light_tris,
                               int32_t light_count, float* image);
 
+/* The counters a render keeps, as the reference's CpuRaytracer keeps them in Stats:
+ * RaytracingStats (src/stats/raytracing_stats.h:7-11; counted at cpu_raytracer.cc:103 per
+ * pixel, :131 / :147 per reflection / refraction ray, :223 per IsShadowed call) and the
+ * AccelerationStats query counters (acceleration_stats.h:14-16): one query per closest-hit or
+ * any-hit trace; traversal_count = inner nodes visited and intersection_count = triangle
+ * tests of those queries (the BVH's own work, not the reference SBvh's). */
+typedef struct dpl_whitted_stats {
+    int64_t primary_ray_count;
+    int64_t secondary_ray_count;
+    int64_t shadow_ray_count;
+    int64_t query_count;
+    int64_t traversal_count;
+    int64_t intersection_count;
+    double raytracing_ms; /* device time of the render kernel */
+} dpl_whitted_stats;
+
+/* dpl_render_whitted that also fills *stats (host memory). Synchronises. Replaces the
+ * stats bookkeeping of CpuRaytracer::StartRender / GpuRenderer::StartRender
+ * (cpu_raytracer.cc:84-88, src/renderer/gpu_renderer.cc:57-61). */
+dpl_status dpl_render_whitted_stats(dpl_context* ctx, const dpl_bvh* bvh, const dpl_whitted_params* params,
+                                    const float* tri_xyz, const float* vertex_normals, const int32_t* mat_id,
+                                    const dpl_material* materials, int32_t material_count, const int32_t* light_tris,
+                                    int32_t light_count, float* image, dpl_whitted_stats* stats);
+
 /* Autotuner: times builds of the given workload over nn_search_radius {8, 16, 32}, block_size
  * {128, 256, 512} ({256, 512} batched) and sort_digit_bits {4, 8}, and returns in *best the
  * fastest whose SAH cost is at most (1 + sah_tolerance) x the lowest SAH among them (the

@@ -240,6 +240,13 @@ int32_t orc_sobb_set(int32_t kdop_size, int32_t* ndirs, float* dirs, int32_t* ab
 void orc_ploc_trace(void* h, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads) {
     TraceFinal(static_cast<PlocHandle*>(h)->r.tree, rays, n, flags, hits, threads);
 }
+// The same trace with its traversal work: counts[0] inner nodes visited, counts[1] triangle tests.
+void orc_ploc_trace_counted(void* h, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads,
+                            int64_t* counts) {
+    TraceCount tc;
+    TraceFinalCounted(static_cast<PlocHandle*>(h)->r.tree, rays, n, flags, hits, threads, tc);
+    counts[0] = tc.nodes, counts[1] = tc.tris;
+}
 
 // Trace over any tree in the final layout (e.g. one downloaded from the device).
 // sobb_kdop: 0 for AABB child boxes, else the kdop_size of the SOBB direction set.
@@ -254,6 +261,22 @@ void orc_trace_final(const void* fnodes, int64_t nnodes, const float* tri_ordere
         t.set = MakeSobbSet(sobb_kdop);
     }
     TraceFinal(t, rays, n, flags, hits, threads);
+}
+// orc_trace_final with the traversal work: counts[0] inner nodes visited, counts[1] triangle tests.
+void orc_trace_final_counted(const void* fnodes, int64_t nnodes, const float* tri_ordered, int64_t nprim,
+                             const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads,
+                             int32_t sobb_kdop, int64_t* counts) {
+    FinalTree t;
+    t.nodes.resize(nnodes);
+    std::memcpy(t.nodes.data(), fnodes, size_t(nnodes) * sizeof(FinalTree::Node));
+    t.tri_ordered.assign(tri_ordered, tri_ordered + 12 * nprim);
+    if (sobb_kdop) {
+        t.sobb = true;
+        t.set = MakeSobbSet(sobb_kdop);
+    }
+    TraceCount tc;
+    TraceFinalCounted(t, rays, n, flags, hits, threads, tc);
+    counts[0] = tc.nodes, counts[1] = tc.tris;
 }
 // SAH of a downloaded tree; root_box6 (lo xyz, hi xyz) is the tree's axis-aligned box.
 double orc_sah_final(const void* fnodes, int64_t nnodes, const float* root_box6, int32_t sobb_kdop) {
@@ -274,14 +297,20 @@ void orc_trace_brute(const float* tri, int64_t ntri, const RayRec* rays, int64_t
 
 // Whitted ray tracing over the PLOC tree of `ploc` (closest hits are tree-independent).
 // mats: M x 16 floats (diffuse3 specular3 transmittance3 emission3 shininess ior r_ior pad).
+// counts (may be NULL): primary, secondary, shadow, queries, inner nodes, triangle tests.
 void orc_whitted(void* ploc, const float* tri_xyz, const float* vertex_normals, int64_t n, const int32_t* mat_id,
                  const float* mats, int W, int H, int max_depth, int samples, int cull, const float* background,
-                 const float* cam, int recursive, float* image, int threads) {
+                 const float* cam, int recursive, float* image, int threads, int64_t* counts) {
     const FinalTree& T = static_cast<PlocHandle*>(ploc)->r.tree;
     WParams p{W, H, max_depth, samples, cull, {background[0], background[1], background[2]}, {}};
     for (int k = 0; k < 11; k++) p.cam[k] = cam[k];
+    WStats st;
     WhittedRender(T, tri_xyz, vertex_normals, n, mat_id, reinterpret_cast<const WMaterial*>(mats), p, recursive != 0,
-                  image, threads);
+                  image, threads, counts ? &st : nullptr);
+    if (counts) {
+        const int64_t c[6] = {st.primary, st.secondary, st.shadow, st.queries, st.nodes, st.tris};
+        std::memcpy(counts, c, sizeof(c));
+    }
 }
 float orc_pow_pos(float x, float y) { return PowPos(x, y); }
 float orc_area_light_sample01(uint32_t seed, uint32_t px, uint32_t py, uint32_t light_id, uint32_t s, uint32_t dim) {

@@ -297,7 +297,17 @@ bool IntersectWatertight(const WtRay& w, const float* v0, const float* v1, const
 bool IntersectBoxRobust(const WtRay& w, float tmin, float tmax, const float lo[3], const float hi[3], float& tnear);
 
 void TraceFinal(const FinalTree& t, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads);
-void TraceFinalOne(const FinalTree& t, const RayRec& r, uint32_t flags, HitRec& h);
+// Traversal work of a trace (the counterparts of AccelerationStats::traversal_count and
+// intersection_count, src/stats/acceleration_stats.h:15-16): inner nodes visited and
+// triangle tests.
+struct TraceCount {
+    int64_t nodes = 0;
+    int64_t tris = 0;
+};
+void TraceFinalOne(const FinalTree& t, const RayRec& r, uint32_t flags, HitRec& h, TraceCount* count = nullptr);
+// TraceFinal with the traversal work summed over all rays.
+void TraceFinalCounted(const FinalTree& t, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads,
+                       TraceCount& total);
 void TraceBrute(const float* tri_xyz, int64_t ntri, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits,
                 int threads);
 double IntersectFp64(const float* tri_xyz9, const RayRec& r, double& b1, double& b2);
@@ -332,8 +342,23 @@ struct WParams {
 float PowPos(float x, float y);
 float AreaLightSample01(uint32_t seed, uint32_t px, uint32_t py, uint32_t light_id, uint32_t sample_idx,
                         uint32_t dimension);
+// The counters the reference's CpuRaytracer keeps (src/renderer/cpu_raytracer.cc:103,131,147,
+// 223; acceleration stats from every Intersect / IntersectAny, sbvh.cc:61,148).
+struct WStats {
+    int64_t primary = 0;    // one per pixel
+    int64_t secondary = 0;  // reflection and refraction rays
+    int64_t shadow = 0;     // IsShadowed calls
+    int64_t queries = 0;    // closest-hit and any-hit traces
+    int64_t nodes = 0;      // inner nodes visited
+    int64_t tris = 0;       // triangle tests
+    void Add(const WStats& o) {
+        primary += o.primary, secondary += o.secondary, shadow += o.shadow;
+        queries += o.queries, nodes += o.nodes, tris += o.tris;
+    }
+};
 void WhittedRender(const FinalTree& tree, const float* tri, const float* vn, int64_t n, const int32_t* mat_id,
-                   const WMaterial* mats, const WParams& p, bool recursive, float* image, int threads);
+                   const WMaterial* mats, const WParams& p, bool recursive, float* image, int threads,
+                   WStats* stats = nullptr);
 int TracePathVertices(const FinalTree& T, const float* tri_xyz, const int32_t* mat_id, const float* albedo,
                       const float* basis, float far, int x, int y, int W, int spp, int s, int depth, uint64_t seed,
                       int32_t* mats, Vec3* betas);

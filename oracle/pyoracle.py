@@ -61,7 +61,9 @@ def _load():
         "orc_ploc_export": (None, [P] * 14),
         "orc_ploc_sah": (D, [P]),
         "orc_ploc_trace": (None, [P, P, I64, U32, P, I32]),
+        "orc_ploc_trace_counted": (None, [P, P, I64, U32, P, I32, P]),
         "orc_trace_final": (None, [P, I64, P, I64, P, I64, U32, P, I32, I32]),
+        "orc_trace_final_counted": (None, [P, I64, P, I64, P, I64, U32, P, I32, I32, P]),
         "orc_sah_final": (D, [P, I64, P, I32]),
         "orc_trace_brute": (None, [P, I64, P, I64, U32, P, I32]),
         "orc_pcg_stream": (None, [U64, U64, I32, P]),
@@ -168,6 +170,18 @@ def trace_final(fnodes, tri_ordered, rays, flags=0, threads=None, sobb_kdop=0) -
     lib.orc_trace_final(_p(fn), fn.shape[0], _p(to), to.size // 12, _p(r), r.shape[0], flags, _p(hits),
                         threads or nthreads(), sobb_kdop)
     return hits
+
+
+def trace_final_counted(fnodes, tri_ordered, rays, flags=0, threads=None, sobb_kdop=0):
+    """trace_final plus (inner nodes visited, triangle tests) summed over the rays."""
+    fn = np.ascontiguousarray(fnodes)
+    to = _f32(tri_ordered)
+    r = _f32(rays)
+    hits = np.zeros(r.shape[0], HIT_DTYPE)
+    counts = np.zeros(2, np.int64)
+    lib.orc_trace_final_counted(_p(fn), fn.shape[0], _p(to), to.size // 12, _p(r), r.shape[0], flags, _p(hits),
+                                threads or nthreads(), sobb_kdop, _p(counts))
+    return hits, int(counts[0]), int(counts[1])
 
 
 def sah_final(fnodes, root_box6, sobb_kdop=0) -> float:
@@ -294,6 +308,14 @@ class Ploc:
         lib.orc_ploc_trace(self.h, _p(r), r.shape[0], flags, _p(hits), threads or nthreads())
         return hits
 
+    def trace_counted(self, rays, flags=0, threads=None):
+        """(hits, inner nodes visited, triangle tests) over this tree."""
+        r = _f32(rays)
+        hits = np.zeros(r.shape[0], HIT_DTYPE)
+        counts = np.zeros(2, np.int64)
+        lib.orc_ploc_trace_counted(self.h, _p(r), r.shape[0], flags, _p(hits), threads or nthreads(), _p(counts))
+        return hits, int(counts[0]), int(counts[1])
+
 
 # ---- diffuse path tracing ------------------------------------------------------------------
 def _sig_render():
@@ -348,7 +370,7 @@ def path_grad_fp64(pm, pixels, spp, depth, albedo64, emission64, target):
 def _sig_whitted():
     P, I32, I64, U32, F = C.c_void_p, C.c_int32, C.c_int64, C.c_uint32, C.c_float
     for name, res, args in [
-        ("orc_whitted", None, [P, P, P, I64, P, P, I32, I32, I32, I32, I32, P, P, I32, P, I32]),
+        ("orc_whitted", None, [P, P, P, I64, P, P, I32, I32, I32, I32, I32, P, P, I32, P, I32, P]),
         ("orc_pow_pos", F, [F, F]),
         ("orc_area_light_sample01", F, [U32, U32, U32, U32, U32, U32]),
     ]:
@@ -368,18 +390,28 @@ def area_light_sample01(seed, px, py, light, s, dim) -> float:
     return lib.orc_area_light_sample01(seed, px, py, light, s, dim)
 
 
+WHITTED_COUNTS = ("primary_ray_count", "secondary_ray_count", "shadow_ray_count", "query_count",
+                  "traversal_count", "intersection_count")
+
+
 def whitted(ploc: "Ploc", mat_id, materials, W, H, max_depth, samples, camera, background=(0.0, 0.0, 0.0),
-            cull=False, vertex_normals=None, recursive=False, threads=None):
+            cull=False, vertex_normals=None, recursive=False, threads=None, counts=False):
     """CPU restatement of the reference's Whitted CpuRaytracer over the oracle's PLOC tree
     (watertight hits). recursive=True evaluates the reference's recursion as written;
-    otherwise the weighted depth-first sum the GPU kernel uses. Returns (H, W, 3) float32."""
+    otherwise the weighted depth-first sum the GPU kernel uses. Returns (H, W, 3) float32, and
+    with counts=True also the reference's ray counters and the traversal work as a dict keyed
+    by WHITTED_COUNTS (traversal_count = inner nodes visited, intersection_count = triangle
+    tests)."""
     mats = _f32(materials).reshape(-1, 16)
     mat = np.ascontiguousarray(mat_id, np.int32)
     cam = _f32(camera.as_array() if hasattr(camera, "as_array") else camera)
     bg = _f32(background)
     vn = _f32(vertex_normals) if vertex_normals is not None else None
     img = np.zeros((H, W, 3), np.float32)
+    c = np.zeros(6, np.int64)
     lib.orc_whitted(ploc.h, _p(ploc.tris), _p(vn) if vn is not None else None, ploc.tris.shape[0], _p(mat), _p(mats),
                     W, H, max_depth, samples, 1 if cull else 0, _p(bg), _p(cam), 1 if recursive else 0, _p(img),
-                    threads or nthreads())
+                    threads or nthreads(), _p(c) if counts else None)
+    if counts:
+        return img, dict(zip(WHITTED_COUNTS, (int(v) for v in c)))
     return img

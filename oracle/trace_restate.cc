@@ -115,7 +115,7 @@ static inline bool Better(float t, int32_t prim, float best_t, int32_t best_prim
     return t < best_t || (t == best_t && best_prim >= 0 && prim < best_prim);
 }
 
-void TraceFinalOne(const FinalTree& T, const RayRec& r, uint32_t flags, HitRec& h) {
+void TraceFinalOne(const FinalTree& T, const RayRec& r, uint32_t flags, HitRec& h, TraceCount* count) {
     const bool any = flags & 1u, cull = flags & 2u;
     WtRay w = PrepareRay(r);
     SobbRay sr{};
@@ -128,6 +128,7 @@ void TraceFinalOne(const FinalTree& T, const RayRec& r, uint32_t flags, HitRec& 
     while (sp > 0) {
         int32_t ref = stack[--sp];
         if (ref >= 0) {
+            if (count) count->nodes++;
             const FinalTree::Node& nd = T.nodes[ref];
             float tl, tr;
             bool hl, hr;
@@ -163,6 +164,7 @@ void TraceFinalOne(const FinalTree& T, const RayRec& r, uint32_t flags, HitRec& 
                 int32_t prim;
                 std::memcpy(&prim, &d[3], 4);
                 float t, b1, b2;
+                if (count) count->tris++;
                 if (!IntersectWatertight(w, d, d + 4, d + 8, cull, t, b1, b2)) continue;
                 if (!(t > r.tmin)) continue;
                 if (any) {
@@ -198,6 +200,13 @@ static void ParallelFor(int64_t n, int threads, F f) {
 
 void TraceFinal(const FinalTree& T, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads) {
     ParallelFor(n, threads, [&](int64_t i) { TraceFinalOne(T, rays[i], flags, hits[i]); });
+}
+
+void TraceFinalCounted(const FinalTree& T, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads,
+                       TraceCount& total) {
+    std::vector<TraceCount> per(static_cast<size_t>(n));
+    ParallelFor(n, threads, [&](int64_t i) { TraceFinalOne(T, rays[i], flags, hits[i], &per[size_t(i)]); });
+    for (const TraceCount& c : per) total.nodes += c.nodes, total.tris += c.tris;
 }
 
 void TraceBrute(const float* tri_xyz, int64_t ntri, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits,

@@ -149,10 +149,13 @@ float TriangleArea(const float* t) {  // triangle.h:28-34 (0.5 |e1 x e2|)
     return 0.5f * Length(Cross(v1 - v0, v2 - v0));
 }
 
-bool Closest(const WScene& S, const WRay& r, bool cull, WHit& h) {
+// A closest-hit query (AccelerationStructure::Intersect); st counts it and its traversal.
+bool Closest(const WScene& S, const WRay& r, bool cull, WHit& h, WStats& st) {
     RayRec rr{r.o.x, r.o.y, r.o.z, kEpsilon, r.d.x, r.d.y, r.d.z, r.tmax};
     HitRec hr;
-    TraceFinalOne(*S.tree, rr, cull ? 2u : 0u, hr);
+    TraceCount tc;
+    TraceFinalOne(*S.tree, rr, cull ? 2u : 0u, hr, &tc);
+    st.queries++, st.nodes += tc.nodes, st.tris += tc.tris;
     if (hr.prim < 0) return false;
     const float* t = S.tri + 9 * int64_t(hr.prim);
     h.t = hr.t;
@@ -170,10 +173,12 @@ bool Closest(const WScene& S, const WRay& r, bool cull, WHit& h) {
     return true;
 }
 
-bool AnyHit(const WScene& S, const WRay& r) {
+bool AnyHit(const WScene& S, const WRay& r, WStats& st) {
     RayRec rr{r.o.x, r.o.y, r.o.z, kEpsilon, r.d.x, r.d.y, r.d.z, r.tmax};
     HitRec hr;
-    TraceFinalOne(*S.tree, rr, 1u, hr);
+    TraceCount tc;
+    TraceFinalOne(*S.tree, rr, 1u, hr, &tc);
+    st.queries++, st.nodes += tc.nodes, st.tris += tc.tris;
     return hr.prim >= 0;
 }
 
@@ -182,13 +187,14 @@ Vec3 OffsetRayOrigin(const Vec3& pos, float eps, const Vec3& gn, const Vec3& dir
     return pos + offset * eps;
 }
 
-bool IsShadowed(const WScene& S, const WHit& h, const Vec3& lpos) {
+bool IsShadowed(const WScene& S, const WHit& h, const Vec3& lpos, WStats& st) {
+    st.shadow++;  // counted on entry, as cpu_raytracer.cc:223 does
     const Vec3 to_light = lpos - h.pos;
     const float dist = Length(to_light);
     if (dist < kEpsilon) return false;
     const Vec3 dir = to_light / dist;
     const WRay sr{OffsetRayOrigin(h.pos, h.eps, h.geom_normal, dir), dir, dist * (1.0f - h.eps)};
-    return AnyHit(S, sr);
+    return AnyHit(S, sr, st);
 }
 
 Vec3 Phong(const WScene& S, const WHit& h, const WMaterial& m, const Vec3& lpos, const Vec3& lcol) {
@@ -201,7 +207,7 @@ Vec3 Phong(const WScene& S, const WHit& h, const WMaterial& m, const Vec3& lpos,
     return (I_a + I_d) + I_s;
 }
 
-Vec3 LocalAreaLights(const WScene& S, int x, int y, const WHit& h, const WMaterial& m) {
+Vec3 LocalAreaLights(const WScene& S, int x, int y, const WHit& h, const WMaterial& m, WStats& st) {
     Vec3 color{0.0f, 0.0f, 0.0f};
     const int cnt = S.p.samples;
     for (size_t al = 0; al < S.lights.size(); al++) {
@@ -218,7 +224,7 @@ Vec3 LocalAreaLights(const WScene& S, int x, int y, const WHit& h, const WMateri
             const float v = r1 + r2 > 1.0f ? 1.0f - r2 : r2;
             const Vec3 a = V(ltri), b = V(ltri + 3), c = V(ltri + 6);
             const Vec3 lpos = (a + (b - a) * u) + (c - a) * v;
-            if (IsShadowed(S, h, lpos)) continue;
+            if (IsShadowed(S, h, lpos, st)) continue;
             const Vec3 to_light = lpos - h.pos;
             const float d = Length(to_light);
             const Vec3 d_l = to_light / d;
@@ -261,20 +267,20 @@ WRay PrimaryRay(const WScene& S, const float* basis, int x, int y) {  // cpu_ray
 }
 
 // Depth-first over the ray tree: reflection subtree before refraction subtree.
-Vec3 WhittedPixel(const WScene& S, const float* basis, int x, int y) {
+Vec3 WhittedPixel(const WScene& S, const float* basis, int x, int y, WStats& st) {
     struct Item {
         WRay r;
         Vec3 w;
         int depth;
     };
-    Item st[2 * kWhittedMaxDepth + 2];
+    Item stack[2 * kWhittedMaxDepth + 2];
     int sp = 0;
-    st[sp++] = Item{PrimaryRay(S, basis, x, y), Vec3{1.0f, 1.0f, 1.0f}, 0};
+    stack[sp++] = Item{PrimaryRay(S, basis, x, y), Vec3{1.0f, 1.0f, 1.0f}, 0};
     Vec3 acc{0.0f, 0.0f, 0.0f};
     while (sp > 0) {
-        const Item it = st[--sp];
+        const Item it = stack[--sp];
         WHit h;
-        if (!Closest(S, it.r, S.p.cull != 0, h)) {
+        if (!Closest(S, it.r, S.p.cull != 0, h, st)) {
             acc = acc + it.w * V(S.p.background);
             continue;
         }
@@ -283,28 +289,38 @@ Vec3 WhittedPixel(const WScene& S, const float* basis, int x, int y) {
             acc = acc + it.w * V(m.emission);
             continue;
         }
-        acc = acc + it.w * LocalAreaLights(S, x, y, h, m);
+        acc = acc + it.w * LocalAreaLights(S, x, y, h, m, st);
         if (it.depth < S.p.max_depth) {
-            if (NonZero(m.transmittance))
-                st[sp++] = Item{RefractionRay(it.r, h, m.ior, m.r_ior), it.w * V(m.transmittance), it.depth + 1};
-            if (NonZero(m.specular)) st[sp++] = Item{ReflectionRay(it.r, h), it.w * V(m.specular), it.depth + 1};
+            if (NonZero(m.transmittance)) {
+                stack[sp++] = Item{RefractionRay(it.r, h, m.ior, m.r_ior), it.w * V(m.transmittance), it.depth + 1};
+                st.secondary++;
+            }
+            if (NonZero(m.specular)) {
+                stack[sp++] = Item{ReflectionRay(it.r, h), it.w * V(m.specular), it.depth + 1};
+                st.secondary++;
+            }
         }
     }
     return acc;
 }
 
 // The reference's recursion as written (cpu_raytracer.cc:108-161).
-Vec3 TraceRayRecursive(const WScene& S, int x, int y, const WRay& r, int depth) {
+Vec3 TraceRayRecursive(const WScene& S, int x, int y, const WRay& r, int depth, WStats& st) {
     WHit h;
-    if (!Closest(S, r, S.p.cull != 0, h)) return V(S.p.background);
+    if (!Closest(S, r, S.p.cull != 0, h, st)) return V(S.p.background);
     const WMaterial& m = S.mats[S.mat_id[h.prim]];
     if (NonZero(m.emission)) return V(m.emission);
-    Vec3 color = LocalAreaLights(S, x, y, h, m);
+    Vec3 color = LocalAreaLights(S, x, y, h, m, st);
     if (depth < S.p.max_depth) {
-        if (NonZero(m.specular)) color = color + TraceRayRecursive(S, x, y, ReflectionRay(r, h), depth + 1) * V(m.specular);
-        if (NonZero(m.transmittance))
+        if (NonZero(m.specular)) {
+            st.secondary++;
+            color = color + TraceRayRecursive(S, x, y, ReflectionRay(r, h), depth + 1, st) * V(m.specular);
+        }
+        if (NonZero(m.transmittance)) {
+            st.secondary++;
             color = color +
-                    TraceRayRecursive(S, x, y, RefractionRay(r, h, m.ior, m.r_ior), depth + 1) * V(m.transmittance);
+                    TraceRayRecursive(S, x, y, RefractionRay(r, h, m.ior, m.r_ior), depth + 1, st) * V(m.transmittance);
+        }
     }
     return color;
 }
@@ -312,7 +328,8 @@ Vec3 TraceRayRecursive(const WScene& S, int x, int y, const WRay& r, int depth) 
 }  // namespace
 
 void WhittedRender(const FinalTree& tree, const float* tri, const float* vn, int64_t n, const int32_t* mat_id,
-                   const WMaterial* mats, const WParams& p, bool recursive, float* image, int threads) {
+                   const WMaterial* mats, const WParams& p, bool recursive, float* image, int threads,
+                   WStats* stats) {
     WScene S;
     S.tree = &tree;
     S.tri = tri;
@@ -325,11 +342,21 @@ void WhittedRender(const FinalTree& tree, const float* tri, const float* vn, int
         if (NonZero(mats[mat_id[i]].emission)) S.lights.push_back(int32_t(i));
     float basis[12];
     CameraBasis(p.cam, p.W, p.H, basis);
-    ParallelRows(int64_t(p.W) * p.H, threads, [&](int64_t i) {
+    const int64_t pixels = int64_t(p.W) * p.H;
+    std::vector<WStats> per(stats ? size_t(pixels) : 0);
+    ParallelRows(pixels, threads, [&](int64_t i) {
         const int x = int(i % p.W), y = int(i / p.W);
-        const Vec3 c = recursive ? TraceRayRecursive(S, x, y, PrimaryRay(S, basis, x, y), 0) : WhittedPixel(S, basis, x, y);
+        WStats st;
+        const Vec3 c = recursive ? TraceRayRecursive(S, x, y, PrimaryRay(S, basis, x, y), 0, st)
+                                 : WhittedPixel(S, basis, x, y, st);
+        st.primary++;  // TracePixel (cpu_raytracer.cc:103)
         image[3 * i] = c.x, image[3 * i + 1] = c.y, image[3 * i + 2] = c.z;
+        if (stats) per[size_t(i)] = st;
     });
+    if (stats) {
+        *stats = WStats{};
+        for (const WStats& st : per) stats->Add(st);
+    }
 }
 
 }  // namespace orc

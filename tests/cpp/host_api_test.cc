@@ -25,7 +25,8 @@ extern "C" void* orc_ploc_build(const float* tri, int64_t n, int morton_type, in
 extern "C" void orc_ploc_free(void* h);
 extern "C" void orc_whitted(void* ploc, const float* tri_xyz, const float* vertex_normals, int64_t n,
                             const int32_t* mat_id, const float* mats, int W, int H, int max_depth, int samples, int cull,
-                            const float* background, const float* cam, int recursive, float* image, int threads);
+                            const float* background, const float* cam, int recursive, float* image, int threads,
+                            int64_t* counts);
 
 using namespace diplodocus_amd;
 
@@ -186,7 +187,7 @@ bool WhittedMatchesOracle(const GpuBvh& bvh, const RenderConfig& rc, const Camer
     const float camv[11] = {cam.pos.x, cam.pos.y, cam.pos.z, cam.dir.x, cam.dir.y, cam.dir.z,
                             cam.up.x,  cam.up.y,  cam.up.z,  cam.fov,   cam.far};
     orc_whitted(ploc, xyz.data(), vn.data(), int64_t(tris.size()), mid.data(), mrows.data(), rc.width, rc.height,
-                rc.max_depth, rc.area_light_sample_cnt, rc.backface_culling ? 1 : 0, bg, camv, 0, ref.data(), 8);
+                rc.max_depth, rc.area_light_sample_cnt, rc.backface_culling ? 1 : 0, bg, camv, 0, ref.data(), 8, nullptr);
     orc_ploc_free(ploc);
     size_t diff = 0;
     *sum = 0.0;
@@ -319,7 +320,7 @@ void GpuChecks() {
     const float camv[11] = {cam.pos.x, cam.pos.y, cam.pos.z, cam.dir.x, cam.dir.y, cam.dir.z,
                             cam.up.x,  cam.up.y,  cam.up.z,  cam.fov,   cam.far};
     orc_whitted(ploc, xyz.data(), vn.data(), int64_t(tris.size()), mid.data(), mrows.data(), rc.width, rc.height,
-                rc.max_depth, rc.area_light_sample_cnt, 0, bg, camv, 0, ref.data(), 8);
+                rc.max_depth, rc.area_light_sample_cnt, 0, bg, camv, 0, ref.data(), 8, nullptr);
     orc_ploc_free(ploc);
     size_t diff = 0;
     double wsum = 0.0;
@@ -350,6 +351,20 @@ void GpuChecks() {
             const bool same = WhittedMatchesOracle(static_cast<const GpuBvh&>(*as), src, scene->GetCamera(),
                                                    scene->Triangles(), scene->Materials(), &ssum);
             Expect(same && ssum > 0.0, "loaded CornellBox-Sphere: Whitted bit-identical to the oracle");
+            // the renderer's counters, as the reference's renderer keeps them in Stats
+            Stats rs;
+            const std::vector<float> img2 = GpuWhittedRenderer(static_cast<const GpuBvh&>(*as))
+                                                .Render(src, scene->GetCamera(), scene->Triangles(),
+                                                        scene->Materials(), &rs);
+            const RaytracingStats& r = rs.rt_stats;
+            Expect(r.primary_ray_count == 96 * 96 && r.secondary_ray_count > 0 && r.shadow_ray_count > 0 &&
+                       r.raytracing_time > 0.0,
+                   "whitted stats: ray counters");
+            Expect(rs.accel_stats.query_count == r.primary_ray_count + r.secondary_ray_count + r.shadow_ray_count &&
+                       rs.accel_stats.traversal_count > rs.accel_stats.query_count,
+                   "whitted stats: every traced ray is one query");
+            std::printf("whitted stats: %d primary, %d secondary, %d shadow rays, %.3f ms\n", r.primary_ray_count,
+                        r.secondary_ray_count, r.shadow_ray_count, r.raytracing_time * 1e3);
             std::printf("loaded scene: %zu triangles, %d nodes, whitted image sum %.4f\n", scene->Triangles().size(),
                         st.accel_stats.node_count, ssum);
         }

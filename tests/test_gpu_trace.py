@@ -55,6 +55,24 @@ def hits_equal(a, b):
     return (a.view(np.uint32).reshape(-1, 4) == b.view(np.uint32).reshape(-1, 4)).all(axis=1)
 
 
+@pytest.mark.parametrize("bv,kdop", [(0, 32), (1, 32), (1, 14)])
+@pytest.mark.parametrize("flags", [0, 1, 2])
+def test_trace_counters_match_oracle(oracle, ctx, bv, kdop, flags):
+    """The traversal counters (AccelerationStats::traversal_count / intersection_count as inner
+    nodes visited and triangle tests) equal the oracle's count of the same ordered traversal
+    over the same tree, ray for ray summed."""
+    s = scenes.mixed_2m(n_small=30_000, n_thin=3_000)
+    say(f"trace counters bv={bv} kdop={kdop} flags={flags}")
+    g = gpu_bvh(ctx, s.tris, mt=1, bv=bv, kdop=kdop)
+    rays = scenes.random_rays_towards(s.tris, 6000, seed=7 + flags)
+    h, nodes, tests = g.trace_counters(rays, flags)
+    d = g.download()
+    ref, rn, rt = oracle.trace_final_counted(d["nodes"], d["tri_ordered"], rays, flags, sobb_kdop=kdop if bv else 0)
+    assert (nodes, tests) == (rn, rt)
+    assert hits_equal(h, ref).all()
+    assert tests >= int((h["prim"] >= 0).sum()) and nodes >= len(rays)
+
+
 @pytest.mark.parametrize("scene", ["cornell", "icosphere"])
 @pytest.mark.parametrize("flags", [0, 2])
 def test_trace_matches_brute_force(oracle, ctx, scene, flags):

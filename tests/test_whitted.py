@@ -122,6 +122,29 @@ def test_weighted_sum_matches_the_reference_recursion(oracle, cull):
     np.testing.assert_allclose(a, b, rtol=2e-6, atol=1e-7)
 
 
+def test_ray_counters_closed_forms(oracle):
+    """The reference's counters (cpu_raytracer.cc:103,131,147,223): one primary ray per pixel;
+    at depth 0 no secondary rays and, per pixel whose primary hit is not emissive, one shadow
+    ray per light and sample; every trace is a query; the recursion as written and the weighted
+    depth-first evaluation count the same."""
+    s, mats = scenes.cornell_whitted()
+    p = oracle.Ploc(s.tris, 0, 16, 4)
+    W = H = 48
+    _, c0 = oracle.whitted(p, s.mat_id, mats, W, H, 0, 3, s.camera, counts=True)
+    hits = oracle.trace_brute(s.tris, scenes.camera_rays(s.camera, W, H))["prim"]
+    emissive = mats[:, 9:12].sum(1) > 0
+    lit = int(((hits >= 0) & ~emissive[s.mat_id[np.maximum(hits, 0)]]).sum())
+    assert c0["primary_ray_count"] == W * H and c0["secondary_ray_count"] == 0
+    assert c0["shadow_ray_count"] == lit * 2 * 3
+    # no light sample lies within 1e-6 of a shaded point here, so every shadow ray is traced
+    assert c0["query_count"] == W * H + c0["shadow_ray_count"]
+    _, c4 = oracle.whitted(p, s.mat_id, mats, W, H, 4, 3, s.camera, counts=True)
+    _, r4 = oracle.whitted(p, s.mat_id, mats, W, H, 4, 3, s.camera, counts=True, recursive=True)
+    assert c4 == r4 and c4["secondary_ray_count"] > 0
+    assert c4["query_count"] == c4["primary_ray_count"] + c4["secondary_ray_count"] + c4["shadow_ray_count"]
+    assert c4["traversal_count"] > c0["traversal_count"] and c4["intersection_count"] > c0["intersection_count"]
+
+
 def icosphere_whitted():
     s = scenes.icosphere_room()
     mats = np.stack([scenes.whitted_material(tuple(s.albedo[m]),
@@ -151,6 +174,35 @@ def test_gpu_whitted_cornell_bitexact(oracle, ctx):
     img = render.render_whitted(g, wp, _gpu(s.tris), _gpu(s.mat_id), _gpu(mats), _gpu(lights)).cpu().numpy()
     ref = oracle.whitted(oracle.Ploc(s.tris, 0, 16, 4), s.mat_id, mats, 256, 256, 3, 8, s.camera)
     assert (img.view(np.uint32) == ref.view(np.uint32)).all(), np.abs(img - ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("scene,bv", [("cornell", 0), ("icosphere", 1)])
+def test_gpu_whitted_stats_match_oracle(oracle, ctx, scene, bv):
+    """dpl_render_whitted_stats: the image is the same as without counters, and the ray
+    counters and traversal work equal the oracle's counts exactly."""
+    from diplodocus_amd import accel, render
+
+    if scene == "cornell":
+        s, mats = scenes.cornell_whitted()
+        vn = None
+    else:
+        s, mats, vn = icosphere_whitted()
+    g = accel.GpuBvh(ctx).build(s.tris, accel.make_params(morton_type=1, radius=16, max_leaf=4, bv_type=bv))
+    lights = render.emissive_triangles(s.mat_id, mats)
+    wp = render.make_whitted_params(96, 80, 4, 4, s.camera)
+    args = (_gpu(s.tris), _gpu(s.mat_id), _gpu(mats), _gpu(lights))
+    plain = render.render_whitted(g, wp, *args, vertex_normals=None if vn is None else _gpu(vn)).cpu().numpy()
+    img, st = render.render_whitted(g, wp, *args, vertex_normals=None if vn is None else _gpu(vn), stats=True)
+    img = img.cpu().numpy()
+    assert (img.view(np.uint32) == plain.view(np.uint32)).all()
+    # the same tree as the GPU's (bit-exact build), so the traversal work is comparable
+    ref, rc = oracle.whitted(oracle.Ploc(s.tris, 1, 16, 4, bv_type=bv), s.mat_id, mats, 96, 80, 4, 4, s.camera,
+                             vertex_normals=vn, counts=True)
+    assert (img.view(np.uint32) == ref.view(np.uint32)).all()
+    got = {k: st[k] for k in rc}
+    assert got == rc, (got, rc)
+    assert st["raytracing_ms"] > 0.0
 
 
 @pytest.mark.gpu
