@@ -91,8 +91,11 @@ struct Material {
     float dissolve = 1.0f;
 };
 
-// src/config/render_config.h:17-27 (the fields the Whitted renderer reads)
+enum class RendererType { kCpu, kGpu };
+
+// src/config/render_config.h:17-27
 struct RenderConfig {
+    RendererType renderer_type = RendererType::kCpu;
     int width = 800;
     int height = 600;
     Vec3 background_color;

@@ -1,20 +1,25 @@
 // Drives the C++ host interface the way the reference's renderer drives an
 // AccelerationStructure (src/renderer/cpu_raytracer.cc:56-57,112,236): build through the
 // factory, per-ray Intersect / IntersectAny, plus the batched calls, and the path tracer.
-//   host_api_test --cpu [--scenes DIR]   interface checks that need no GPU
+//   host_api_test --cpu [--scenes DIR] [--configs DIR]   interface checks that need no GPU
 //   host_api_test [--scenes DIR]         the GPU checks; hits are compared with the oracle's
 //                                        brute force, images with the oracle's renderer
-// DIR holds the OBJ scene fixtures (CornellBox-Original / CornellBox-Sphere); the scene
-// checks are skipped without it.
+// The scenes DIR holds the OBJ scene fixtures (CornellBox-Original / CornellBox-Sphere), the
+// configs DIR the JSON configuration fixtures; their checks are skipped without them.
 // Prints one line per check group and "host_api_test: OK" at the end; exits non-zero on the
 // first failure.
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <string>
 #include <vector>
 
+#include <filesystem>
+#include <sstream>
+
+#include "config.h"
 #include "dpl_host.h"
 #include "scene_loader.h"
 
@@ -34,6 +39,8 @@ namespace {
 
 int g_failures = 0;
 std::string g_scene_dir;
+std::string g_config_dir;
+
 
 void Expect(bool ok, const std::string& what) {
     if (!ok) {
@@ -105,6 +112,102 @@ std::vector<Ray> CameraRays(int w, int h) {
     return rays;
 }
 
+// JSON configuration (the reference's keys and defaults) and the stats exporters.
+void ConfigChecks() {
+    JsonConfigLoader loader;
+    if (!g_config_dir.empty()) {
+        // the reference's own sample configuration (sample_config.json)
+        const auto c = loader.Load(std::filesystem::path(g_config_dir) / "sample_config.json");
+        Expect(c.has_value(), "config: sample_config.json loads: " + JsonConfigLoader::LastError());
+        if (c) {
+            const RenderConfig& r = c->render_config;
+            Expect(r.renderer_type == RendererType::kCpu && r.width == 512 && r.height == 512 && r.max_depth == 8 &&
+                       r.area_light_sample_cnt == 8 && r.background_color == Vec3{} && r.pixel_sample_cnt == 4 &&
+                       r.seed == 42 && !r.backface_culling,
+                   "config: render section and defaults");
+            Expect(c->acceleration_structure_config.acceleration_structure_type == AccelerationStructureType::kSbvh &&
+                       c->acceleration_structure_config.nn_search_radius == 10,
+                   "config: acceleration structure section and defaults");
+            Expect(c->scene_load_config.name == "CornellBox-Bunny" && c->scene_load_config.dirpath == "res",
+                   "config: scene section");
+            Expect(c->image_export_config.filepath == "output.ppm" &&
+                       c->stats_export_config.stats_export_format == StatsExportFormat::kConsole,
+                   "config: image and stats sections");
+        }
+        const auto f = loader.Load(std::filesystem::path(g_config_dir) / "full_config.json");
+        Expect(f.has_value(), "config: full_config.json loads: " + JsonConfigLoader::LastError());
+        if (f) {
+            const RenderConfig& r = f->render_config;
+            Expect(r.renderer_type == RendererType::kGpu && r.width == 1024 && r.height == 768 &&
+                       r.background_color == Vec3{0.25f, 0.5f, 0.1f} && r.backface_culling && r.max_depth == 6 &&
+                       r.area_light_sample_cnt == 16 && r.pixel_sample_cnt == 4 && r.seed == 7,
+                   "config: numbers convert, null keeps the default, Vec3 reads three numbers");
+            const AccelerationStructureConfig& a = f->acceleration_structure_config;
+            Expect(a.acceleration_structure_type == AccelerationStructureType::kPlocEmcSobb && a.max_depth == 48 &&
+                       a.max_triangles_per_leaf == 8 && a.nn_search_radius == 32 && a.kdop_size == 18 &&
+                       a.bin_count == 32,
+                   "config: acceleration keys (a repeated key keeps its last value)");
+            Expect(f->scene_load_config.dirpath == "scenes/obj" && !f->scene_load_config.triangulate &&
+                       f->stats_export_config.stats_export_format == StatsExportFormat::kCsv &&
+                       f->stats_export_config.filepath == "stats.csv" &&
+                       f->image_export_config.filepath == "renders/cornell.ppm",
+                   "config: strings with escapes, enums, paths");
+        }
+    }
+    Expect(!loader.Load("/nonexistent/config.json").has_value(), "config: a missing file fails");
+    const char* bad[] = {"", "{", "{\"render\": {\"width\": 01}}", "{\"render\": {\"width\": \"800\"}}",
+                         "{\"render\": {\"renderer_type\": \"vulkan\"}}", "{\"render\": {}} x",
+                         "{\"render\": {\"backface_culling\": 1}}", "{\"render\": {\"background_color\": [1, 2]}}",
+                         "{\"a\": [1, 2,]}", "// c\n{}", "{\"s\": \"\\ud800\"}"};
+    for (const char* b : bad) Expect(!loader.LoadFromString(b).has_value(), std::string("config: rejects ") + b);
+    const auto d = loader.LoadFromString("[1, 2, 3]");
+    Expect(d.has_value() && d->render_config.width == 800, "config: a non-object document keeps every default");
+    const auto e = loader.LoadFromString("{\"render\": 5, \"acceleration_structure\": {\"kdop_size\": 14.0}}");
+    Expect(e.has_value() && e->render_config.width == 800 && e->acceleration_structure_config.kdop_size == 14,
+           "config: a non-object section keeps its defaults");
+
+    // stats exporters
+    Stats st;
+    st.rt_stats = RaytracingStats{1.25, 262144, 1000, 2000000};
+    st.accel_stats = AccelerationStats{4.5, 2047, 1023, 1024, 65504, 100, 450, 1234};
+    std::ostringstream con;
+    CreateStatsExporter(StatsExportConfig{}, &con)->ExportOne(StatsExportConfig{}, st);
+    const std::string want =
+        "########################################\n#                                      #\n"
+        "#              Statistics              #\n#                                      #\n"
+        "########################################\n\n"
+        "----------------------------------------\n               Ray tracing              \n"
+        "----------------------------------------\n"
+        " Ray tracing time:              1.25 s\n Primary ray count:             262144\n"
+        " Secondary ray count:           1000\n Shadow ray count:              2000000\n\n"
+        "----------------------------------------\n               Acceleration             \n"
+        "----------------------------------------\n# Construction:                         \n"
+        " Build time:                    4.5 ms\n Node count:                    2047\n"
+        " Inner node count:              1023\n Leaf node count:               1024\n"
+        " Memory consumption:            65.5 KB\n\n# Traversal:                             \n"
+        " Query count:                   100\n Intersection count:            450\n"
+        " Intersections per query (avg): 4.5\n Traversal count:               1234\n"
+        " Traversals per query:          12.3\n\n########################################\n";
+    Expect(con.str() == want, "stats: console report");
+    std::ostringstream csv;
+    StatsExportConfig sc;
+    sc.stats_export_format = StatsExportFormat::kCsv;
+    CreateStatsExporter(sc, &csv)->ExportOne(sc, st);
+    Expect(csv.str() == CsvStatsExporter::Header() + "\n1.25,262144,1000,2000000,4.5,2047,1023,1024,65504,100,450,1234\n",
+           "stats: CSV header and row");
+    const std::filesystem::path tmp = std::filesystem::temp_directory_path() / "dpl_host_api_test_stats.csv";
+    std::filesystem::remove(tmp);
+    sc.filepath = tmp;
+    CsvStatsExporter().ExportOne(sc, st);
+    CsvStatsExporter().ExportOne(sc, st);
+    std::ifstream in(tmp);
+    std::string l0, l1, l2, l3;
+    std::getline(in, l0), std::getline(in, l1), std::getline(in, l2);
+    Expect(l0 == CsvStatsExporter::Header() && l1 == l2 && l1 == CsvStatsExporter::Row(st) && !std::getline(in, l3),
+           "stats: CSV file gets one header and a row per export");
+    std::filesystem::remove(tmp);
+}
+
 void CpuChecks() {
     Expect(ParseAccelerationStructureType("ploc_emc_sobb") == AccelerationStructureType::kPlocEmcSobb, "parse");
     Expect(ParseAccelerationStructureType("sbvh") == AccelerationStructureType::kSbvh, "parse sbvh");
@@ -157,6 +260,7 @@ void CpuChecks() {
                "scene loader: a missing scene fails with a reason");
         Expect(ParseSceneLoadFormat("obj") == SceneLoadFormat::kObj, "scene load format");
     }
+    ConfigChecks();
     std::printf("cpu checks: %s\n", g_failures ? "failed" : "ok");
 }
 
@@ -379,6 +483,7 @@ int main(int argc, char** argv) {
         const std::string a = argv[i];
         if (a == "--cpu") cpu_only = true;
         if (a == "--scenes" && i + 1 < argc) g_scene_dir = argv[++i];
+        if (a == "--configs" && i + 1 < argc) g_config_dir = argv[++i];
     }
     CpuChecks();
     if (!cpu_only) {
