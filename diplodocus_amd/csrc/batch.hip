@@ -331,6 +331,10 @@ __global__ __launch_bounds__(kBT) void KBatchBuild(BatchArgs A) {
         const bool bad = S.bad != 0;
         if (A.morton_type == 1 && threadIdx.x == 0) MakeEmcLayout(S.bounds, S.emc);
         __syncthreads();
+        if (A.morton_type == 1) {
+            BuildEmcTables(S.emc, threadIdx.x, kBT);
+            __syncthreads();
+        }
         const CodeFrame f = MakeCodeFrame(S.bounds, A.morton_type);
         for (int64_t i = threadIdx.x; i < n; i += kBT) {
             float t[9];

@@ -124,6 +124,8 @@ __global__ __launch_bounds__(256) void KCodes(const float* __restrict__ tri, int
     if (kMorton == 1) {
         if (threadIdx.x == 0) MakeEmcLayout(f.d, sL);
         __syncthreads();
+        BuildEmcTables(sL, threadIdx.x, blockDim.x);
+        __syncthreads();
     }
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x) {
         float v[9];
@@ -210,8 +212,14 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
     }
     int ipt = 1;
     while (ipt < kIpt && c >= int64_t(BLOCK) * 2 * ipt * min_tiles) ipt *= 2;
-    const int chunk = BLOCK * ipt;
+    // BLOCK * ipt nearest neighbours per tile (its clusters and R on either side): every
+    // thread computes exactly ipt of them
+    const int chunk = BLOCK * ipt - 2 * R;
     const int T = (c + chunk - 1) / chunk;
+    // the next iteration has at most Tn tiles (fewer clusters, chunks of at least
+    // BLOCK - 2 R): each tile clears kc of its look-back states
+    const int Tn = (c + (BLOCK - 2 * R) - 1) / (BLOCK - 2 * R);
+    const int kc = (Tn + T - 1) / T;
     // The grid is sized from a count the host read some iterations ago; surplus workgroups
     // leave before touching the (device-scope, cross-XCD) tile counter.
     if (int(blockIdx.x) >= T) return;
@@ -225,8 +233,7 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
         __syncthreads();
         const int tile = s_tile;
         if (tile >= T) break;
-        // the next iteration has at most ceil(c / BLOCK) tiles: clear ipt states per tile
-        if (threadIdx.x < ipt) status_next[tile * ipt + threadIdx.x] = 0ull;
+        if (int(threadIdx.x) < kc && tile * kc + int(threadIdx.x) < Tn) status_next[tile * kc + threadIdx.x] = 0ull;
         if (threadIdx.x == 0 && tile == 0) b.tile_counter[(iter + 1) & 1] = 0u;
         const int a = tile * chunk;
         const int e = a + chunk < c ? a + chunk : c;
@@ -328,12 +335,19 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
             }
             out[spos++] = o;
         }
+        // with at least as many workgroups as tiles, the workgroups' first fetches hand out
+        // every tile: leave without another round trip to the counter
+        if (T <= int(gridDim.x)) break;
         __syncthreads();
     }
 }
 
 int PlocChunk(int block) { return block * kIpt; }
-int64_t PlocStatusWords(int64_t n, int block) { return (n + block - 1) / block + kIpt; }
+// tiles hold at least block - 2 kMaxRadius clusters
+int64_t PlocStatusWords(int64_t n, int block) {
+    const int64_t m = block - 2 * kMaxRadius;
+    return (n + m - 1) / m + 1;
+}
 int PlocTail() { return kTail; }
 
 hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int tail, int max_leaf, int grid,
@@ -413,7 +427,9 @@ __global__ __launch_bounds__(256) void KEmit(BuildBuffers b) {
 }
 
 // SOBB trees: the child slabs of every final inner node, one thread per child.
-__global__ __launch_bounds__(256) void KFitSobb(BuildBuffers b) {
+// At least four waves per SIMD: the unrolled 26-DOP triple scan would otherwise be
+// scheduled for instruction-level parallelism at the cost of every register.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void KFitSobb(BuildBuffers b) {
     const int32_t K = b.counters[0];
     __shared__ int4 s_tri[kMaxTriples];
     __shared__ float s_w[kMaxDirs * 256];

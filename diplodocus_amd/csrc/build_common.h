@@ -7,6 +7,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "dpl_internal.h"
 
 namespace dpl {
@@ -66,9 +68,15 @@ __device__ __forceinline__ DecodedBounds Decode(const uint32_t* b) {
 // codes
 // ---------------------------------------------------------------------------------------
 struct EmcLayoutDev {
+    // deposit tables: the code bits that nibble j (from the top) of axis a's left-aligned
+    // 24-bit quantised coordinate sets, for each of its 16 values; the same for the three
+    // nibbles of the 12 size bits. A code is the OR of 21 table entries.
+    uint64_t dep[3][6][16];
+    uint64_t dep_size[3][16];
     int nbits[3];
-    int8_t axis_of[48];
     float scale[3];
+    int8_t axis_of[48];
+    int8_t pos[3][24];  // code bit (0 = least significant) of axis a's j-th bit from the top
 };
 
 // Same allocation as the oracle's MakeEmcLayout: each next spatial bit goes to the axis
@@ -89,11 +97,36 @@ __device__ inline void MakeEmcLayout(const DecodedBounds& d, EmcLayoutDev& L) {
         }
         L.axis_of[t] = int8_t(best);
         if (best >= 0) {
+            // spatial bit t sits in group t / 4 (five code bits from bit 59 down: the size bit,
+            // then four spatial bits) at place t % 4
+            L.pos[best][L.nbits[best]] = int8_t(58 - 5 * (t / 4) - (t % 4));
             L.nbits[best]++;
             cell[best] = cell[best] * 0.5f;
         }
     }
     for (int a = 0; a < 3; a++) L.scale[a] = ext[a] > 0.0f ? float(1u << L.nbits[a]) / ext[a] : 0.0f;
+}
+
+// The deposit tables of L (after MakeEmcLayout), built by threads tid = 0 .. nthreads - 1.
+__device__ inline void BuildEmcTables(EmcLayoutDev& L, int tid, int nthreads) {
+    for (int e = tid; e < 3 * 6 * 16 + 3 * 16; e += nthreads) {
+        uint64_t v = 0;
+        if (e < 3 * 6 * 16) {
+            const int a = e / 96, j = (e / 16) % 6, x = e % 16;
+            for (int i = 0; i < 4; i++) {
+                const int bit = 4 * j + i;  // the axis's bit, counted from its top bit
+                if (bit < L.nbits[a] && ((x >> (3 - i)) & 1)) v |= 1ull << L.pos[a][bit];
+            }
+            L.dep[a][j][x] = v;
+        } else {
+            const int k = e - 3 * 6 * 16, s = k / 16, x = k % 16;
+            for (int i = 0; i < 4; i++) {
+                const int g = 4 * s + i;  // size bit 11 - g goes to code bit 59 - 5 g
+                if ((x >> (3 - i)) & 1) v |= 1ull << (59 - 5 * g);
+            }
+            L.dep_size[s][x] = v;
+        }
+    }
 }
 
 // SAH collapse data of a node (Ct = Ci = 1): it becomes a leaf when it holds at most
@@ -187,14 +220,30 @@ __device__ __forceinline__ void LeafKdop(const BuildBuffers& b, const float v[9]
     }
 }
 
+// All 2 W4 loads are issued before the first store: the output may alias nothing the loads
+// read, but the compiler cannot know that, and interleaving would make every float4 a
+// dependent round trip to memory.
+template <int W4>
+__device__ __forceinline__ void UnionKdopT(const BuildBuffers& b, int32_t id, int32_t l, int32_t r) {
+    const float4* kl = reinterpret_cast<const float4*>(b.kdop) + int64_t(l) * W4;
+    const float4* kr = reinterpret_cast<const float4*>(b.kdop) + int64_t(r) * W4;
+    float4 x[W4], y[W4];
+#pragma unroll
+    for (int k = 0; k < W4; k++) x[k] = kl[k];
+#pragma unroll
+    for (int k = 0; k < W4; k++) y[k] = kr[k];
+    float4* ko = reinterpret_cast<float4*>(b.kdop) + int64_t(id) * W4;
+#pragma unroll
+    for (int k = 0; k < W4; k++)
+        ko[k] = make_float4(Min2(x[k].x, y[k].x), Min2(x[k].y, y[k].y), Min2(x[k].z, y[k].z), Min2(x[k].w, y[k].w));
+}
+
 __device__ __forceinline__ void UnionKdop(const BuildBuffers& b, int32_t id, int32_t l, int32_t r) {
-    const int w4 = KdopWidth(b.ndirs) / 4;
-    const float4* kl = reinterpret_cast<const float4*>(b.kdop) + int64_t(l) * w4;
-    const float4* kr = reinterpret_cast<const float4*>(b.kdop) + int64_t(r) * w4;
-    float4* ko = reinterpret_cast<float4*>(b.kdop) + int64_t(id) * w4;
-    for (int k = 0; k < w4; k++) {
-        const float4 x = kl[k], y = kr[k];
-        ko[k] = make_float4(Min2(x.x, y.x), Min2(x.y, y.y), Min2(x.z, y.z), Min2(x.w, y.w));
+    switch (KdopWidth(b.ndirs) / 4) {
+        case 5: UnionKdopT<5>(b, id, l, r); break;  // 13 directions
+        case 3: UnionKdopT<3>(b, id, l, r); break;  // 9
+        case 2: UnionKdopT<2>(b, id, l, r); break;  // 7
+        default: break;
     }
 }
 
@@ -234,37 +283,28 @@ constexpr TripleTable13 MakeTripleTable13() {
 }
 constexpr TripleTable13 kTriples13 = MakeTripleTable13();
 
-// The table in constant memory: the scan reads it with uniform 16-byte scalar loads.
-__constant__ const TripleTable13 c_triples13 = kTriples13;
-
-// wl: this thread's widths in LDS, width k at wl[k * stride].
-__device__ __forceinline__ void ScanTriplesUniform(const float* wl, int stride, float& best, int& bt) {
-    // triples come in lexicographic order: w_a and w_a w_b change only when a or b does
-    int pa = -1, pb = -1;
-    float wa = 0.0f, wb = 0.0f, wab = 0.0f;
-#pragma unroll 4
-    for (int t = 0; t < kTriples13.n; t++) {
-        const Triple13 T = c_triples13.t[t];
-        if (T.a != pa) {
-            pa = T.a;
-            wa = wl[T.a * stride];
-            pb = -1;
-        }
-        if (T.b != pb) {
-            pb = T.b;
-            wb = wl[T.b * stride];
-            wab = wa * wb;
-        }
-        const float wc = wl[T.c * stride];
-        const float sa = (wab + wb * wc + wa * wc) * T.inv_det;
-        if (t == 0 || sa < best) {
-            best = sa;
-            bt = t;
-        }
+// The 26-DOP triple scan unrolled at compile time: each step is the triple's area estimate
+// (w_a w_b + w_b w_c + w_a w_c) / |det| from widths held in registers, the first minimum
+// kept (strictly smaller wins); the result is the triple's a | b << 4 | c << 8.
+template <int T>
+__device__ __forceinline__ void ScanStep13(const float (&w)[13], float& best, uint32_t& abc) {
+    constexpr Triple13 tr = kTriples13.t[T];
+    constexpr uint32_t code = uint32_t(tr.a) | (uint32_t(tr.b) << 4) | (uint32_t(tr.c) << 8);
+    const float wab = w[tr.a] * w[tr.b];
+    const float sa = ((wab + w[tr.b] * w[tr.c]) + w[tr.a] * w[tr.c]) * tr.inv_det;
+    if (T == 0 || sa < best) {
+        best = sa;
+        abc = code;
     }
 }
 
-
+template <int... T>
+__device__ __forceinline__ uint32_t ScanTriples13(const float (&w)[13], std::integer_sequence<int, T...>) {
+    float best = 0.0f;
+    uint32_t abc = 0;
+    (ScanStep13<T>(w, best, abc), ...);
+    return abc;
+}
 
 // Writes the SOBB (lo_a hi_a lo_b hi_b lo_c hi_c) of node `id` to f and returns a | b<<4 | c<<8.
 // ND is the size of the direction set (3, 7, 9 or 13).
@@ -273,8 +313,11 @@ __device__ __forceinline__ void ScanTriplesUniform(const float* wl, int stride, 
 __device__ __forceinline__ void SlabOf(const BuildBuffers& b, const PNode& nd, int32_t id, int k, float pad, float& lo,
                                        float& hi) {
     if (k < 3) {
-        const float l = k == 0 ? nd.lo[0] : (k == 1 ? nd.lo[1] : nd.lo[2]);
-        const float h = k == 0 ? nd.hi[0] : (k == 1 ? nd.hi[1] : nd.hi[2]);
+        // selects between named values: an index into the node's arrays would put them in
+        // scratch memory
+        const float l0 = nd.lo[0], l1 = nd.lo[1], l2 = nd.lo[2], h0 = nd.hi[0], h1 = nd.hi[1], h2 = nd.hi[2];
+        const float l = k == 0 ? l0 : (k == 1 ? l1 : l2);
+        const float h = k == 0 ? h0 : (k == 1 ? h1 : h2);
         lo = l - pad;
         hi = h + pad;
     } else {
@@ -294,6 +337,43 @@ __device__ __forceinline__ uint32_t FitSobbT(const BuildBuffers& b, const int4* 
                                              float* f, float* wl, int stride) {
     constexpr int NX = ND - 3;
     const PNode nd = b.nodes[id];
+    if constexpr (ND == 13) {
+        // widths and slabs in registers: [lo_3..lo_12, -hi_3..-hi_12] after the three axes
+        const float4* kd = reinterpret_cast<const float4*>(b.kdop) + int64_t(id) * (NX / 2);
+        float s[2 * NX];
+#pragma unroll
+        for (int q = 0; q < NX / 2; q++) {
+            const float4 v = kd[q];
+            s[4 * q] = v.x, s[4 * q + 1] = v.y, s[4 * q + 2] = v.z, s[4 * q + 3] = v.w;
+        }
+        float lo[13], hi[13], w[13];
+#pragma unroll
+        for (int a = 0; a < 3; a++) {
+            lo[a] = nd.lo[a] - pad;
+            hi[a] = nd.hi[a] + pad;
+            w[a] = hi[a] - lo[a];
+        }
+#pragma unroll
+        for (int k = 0; k < NX; k++) {
+            lo[3 + k] = s[k];
+            hi[3 + k] = -s[NX + k];
+            w[3 + k] = -s[NX + k] - s[k];
+        }
+        const uint32_t abc = ScanTriples13(w, std::make_integer_sequence<int, kTriples13.n>{});
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            const uint32_t k = (abc >> (4 * q)) & 15u;
+            float l = lo[0], h = hi[0];
+#pragma unroll
+            for (int d = 1; d < 13; d++) {
+                l = k == uint32_t(d) ? lo[d] : l;
+                h = k == uint32_t(d) ? hi[d] : h;
+            }
+            f[2 * q] = l;
+            f[2 * q + 1] = h;
+        }
+        return abc;
+    }
 #pragma unroll
     for (int a = 0; a < 3; a++) wl[a * stride] = (nd.hi[a] + pad) - (nd.lo[a] - pad);
     if constexpr (NX > 0) {
@@ -309,17 +389,13 @@ __device__ __forceinline__ uint32_t FitSobbT(const BuildBuffers& b, const int4* 
     }
     float best = 0.0f;
     int bt = 0;
-    if constexpr (ND == 13) {
-        ScanTriplesUniform(wl, stride, best, bt);
-    } else {
-        for (int t = 0; t < b.ntriples; t++) {
-            const int4 T = triples[t];
-            const float wa = wl[T.x * stride], wb = wl[T.y * stride], wc = wl[T.z * stride];
-            const float sa = (wa * wb + wb * wc + wa * wc) * __int_as_float(T.w);
-            if (t == 0 || sa < best) {
-                best = sa;
-                bt = t;
-            }
+    for (int t = 0; t < b.ntriples; t++) {
+        const int4 T = triples[t];
+        const float wa = wl[T.x * stride], wb = wl[T.y * stride], wc = wl[T.z * stride];
+        const float sa = (wa * wb + wb * wc + wa * wc) * __int_as_float(T.w);
+        if (t == 0 || sa < best) {
+            best = sa;
+            bt = t;
         }
     }
     const int4 T = triples[bt];
@@ -352,37 +428,95 @@ struct WindowLds {
     float* hz;
 };
 
-__device__ __forceinline__ int32_t NearestNeighbour(const WindowLds& w, int i, int base, int c, int R) {
+// key(j) < key(k) for the nearest-neighbour order (area, PairHash(i, .), index); the hashes
+// are computed only on an exact area tie.
+__device__ __forceinline__ bool NnBefore(int i, float dj, int32_t j, float dk, int32_t k) {
+    if (dj != dk) return dj < dk;
+    const uint32_t hj = PairHash(i, j), hk = PairHash(i, k);
+    return hj < hk || (hj == hk && j < k);
+}
+
+// SurfaceAreaUnion for the nearest-neighbour search, with the IEEE minimum / maximum
+// instructions: on finite boxes they pick the same values as the select forms except for the
+// sign of a zero, which changes at most the sign of a zero area, and the search only
+// compares areas (where -0 == +0).
+__device__ __forceinline__ float NnArea(float alx, float aly, float alz, float ahx, float ahy, float ahz, const WindowLds& w,
+                                        int jj) {
+    constexpr float kMax = 3.40282347e38f;
+    const float sx = __builtin_elementwise_minimum(
+        __builtin_elementwise_maximum(ahx, w.hx[jj]) - __builtin_elementwise_minimum(alx, w.lx[jj]), kMax);
+    const float sy = __builtin_elementwise_minimum(
+        __builtin_elementwise_maximum(ahy, w.hy[jj]) - __builtin_elementwise_minimum(aly, w.ly[jj]), kMax);
+    const float sz = __builtin_elementwise_minimum(
+        __builtin_elementwise_maximum(ahz, w.hz[jj]) - __builtin_elementwise_minimum(alz, w.lz[jj]), kMax);
+    return 2.0f * (sx * sy + sy * sz + sx * sz);
+}
+
+// One candidate of a chain: the chain's best (bd, bj) moves to (d, j) when key(j) is smaller
+// (within a chain j grows, so an equal area is taken only for a smaller hash).
+__device__ __forceinline__ void NnOffer(int i, int j, float d, bool valid, float& bd, int32_t& bj) {
+    bool take = valid && d < bd;
+    if (valid && d == bd) take = bj < 0 || PairHash(i, j) < PairHash(i, bj);
+    bd = take ? d : bd;
+    bj = take ? j : bj;
+}
+
+// The part of the nearest-neighbour search of cluster i over candidates j = jlo + part,
+// jlo + part + step, ... (0 < |j - i| <= R): the minimum of key (SA(box_i U box_j),
+// PairHash(i, j), j) in (bd, bj), bj = -1 when there is none. Four independent chains keep
+// four running minima merged at the end: the key is a strict total order, so the minimum of
+// the chain minima is the minimum, and the chains' LDS loads and arithmetic overlap.
+__device__ __forceinline__ void NnScan(const WindowLds& w, int i, int base, int c, int R, int part, int step, float& bd,
+                                       int32_t& bj) {
     const int ii = i - base;
     const float alx = w.lx[ii], aly = w.ly[ii], alz = w.lz[ii], ahx = w.hx[ii], ahy = w.hy[ii], ahz = w.hz[ii];
     const int jlo = i - R < 0 ? 0 : i - R;
     const int jhi = i + R > c - 1 ? c - 1 : i + R;
-    float best_d = 0.0f;
-    int32_t best_j = -1;
-    uint32_t best_h = 0;
-    bool hv = false;
-    for (int j = jlo; j <= jhi; j++) {
-        if (j == i) continue;
-        const int jj = j - base;
-        float d = SurfaceAreaUnion(alx, aly, alz, ahx, ahy, ahz, w.lx[jj], w.ly[jj], w.lz[jj], w.hx[jj], w.hy[jj],
-                                   w.hz[jj]);
-        if (best_j < 0 || d < best_d) {
-            best_d = d;
-            best_j = j;
-            hv = false;
-        } else if (d == best_d) {
-            if (!hv) {
-                best_h = PairHash(i, best_j);
-                hv = true;
-            }
-            uint32_t h = PairHash(i, j);
-            if (h < best_h || (h == best_h && j < best_j)) {
-                best_j = j;
-                best_h = h;
-            }
-        }
+    constexpr float kInf = __builtin_huge_valf();
+    float bd0 = kInf, bd1 = kInf, bd2 = kInf, bd3 = kInf;
+    int32_t bj0 = -1, bj1 = -1, bj2 = -1, bj3 = -1;
+    for (int j0 = jlo + part; j0 <= jhi; j0 += 4 * step) {
+        const int j1 = j0 + step, j2 = j0 + 2 * step, j3 = j0 + 3 * step;
+        // past the end: re-read the last candidate (in the window) and discard it
+        const float d0 = NnArea(alx, aly, alz, ahx, ahy, ahz, w, j0 - base);
+        const float d1 = NnArea(alx, aly, alz, ahx, ahy, ahz, w, (j1 <= jhi ? j1 : jhi) - base);
+        const float d2 = NnArea(alx, aly, alz, ahx, ahy, ahz, w, (j2 <= jhi ? j2 : jhi) - base);
+        const float d3 = NnArea(alx, aly, alz, ahx, ahy, ahz, w, (j3 <= jhi ? j3 : jhi) - base);
+        NnOffer(i, j0, d0, j0 != i, bd0, bj0);
+        NnOffer(i, j1, d1, j1 <= jhi && j1 != i, bd1, bj1);
+        NnOffer(i, j2, d2, j2 <= jhi && j2 != i, bd2, bj2);
+        NnOffer(i, j3, d3, j3 <= jhi && j3 != i, bd3, bj3);
     }
-    return best_j;
+    bd = bd0;
+    bj = bj0;
+    if (bj1 >= 0 && (bj < 0 || NnBefore(i, bd1, bj1, bd, bj))) bd = bd1, bj = bj1;
+    if (bj2 >= 0 && (bj < 0 || NnBefore(i, bd2, bj2, bd, bj))) bd = bd2, bj = bj2;
+    if (bj3 >= 0 && (bj < 0 || NnBefore(i, bd3, bj3, bd, bj))) bd = bd3, bj = bj3;
+}
+
+// argmin over 0 < |j - i| <= R of (SA(box_i U box_j), PairHash(i, j), j).
+__device__ __forceinline__ int32_t NearestNeighbour(const WindowLds& w, int i, int base, int c, int R) {
+    float bd;
+    int32_t bj;
+    NnScan(w, i, base, c, R, 0, 1, bd, bj);
+    return bj;
+}
+
+// The same search split over `lanes` adjacent lanes of a wave (1, 2 or 4; lane part =
+// lane % lanes scans every lanes-th candidate), combined by shuffles: every lane of the
+// group returns the neighbour. Lanes with active = false take part in the shuffles only.
+__device__ __forceinline__ int32_t NearestNeighbourLanes(const WindowLds& w, int i, int base, int c, int R, int lanes,
+                                                         bool active) {
+    const int part = int(threadIdx.x) % lanes;
+    float bd = 0.0f;
+    int32_t bj = -1;
+    if (active) NnScan(w, i, base, c, R, part, lanes, bd, bj);
+    for (int m = 1; m < lanes; m <<= 1) {
+        const float od = __shfl_xor(bd, m, kWave);
+        const int32_t oj = __shfl_xor(bj, m, kWave);
+        if (oj >= 0 && (bj < 0 || NnBefore(i, od, oj, bd, bj))) bd = od, bj = oj;
+    }
+    return bj;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -468,20 +602,14 @@ __device__ __forceinline__ uint64_t CodeOf(const float v[9], const CodeFrame& f,
         sz = (bits - 0x2B800000u) >> 17;
         if (sz > 4095u) sz = 4095u;
     }
-    uint64_t code = 0;
-    for (int g = 0; g < 12; g++) {
-        code = (code << 1) | ((sz >> (11 - g)) & 1u);
+    // 12 groups of [size bit, four spatial bits]: OR of the deposit-table entries of every
+    // nibble (the tables place each bit where MakeEmcLayout's allocation puts it)
+    uint64_t code = L.dep_size[0][(sz >> 8) & 15u] | L.dep_size[1][(sz >> 4) & 15u] | L.dep_size[2][sz & 15u];
 #pragma unroll
-        for (int k = 0; k < 4; k++) {
-            const int a = L.axis_of[4 * g + k];
-            uint64_t bit = 0;
-            if (a >= 0) {
-                const int r0 = a == 0 ? --rem[0] : (a == 1 ? --rem[1] : --rem[2]);
-                const uint32_t qa = a == 0 ? q[0] : (a == 1 ? q[1] : q[2]);
-                bit = (qa >> r0) & 1u;
-            }
-            code = (code << 1) | bit;
-        }
+    for (int a = 0; a < 3; a++) {
+        const uint32_t top = q[a] << (24 - rem[a]);  // left-aligned in 24 bits (q = 0 when rem = 0)
+#pragma unroll
+        for (int j = 0; j < 6; j++) code |= L.dep[a][j][(top >> (20 - 4 * j)) & 15u];
     }
     return code;
 }
@@ -677,7 +805,14 @@ __device__ int TailIterations(const BuildBuffers& b, const PNode* in, int c, int
     int it = 0;
     while (c > 1) {
         if (ccount_log && threadIdx.x == 0) ccount_log[it] = c;
-        for (int i = threadIdx.x; i < c; i += THREADS) L.nn[i] = NearestNeighbour(w, i, 0, c, R);
+        if (c > THREADS / 2) {
+            for (int i = threadIdx.x; i < c; i += THREADS) L.nn[i] = NearestNeighbour(w, i, 0, c, R);
+        } else {  // spare lanes: two or four per cluster
+            const int lanes = c > THREADS / 4 ? 2 : 4;
+            const int i = int(threadIdx.x) / lanes;
+            const int32_t j = NearestNeighbourLanes(w, i < c ? i : 0, 0, c, R, lanes, i < c);
+            if (i < c && int(threadIdx.x) % lanes == 0) L.nn[i] = j;
+        }
         __syncthreads();
         uint32_t sflags = 0, mflags = 0;
         int32_t nnj[IPT];

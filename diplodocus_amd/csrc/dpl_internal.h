@@ -70,11 +70,14 @@ __device__ __forceinline__ uint32_t PairHash(int32_t i, int32_t j) {
     return HashU32(lo * 0x9E3779B1u ^ hi);
 }
 
+// A size that overflows is clamped to the largest float: the area is never NaN, so the
+// nearest-neighbour keys (area, pair hash, index) are totally ordered.
 __device__ __forceinline__ float SurfaceAreaUnion(float alx, float aly, float alz, float ahx, float ahy, float ahz,
                                                   float blx, float bly, float blz, float bhx, float bhy, float bhz) {
-    float sx = Max2(ahx, bhx) - Min2(alx, blx);
-    float sy = Max2(ahy, bhy) - Min2(aly, bly);
-    float sz = Max2(ahz, bhz) - Min2(alz, blz);
+    constexpr float kMax = 3.40282347e38f;
+    float sx = Min2(Max2(ahx, bhx) - Min2(alx, blx), kMax);
+    float sy = Min2(Max2(ahy, bhy) - Min2(aly, bly), kMax);
+    float sz = Min2(Max2(ahz, bhz) - Min2(alz, blz), kMax);
     return 2.0f * (sx * sy + sy * sz + sx * sz);
 }
 

@@ -96,7 +96,11 @@ __global__ __launch_bounds__(kSortBlock) void KOnesweep(const K* __restrict__ ki
     constexpr int IPT = SortTraits<K>::kIpt;
     constexpr int TILE = SortTraits<K>::kTile;
     __shared__ uint32_t s_wh[kSortWaves][RADIX];
-    __shared__ uint32_t s_off[RADIX];
+    __shared__ uint32_t s_start[RADIX];  // the tile's digit runs: first local position
+    __shared__ uint32_t s_off[RADIX];    // the digit's global destination of that position
+    __shared__ uint32_t s_scan[kSortBlock / kWave + 1];
+    __shared__ K s_key[TILE];
+    __shared__ uint32_t s_val[TILE];
     __shared__ int s_tile;
 
     if (threadIdx.x == 0) s_tile = int(atomicAdd(&tmp.tile_counter[pass], 1u));
@@ -109,14 +113,21 @@ __global__ __launch_bounds__(kSortBlock) void KOnesweep(const K* __restrict__ ki
     const uint64_t lt_mask = (1ull << lane) - 1ull;
     const int64_t wbase = tile * TILE + int64_t(w) * (kWave * IPT);
     K key[IPT];
+    uint32_t val[IPT];
     uint32_t rank[IPT];
     uint32_t dig[IPT];
-    volatile uint32_t* wh = s_wh[w];
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
         const int64_t i = wbase + int64_t(j) * kWave + lane;
         const bool valid = i < n;
         key[j] = valid ? kin[i] : K(0);
+        val[j] = valid ? vin[i] : 0u;
+    }
+    volatile uint32_t* wh = s_wh[w];
+#pragma unroll
+    for (int j = 0; j < IPT; j++) {
+        const int64_t i = wbase + int64_t(j) * kWave + lane;
+        const bool valid = i < n;
         const uint32_t d = uint32_t((key[j] >> shift) & K(RADIX - 1));
         dig[j] = valid ? d : uint32_t(RADIX);
         uint64_t peers = __ballot(valid);
@@ -142,16 +153,23 @@ __global__ __launch_bounds__(kSortBlock) void KOnesweep(const K* __restrict__ ki
     }
     __syncthreads();
 
-    // per digit: wave prefixes, tile count, decoupled look-back across tiles
-    uint32_t* st = tmp.status + size_t(pass) * size_t(tiles) * RADIX;
-    for (int r = threadIdx.x; r < RADIX; r += kSortBlock) {
-        uint32_t acc = 0;
+    // per digit: wave prefixes and the tile's count; local digit runs by a block scan; the
+    // global offsets by a decoupled look-back across tiles
+    static_assert(RADIX <= kSortBlock, "one digit per thread");
+    uint32_t acc = 0;
+    const int r = threadIdx.x;
+    if (r < RADIX) {
 #pragma unroll
         for (int ww = 0; ww < kSortWaves; ww++) {
             const uint32_t t = s_wh[ww][r];
             s_wh[ww][r] = acc;
             acc += t;
         }
+    }
+    uint32_t tile_total;
+    const uint32_t start = BlockExclusiveScan<kSortBlock>(acc, s_scan, tile_total);
+    uint32_t* st = tmp.status + size_t(pass) * size_t(tiles) * RADIX;
+    if (r < RADIX) {
         uint32_t excl = 0;
         if (tile == 0) {
             __hip_atomic_store(&st[r], kStInc | acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -173,18 +191,28 @@ __global__ __launch_bounds__(kSortBlock) void KOnesweep(const K* __restrict__ ki
             __hip_atomic_store(&st[size_t(tile) * RADIX + r], kStInc | (excl + acc), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         }
+        s_start[r] = start;
         s_off[r] = tmp.base[pass * RADIX + r] + excl;
     }
     __syncthreads();
 
+    // stage the tile in LDS in (digit, input) order, then write each digit run contiguously
 #pragma unroll
     for (int j = 0; j < IPT; j++) {
-        const int64_t i = wbase + int64_t(j) * kWave + lane;
-        if (i >= n) continue;
         const uint32_t d = dig[j];
-        const uint32_t dst = s_off[d] + s_wh[w][d] + rank[j];
-        kout[dst] = key[j];
-        vout[dst] = vin[i];
+        if (d >= uint32_t(RADIX)) continue;
+        const uint32_t pos = s_start[d] + s_wh[w][d] + rank[j];
+        s_key[pos] = key[j];
+        s_val[pos] = val[j];
+    }
+    __syncthreads();
+    const int count = int(tile_total);
+    for (int k = threadIdx.x; k < count; k += kSortBlock) {
+        const K kk = s_key[k];
+        const uint32_t d = uint32_t((kk >> shift) & K(RADIX - 1));
+        const uint32_t dst = s_off[d] + (uint32_t(k) - s_start[d]);
+        kout[dst] = kk;
+        vout[dst] = s_val[k];
     }
 }
 

@@ -162,11 +162,14 @@ uint64_t Emc60(const Tri& t, const Vec3& c, const Bounds& b, const EmcLayout& L)
 }
 
 // SA of the union of two boxes: std::min/max-form union, size = max - min, then
-// 2 * (x*y + y*z + x*z) as in src/acceleration/aabb.h:19.
+// 2 * (x*y + y*z + x*z) as in src/acceleration/aabb.h:19. A size that overflows is clamped
+// to the largest float, so the area is never NaN (no inf * 0): nearest-neighbour keys are
+// then totally ordered.
 float SurfaceAreaUnion(const float* alo, const float* ahi, const float* blo, const float* bhi) {
-    float sx = Max2(ahi[0], bhi[0]) - Min2(alo[0], blo[0]);
-    float sy = Max2(ahi[1], bhi[1]) - Min2(alo[1], blo[1]);
-    float sz = Max2(ahi[2], bhi[2]) - Min2(alo[2], blo[2]);
+    constexpr float kMax = 3.40282347e38f;
+    float sx = Min2(Max2(ahi[0], bhi[0]) - Min2(alo[0], blo[0]), kMax);
+    float sy = Min2(Max2(ahi[1], bhi[1]) - Min2(alo[1], blo[1]), kMax);
+    float sz = Min2(Max2(ahi[2], bhi[2]) - Min2(alo[2], blo[2]), kMax);
     return 2.0f * (sx * sy + sy * sz + sx * sz);
 }
 
