@@ -303,7 +303,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     const size_t sort_bytes = RadixSortTempBytes(n, key_bits, p.sort_digit_bits, key64);
     void* sort_tmp = C.sort_tmp.ensure(sort_bytes);
     b.fnodes = bvh->fnodes.ensure<FNode>(size_t(std::max<int64_t>(1, n - 1)) * sizeof(FNode));
-    b.kept_id = p.bv_type == DPL_BV_SOBB ? C.kept_id.ensure<int32_t>(size_t(n) * 4) : nullptr;
+    b.kept_child = p.bv_type == DPL_BV_SOBB ? C.kept_id.ensure<int2>(size_t(n) * 8) : nullptr;
     b.tri_ordered = bvh->tri_ordered.ensure<float4>(size_t(n) * 48);
     b.prim_order = bvh->prim_order.ensure<int32_t>(size_t(n) * 4);
     if (p.bv_type == DPL_BV_SOBB) {

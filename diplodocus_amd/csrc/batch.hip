@@ -399,7 +399,8 @@ __global__ __launch_bounds__(kBT) void KBatchBuild(BatchArgs A) {
         uint32_t lt;
         BlockExclusiveScan<kBT>(uint32_t(leaves), S.scan, lt);
         __syncthreads();
-        for (int64_t id = threadIdx.x; id < nn; id += kBT) EmitNode(v, int32_t(id), K);
+        for (int64_t k = threadIdx.x; k < n; k += kBT) EmitTriangle(v, int32_t(k));
+        for (int64_t id = (n > 1 ? n : 0) + threadIdx.x; id < nn; id += kBT) EmitNode(v, int32_t(id), K);
         __syncthreads();
 
         // ---- SAH terms and the mesh's results ----

@@ -417,16 +417,21 @@ __global__ __launch_bounds__(256) void KClassify(BuildBuffers b, int max_leaf) {
     }
 }
 
-// Gathers the primitives of every final leaf, and writes the node of every kept inner node.
-// keptflag has been scanned in place into exclusive ranks and leafslot into begin offsets.
+// Places every triangle in its final leaf, and writes the node of every kept inner node
+// (ids n .. 2n - 2; a one-leaf scene's root is id 0 when n = 1). keptflag has been scanned
+// in place into exclusive ranks and leafslot into begin offsets.
 __global__ __launch_bounds__(256) void KEmit(BuildBuffers b) {
-    const int64_t nn = 2 * b.n - 1;
+    const int64_t n = b.n, nn = 2 * n - 1;
     const int32_t K = b.counters[0];
-    for (int64_t id = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; id < nn; id += int64_t(gridDim.x) * blockDim.x)
+    for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x)
+        EmitTriangle(b, int32_t(k));
+    const int64_t first = n > 1 ? n : 0;
+    for (int64_t id = first + blockIdx.x * int64_t(blockDim.x) + threadIdx.x; id < nn;
+         id += int64_t(gridDim.x) * blockDim.x)
         EmitNode(b, int32_t(id), K);
 }
 
-// SOBB trees: the child slabs of every final inner node, one thread per child.
+// SOBB trees: the child slabs of every final inner node, one thread per node.
 // At least four waves per SIMD: the unrolled 26-DOP triple scan would otherwise be
 // scheduled for instruction-level parallelism at the cost of every register.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void KFitSobb(BuildBuffers b) {
@@ -436,9 +441,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void K
     for (int t = threadIdx.x; t < b.ntriples; t += blockDim.x) s_tri[t] = b.triples[t];
     __syncthreads();
     const float pad = KdopPad(b.bounds);
-    const int64_t m = K > 0 ? 2 * int64_t(K) : 1;
+    const int64_t m = K > 0 ? int64_t(K) : 1;
     for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < m; i += int64_t(gridDim.x) * blockDim.x)
-        FitChild(b, i, K, s_tri, pad, s_w + threadIdx.x, 256);
+        FitNode(b, i, K, s_tri, pad, s_w + threadIdx.x, 256);
 }
 
 // SAH of the final tree in FP64 (Ct = Ci = 1, divided by SA(root) on the host); the node
@@ -476,7 +481,7 @@ hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s) {
     if (b.ndirs > 0) {
         e = hipGetLastError();
         if (e != hipSuccess) return e;
-        const int fgrid = int(std::min<int64_t>(4096, (2 * b.n + 255) / 256));
+        const int fgrid = int(std::min<int64_t>(4096, (b.n + 255) / 256));
         hipLaunchKernelGGL(KFitSobb, dim3(fgrid), dim3(256), 0, s, b);
     }
     return hipGetLastError();

@@ -642,8 +642,9 @@ __device__ __forceinline__ void InitLeaf(const BuildBuffers& b, int64_t k, float
 __device__ __forceinline__ void ClassifyNode(const BuildBuffers& b, int32_t id, int max_leaf, int& kept,
                                              int& leaves) {
     const int64_t n = b.n;
-    const bool in = Inside(b, id, max_leaf);
     const EvalRec e = b.eval[id];
+    // ancestors hold more triangles: from max_leaf up, none can be a collapsed one
+    const bool in = (e.count_col & 0x7FFFFFFF) < max_leaf && Inside(b, id, max_leaf);
     const bool col = e.count_col < 0;
     uint8_t st = 0;
     if (col && !in) {
@@ -664,39 +665,43 @@ __device__ __forceinline__ void ClassifyNode(const BuildBuffers& b, int32_t id, 
 // A child of a kept node is itself kept (state 2) or a final leaf (state 1).
 __device__ __forceinline__ int32_t ChildRef(const BuildBuffers& b, int32_t id, int32_t K) {
     if (b.state[id] == 2) return K - 1 - b.keptflag[id - b.n];
+    if (id < b.n) return ~((b.leafslot[id] << 4) | 1);  // one triangle: minleaf is its own position
     const EvalRec e = b.eval[id];
     return ~((b.leafslot[e.minleaf] << 4) | (e.count_col & 0x7FFFFFFF));
 }
 
-// Gathers the primitives of a final leaf (left-first DFS of its PLOC subtree) and writes the
-// node of a kept inner node (for SOBB trees its child references only; FitChild adds the
-// child slabs).
+// The final position of sorted triangle k: its final leaf's first slot plus the triangles
+// before it in the leaf's left-first order (the left siblings' counts on the way up).
+__device__ __forceinline__ void EmitTriangle(const BuildBuffers& b, int32_t k) {
+    int32_t x = k, off = 0;
+    int32_t first = k;  // a one-triangle leaf starts at its own position's slot
+    if (b.state[k] == 0) {
+        do {
+            const int32_t p = b.parent[x];
+            const int32_t left = b.nodes[p].left;
+            if (x != left) off += b.eval[left].count_col & 0x7FFFFFFF;
+            x = p;
+        } while (b.state[x] == 0);
+        first = b.eval[x].minleaf;
+    }
+    const int32_t pos = b.leafslot[first] + off;
+    const int32_t orig = b.nodes[k].left;
+    b.prim_order[pos] = orig;
+    const float* p = b.tri + 9 * int64_t(orig);
+    float4* d = b.tri_ordered + 3 * int64_t(pos);
+    d[0] = make_float4(Canon(p[0]), Canon(p[1]), Canon(p[2]), __int_as_float(orig));
+    d[1] = make_float4(Canon(p[3]), Canon(p[4]), Canon(p[5]), 0.0f);
+    d[2] = make_float4(Canon(p[6]), Canon(p[7]), Canon(p[8]), 0.0f);
+}
+
+// Writes the node of a kept inner node (for SOBB trees its child references only; FitNode
+// adds the child slabs), or, when the whole scene is one leaf, that leaf as the root.
 __device__ __forceinline__ void EmitNode(const BuildBuffers& b, int32_t id, int32_t K) {
     const int64_t n = b.n, nn = 2 * n - 1;
     const uint8_t st = b.state[id];
     if (st == 1) {
-        const EvalRec e = b.eval[id];
-        int32_t pos = b.leafslot[e.minleaf];
-        int32_t st[kMaxLeaf + 1];
-        int sp = 0;
-        st[sp++] = id;
-        while (sp > 0) {
-            const int32_t x = st[--sp];
-            if (x < n) {
-                const int32_t orig = b.nodes[x].left;
-                b.prim_order[pos] = orig;
-                const float* p = b.tri + 9 * int64_t(orig);
-                float4* d = b.tri_ordered + 3 * int64_t(pos);
-                d[0] = make_float4(Canon(p[0]), Canon(p[1]), Canon(p[2]), __int_as_float(orig));
-                d[1] = make_float4(Canon(p[3]), Canon(p[4]), Canon(p[5]), 0.0f);
-                d[2] = make_float4(Canon(p[6]), Canon(p[7]), Canon(p[8]), 0.0f);
-                pos++;
-            } else if (sp + 2 <= kMaxLeaf + 1) {
-                st[sp++] = b.nodes[x].right;
-                st[sp++] = b.nodes[x].left;
-            }
-        }
         if (id == nn - 1) {  // the whole scene is one leaf: one node, empty right slot
+            const EvalRec e = b.eval[id];
             const PNode r = b.nodes[id];
             FNode f;
             f.lx0 = f.rx0 = r.lo[0];
@@ -707,7 +712,7 @@ __device__ __forceinline__ void EmitNode(const BuildBuffers& b, int32_t id, int3
             f.lz1 = f.rz1 = r.hi[2];
             f.left = ~((b.leafslot[e.minleaf] << 4) | (e.count_col & 0x7FFFFFFF));
             f.right = ~0;
-            f.pad0 = f.pad1 = 0;  // SOBB trees: FitChild writes the slabs and directions
+            f.pad0 = f.pad1 = 0;  // SOBB trees: FitNode writes the slabs and directions
             b.fnodes[0] = f;
         }
     } else if (st == 2) {
@@ -716,8 +721,8 @@ __device__ __forceinline__ void EmitNode(const BuildBuffers& b, int32_t id, int3
         const int32_t right = ChildRef(b, nd.right, K);
         const int32_t slot = K - 1 - b.keptflag[id - n];
         FNode* out = b.fnodes + slot;
-        if (b.ndirs > 0) {  // child slabs and directions: FitChild, one thread per child
-            b.kept_id[slot] = id;
+        if (b.ndirs > 0) {  // child slabs and directions: FitNode, one thread per node
+            b.kept_child[slot] = make_int2(nd.left, nd.right);
             reinterpret_cast<int4*>(out)[3] = make_int4(left, right, 0, 0);
         } else {
             const PNode L = b.nodes[nd.left], R = b.nodes[nd.right];
@@ -732,27 +737,26 @@ __device__ __forceinline__ void EmitNode(const BuildBuffers& b, int32_t id, int3
     }
 }
 
-// SOBB of child c = i & 1 of final inner node i >> 1 (or, when the whole scene is one leaf,
-// i == 0 fits the root into both halves): slabs to the node's float slots, directions OR-ed
-// into its pad word (zeroed by EmitNode).
-__device__ __forceinline__ void FitChild(const BuildBuffers& b, int64_t i, int32_t K, const int4* s_tri, float pad,
-                                         float* wl, int stride) {
+// SOBBs of both children of final inner node `slot` (or, when the whole scene is one leaf,
+// slot 0 gets the root's SOBB in both halves): slabs to the node's float slots, directions
+// to its pad word (EmitNode wrote the child references and a zero pad word).
+__device__ __forceinline__ void FitNode(const BuildBuffers& b, int64_t slot, int32_t K, const int4* s_tri, float pad,
+                                        float* wl, int stride) {
     float g[6];
+    float2* o = reinterpret_cast<float2*>(b.fnodes + slot);
     if (K == 0) {
-        if (i != 0) return;
+        if (slot != 0) return;
         const uint32_t d = FitSobb(b, s_tri, int32_t(2 * b.n - 2), pad, g, wl, stride);
-        float* fl = &b.fnodes[0].lx0;
-        for (int k = 0; k < 6; k++) fl[k] = fl[6 + k] = g[k];
-        atomicOr(&b.fnodes[0].pad0, int32_t(d | (d << 12)));
+        for (int k = 0; k < 3; k++) o[k] = o[3 + k] = make_float2(g[2 * k], g[2 * k + 1]);
+        b.fnodes[0].pad0 = int32_t(d | (d << 12));
         return;
     }
-    const int64_t slot = i >> 1;
-    const int c = int(i & 1);
-    const PNode nd = b.nodes[b.kept_id[slot]];
-    const uint32_t d = FitSobb(b, s_tri, c ? nd.right : nd.left, pad, g, wl, stride);
-    float2* o = reinterpret_cast<float2*>(reinterpret_cast<float*>(b.fnodes + slot) + 6 * c);
+    const int2 ch = b.kept_child[slot];
+    const uint32_t dl = FitSobb(b, s_tri, ch.x, pad, g, wl, stride);
     o[0] = make_float2(g[0], g[1]), o[1] = make_float2(g[2], g[3]), o[2] = make_float2(g[4], g[5]);
-    atomicOr(&b.fnodes[slot].pad0, int32_t(d << (12 * c)));
+    const uint32_t dr = FitSobb(b, s_tri, ch.y, pad, g, wl, stride);
+    o[3] = make_float2(g[0], g[1]), o[4] = make_float2(g[2], g[3]), o[5] = make_float2(g[4], g[5]);
+    b.fnodes[slot].pad0 = int32_t(dl | (dr << 12));
 }
 
 // The SAH terms (Ct = Ci = 1) one final node contributes: its children's areas, a leaf

@@ -226,7 +226,7 @@ struct BuildBuffers {
     int32_t* counters;     // 8: [0] kept inner, [1] final leaves, [2] max height
     double* sah;           // 1
     FNode* fnodes;         // n (capacity)
-    int32_t* kept_id;      // n: PLOC id of each final inner node (SOBB trees; else null)
+    int2* kept_child;      // n: PLOC ids of the children of each final inner node (SOBB trees; else null)
     float4* tri_ordered;   // 3n
     int32_t* prim_order;   // n
     // SOBB builds (ndirs > 0): k-DOP of every node, 2*ndirs floats (lo..., hi...)
