@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libdpl.so")
 
 DPL_OK, DPL_ERR_INVALID_ARG, DPL_ERR_OOM, DPL_ERR_DEVICE, DPL_ERR_UNSUPPORTED, DPL_ERR_INTERNAL = range(6)
 DPL_MEM_HOST, DPL_MEM_DEVICE = 0, 1
-DPL_TRACE_ANY, DPL_TRACE_CULL_BACK = 1, 2
+DPL_TRACE_ANY, DPL_TRACE_CULL_BACK, DPL_TRACE_WIDE = 1, 2, 4
 
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "DEVICE", 4: "UNSUPPORTED", 5: "INTERNAL"}
 
@@ -41,6 +41,10 @@ class BuildStats(C.Structure):
 class BvhLayout(C.Structure):
     _fields_ = [("prim_count", C.c_int64), ("node_count", C.c_int64), ("ploc_node_count", C.c_int64),
                 ("iteration_count", C.c_int64), ("mesh_count", C.c_int32), ("node_bytes", C.c_int32)]
+
+
+class WideInfo(C.Structure):
+    _fields_ = [("width", C.c_int32), ("node_bytes", C.c_int32), ("node_count", C.c_int64), ("widen_ms", C.c_double)]
 
 
 class RenderParams(C.Structure):
@@ -124,6 +128,9 @@ def _load() -> C.CDLL:
                                            C.POINTER(WhittedStats)]),
         "dpl_trace_graph_create": (I32, [P, P, P, P, I64, U32, P, C.POINTER(P)]),
         "dpl_trace_graph_launch": (I32, [P]),
+        "dpl_bvh_widen": (I32, [P, I32, C.POINTER(WideInfo)]),
+        "dpl_bvh_get_wide": (I32, [P, C.POINTER(WideInfo)]),
+        "dpl_bvh_download_wide": (I32, [P, P]),
         "dpl_trace_graph_destroy": (None, [P]),
         "dpl_autotune": (I32, [P, P, I64, P, I32, I32, C.POINTER(BuildParams), C.c_double, C.c_char_p,
                                C.POINTER(BuildParams), C.POINTER(AutotuneInfo)]),
@@ -145,5 +152,6 @@ EXPORTED = [
     "dpl_trace_batched",
     "dpl_bvh_get_layout", "dpl_bvh_download", "dpl_compute_codes", "dpl_radix_sort_pairs", "dpl_render_forward",
     "dpl_render_adjoint", "dpl_bvh_get_meshes", "dpl_trace_graph_create", "dpl_trace_graph_launch",
-    "dpl_trace_graph_destroy", "dpl_autotune", "dpl_render_whitted", "dpl_render_whitted_stats",
+    "dpl_trace_graph_destroy", "dpl_autotune", "dpl_render_whitted", "dpl_render_whitted_stats", "dpl_bvh_widen",
+    "dpl_bvh_get_wide", "dpl_bvh_download_wide",
 ]

@@ -294,6 +294,25 @@ class GpuBvh:
         self.ctx.check(lib.dpl_bvh_download(self.h, *ptrs))
         return out
 
+    # ---- wide tree (no counterpart in the reference): 4 or 8 children, quantized boxes ----
+    def widen(self, width: int = 4) -> dict:
+        """Adds the wide form; trace(..., flags | DPL_TRACE_WIDE) then traverses it."""
+        info = _lib.WideInfo()
+        self.ctx.check(lib.dpl_bvh_widen(self.h, width, C.byref(info)))
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def wide_info(self) -> dict:
+        info = _lib.WideInfo()
+        self.ctx.check(lib.dpl_bvh_get_wide(self.h, C.byref(info)))
+        return {k: getattr(info, k) for k, _ in info._fields_}
+
+    def download_wide(self) -> np.ndarray:
+        """Wide nodes as uint8 rows of node_bytes."""
+        info = self.wide_info()
+        out = np.zeros((int(info["node_count"]), int(info["node_bytes"])), np.uint8)
+        self.ctx.check(lib.dpl_bvh_download_wide(self.h, out.ctypes.data_as(C.c_void_p)))
+        return out
+
 
 class TraceGraph:
     """A trace pass captured once in a hipGraph and replayed by launch(); rays, ray_mesh

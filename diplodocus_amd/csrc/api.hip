@@ -156,7 +156,7 @@ struct dpl_context {
     // scratch
     DevBuf tri, keys, vals, keys_alt, vals_alt, nodes, parent, clusters0, clusters1, ccount, status0, status1,
         small, eval, state, kept_id, leafslot, keptflag, scan_tmp, sort_tmp, rays, hits,
-        trace_counter, spill, kdop, sobb_set;
+        trace_counter, spill, kdop, sobb_set, wide_tmp;
     // batched builds and traces
     DevBuf b_off, b_foff, b_order, b_results, b_queue, ray_mesh;
     std::string device_name;
@@ -197,6 +197,10 @@ struct dpl_bvh {
     DevBuf batch_tree;                 // BatchTree per mesh
     std::vector<dpl_mesh_info> meshes;
     int64_t fnode_slots = 0;           // final-node slots of all meshes
+    // wide form (dpl_bvh_widen): node i converts final node i; dropped by a rebuild
+    DevBuf wnodes;
+    int32_t wide_width = 0;
+    double widen_ms = 0.0;
 };
 
 namespace {
@@ -484,6 +488,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     bvh->iter_counts = iters;
     bvh->has_intermediates = p.keep_intermediates != 0;
     bvh->codes64 = key64;
+    bvh->wide_width = 0;
     if (out_stats) *out_stats = st;
 }
 
@@ -731,6 +736,7 @@ void BatchImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, const int64_
     bvh->ndirs = 0;
     bvh->params = p;
     bvh->has_intermediates = false;
+    bvh->wide_width = 0;
     bvh->stats = st;
     if (out_stats) *out_stats = st;
 }
@@ -832,10 +838,12 @@ static dpl_status TraceImpl(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray*
     return Guard(ctx, [&] {
         Require(n >= 0, "n must be non-negative");
         Require(n == 0 || (rays && hits), "rays and hits must be non-null");
-        Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK)) == 0u, "unknown trace flags");
+        Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK | DPL_TRACE_WIDE)) == 0u, "unknown trace flags");
         Require(mem == DPL_MEM_HOST || mem == DPL_MEM_DEVICE, "mem must be DPL_MEM_HOST or DPL_MEM_DEVICE");
         Require(bvh->ctx == ctx, "bvh belongs to another context");
         Require(!bvh->batched, "a batched BVH is traced with dpl_trace_batched");
+        const bool wide = flags & DPL_TRACE_WIDE;
+        Require(!wide || bvh->wide_width != 0, "DPL_TRACE_WIDE needs a wide tree (dpl_bvh_widen)");
         if (counts_out) counts_out[0] = counts_out[1] = 0;
         if (n == 0) return;
         hipStream_t s = ctx->stream;
@@ -846,14 +854,24 @@ static dpl_status TraceImpl(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray*
             dh = ctx->hits.ensure<dpl_hit>(size_t(n) * sizeof(dpl_hit));
             DPL_HIP(hipMemcpyAsync(const_cast<dpl_ray*>(dr), rays, size_t(n) * sizeof(dpl_ray), hipMemcpyHostToDevice, s));
         }
-        TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
-                       int32_t(bvh->prim_count), bvh->tree_height, bvh->ndirs, bvh->sobb_dirs.as<float>()};
-        int32_t* spill = ctx->spill.ensure<int32_t>(TraceSpillBytes(ctx->cu_count, bvh->tree_height));
         uint8_t* tc = ctx->trace_counter.as<uint8_t>();
         unsigned long long* counters = nullptr;
         if (counts_out) counters = reinterpret_cast<unsigned long long*>(tc + 64);  // zeroed by the launch
-        hipError_t e = LaunchTrace(tree, dr, n, flags, dh, ctx->cu_count, reinterpret_cast<uint32_t*>(tc), spill,
-                                   counters, s);
+        hipError_t e;
+        if (wide) {
+            const WideTree wt{bvh->wnodes.as<float4>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
+                              int32_t(bvh->prim_count), bvh->wide_width, bvh->tree_height};
+            int32_t* spill = ctx->spill.ensure<int32_t>(
+                std::max<size_t>(16, TraceWideSpillBytes(ctx->cu_count, bvh->wide_width, bvh->tree_height)));
+            e = LaunchTraceWide(wt, dr, n, flags & ~DPL_TRACE_WIDE, dh, ctx->cu_count, reinterpret_cast<uint32_t*>(tc),
+                                spill, counters, s);
+        } else {
+            TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
+                           int32_t(bvh->prim_count), bvh->tree_height, bvh->ndirs, bvh->sobb_dirs.as<float>()};
+            int32_t* spill = ctx->spill.ensure<int32_t>(TraceSpillBytes(ctx->cu_count, bvh->tree_height));
+            e = LaunchTrace(tree, dr, n, flags, dh, ctx->cu_count, reinterpret_cast<uint32_t*>(tc), spill, counters,
+                            s);
+        }
         if (counts_out) {
             DPL_HIP(hipMemcpyAsync(counts_out, counters, 16, hipMemcpyDeviceToHost, s));
             DPL_HIP(hipStreamSynchronize(s));
@@ -933,11 +951,15 @@ dpl_status dpl_trace_graph_create(dpl_context* ctx, const dpl_bvh* bvh, const dp
     g->ctx = ctx;
     dpl_status st = Guard(ctx, [&] {
         Require(n >= 1 && rays && hits, "a captured trace needs n >= 1 and device rays / hits");
-        Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK)) == 0u, "unknown trace flags");
+        Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK | DPL_TRACE_WIDE)) == 0u, "unknown trace flags");
         Require(bvh->ctx == ctx, "bvh belongs to another context");
         Require(bvh->batched == (ray_mesh != nullptr), "ray_mesh is required exactly for a batched BVH");
+        const bool wide = flags & DPL_TRACE_WIDE;
+        Require(!wide || bvh->wide_width != 0, "DPL_TRACE_WIDE needs a wide tree (dpl_bvh_widen)");
         uint32_t* counter = g->counter.ensure<uint32_t>(256);
-        int32_t* spill = g->spill.ensure<int32_t>(std::max<size_t>(16, TraceSpillBytes(ctx->cu_count, bvh->tree_height)));
+        const size_t spill_bytes = wide ? TraceWideSpillBytes(ctx->cu_count, bvh->wide_width, bvh->tree_height)
+                                        : TraceSpillBytes(ctx->cu_count, bvh->tree_height);
+        int32_t* spill = g->spill.ensure<int32_t>(std::max<size_t>(16, spill_bytes));
         DPL_HIP(hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking));
         DPL_HIP(hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal));
         hipError_t e;
@@ -945,6 +967,11 @@ dpl_status dpl_trace_graph_create(dpl_context* ctx, const dpl_bvh* bvh, const dp
             e = LaunchTraceBatched(bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(),
                                    bvh->batch_tree.as<BatchTree>(), bvh->mesh_count, bvh->tree_height, rays, ray_mesh,
                                    n, flags, hits, ctx->cu_count, counter, spill, g->capture);
+        } else if (wide) {
+            const WideTree wt{bvh->wnodes.as<float4>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
+                              int32_t(bvh->prim_count), bvh->wide_width, bvh->tree_height};
+            e = LaunchTraceWide(wt, rays, n, flags & ~DPL_TRACE_WIDE, hits, ctx->cu_count, counter, spill, nullptr,
+                                g->capture);
         } else {
             TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
                            int32_t(bvh->prim_count), bvh->tree_height, bvh->ndirs, bvh->sobb_dirs.as<float>()};
@@ -980,6 +1007,68 @@ void dpl_trace_graph_destroy(dpl_trace_graph* g) {
     if (g->graph) (void)hipGraphDestroy(g->graph);
     if (g->capture) (void)hipStreamDestroy(g->capture);
     delete g;
+}
+
+// ---- wide trees ----
+static void WideInfo(const dpl_bvh* bvh, dpl_wide_info* info) {
+    info->width = bvh->wide_width;
+    info->node_bytes = bvh->wide_width ? WideNodeBytes(bvh->wide_width) : 0;
+    info->node_count = bvh->wide_width ? bvh->node_count : 0;
+    info->widen_ms = bvh->wide_width ? bvh->widen_ms : 0.0;
+}
+
+dpl_status dpl_bvh_widen(dpl_bvh* bvh, int32_t width, dpl_wide_info* info) {
+    if (!bvh) return DPL_ERR_INVALID_ARG;
+    dpl_context* ctx = bvh->ctx;
+    return Guard(ctx, [&] {
+        Require(width == 4 || width == 8, "width must be 4 or 8");
+        Require(!bvh->batched, "a batched BVH has no wide form");
+        hipStream_t s = ctx->stream;
+        const int32_t K = bvh->node_count;
+        bvh->wide_width = 0;
+        float4* out = bvh->wnodes.ensure<float4>(std::max<size_t>(16, size_t(K) * WideNodeBytes(width)));
+        const FNode* src = bvh->fnodes.as<FNode>();
+        FNode* refit = nullptr;
+        float* box = nullptr;
+        int32_t* done = nullptr;
+        if (bvh->ndirs > 0) {  // SOBB child slabs: refit axis-aligned boxes first
+            uint8_t* t = ctx->wide_tmp.ensure<uint8_t>(std::max<size_t>(16, size_t(K) * (sizeof(FNode) + 28)));
+            refit = reinterpret_cast<FNode*>(t);
+            box = reinterpret_cast<float*>(t + size_t(K) * sizeof(FNode));
+            done = reinterpret_cast<int32_t*>(box + 6 * size_t(K));
+        }
+        DPL_HIP(hipEventRecord(ctx->ev[0], s));
+        if (refit) {
+            DPL_HIP(LaunchRefitAabb(src, K, bvh->tri_ordered.as<float4>(), int32_t(bvh->prim_count), bvh->tree_height,
+                                    refit, box, done, s));
+            src = refit;
+        }
+        DPL_HIP(LaunchWiden(src, K, width, out, s));
+        DPL_HIP(hipEventRecord(ctx->ev[1], s));
+        DPL_HIP(hipEventSynchronize(ctx->ev[1]));
+        float ms = 0.0f;
+        DPL_HIP(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        bvh->widen_ms = ms;
+        bvh->wide_width = width;
+        if (info) WideInfo(bvh, info);
+    });
+}
+
+dpl_status dpl_bvh_get_wide(const dpl_bvh* bvh, dpl_wide_info* info) {
+    if (!bvh || !info) return DPL_ERR_INVALID_ARG;
+    WideInfo(bvh, info);
+    return DPL_OK;
+}
+
+dpl_status dpl_bvh_download_wide(const dpl_bvh* bvh, void* nodes) {
+    if (!bvh || !nodes) return DPL_ERR_INVALID_ARG;
+    dpl_context* ctx = bvh->ctx;
+    return Guard(ctx, [&] {
+        Require(bvh->wide_width != 0, "the BVH has no wide tree (dpl_bvh_widen)");
+        DPL_HIP(hipStreamSynchronize(ctx->stream));
+        DPL_HIP(hipMemcpy(nodes, bvh->wnodes.p, size_t(bvh->node_count) * WideNodeBytes(bvh->wide_width),
+                          hipMemcpyDeviceToHost));
+    });
 }
 
 dpl_status dpl_compute_codes(dpl_context* ctx, const float* tri, int64_t n, int32_t morton_type, uint64_t* codes,

@@ -345,6 +345,30 @@ hipError_t LaunchTraceBatched(const FNode* nodes, const float4* tris, const Batc
                               hipStream_t s);
 // Bytes of the per-lane stack spill region the trace and render kernels need for a tree.
 size_t TraceSpillBytes(int cu_count, int height);
+// ---- wide trees: 4 or 8 children per node, 8-bit quantized child boxes ----
+// Node i (of node_count, the two-child tree's count) is node i of the two-child tree opened
+// greedily to W children. Bytes: origin xyz (f32), meta (biased exponent byte per axis,
+// child count << 24), W child references (as FNode's), 6 planes of W bytes (lo x, hi x, lo y,
+// hi y, lo z, hi z; byte k = child k); 64 B for W = 4 (zero tail), 96 B for W = 8. Child k's
+// bound is fmaf(q, 2^(e - 127), origin) and contains the exact child box.
+constexpr int WideNodeBytes(int w) { return w == 4 ? 64 : 96; }
+struct WideTree {
+    const float4* nodes;
+    const float4* tris;
+    int32_t node_count;
+    int32_t prim_count;
+    int32_t width;
+    int32_t height;  // of the two-child tree (sizes the spill stacks)
+};
+// Axis-aligned copy of a final tree (child boxes refitted from the triangles), for SOBB trees;
+// height: the tree's height (height + 1 launches). Scratch: box 6 floats and done 1 int per node.
+hipError_t LaunchRefitAabb(const FNode* nodes, int32_t node_count, const float4* tris, int32_t prim_count,
+                           int32_t height, FNode* out, float* box, int32_t* done, hipStream_t s);
+hipError_t LaunchWiden(const FNode* aabb_nodes, int32_t node_count, int width, void* out, hipStream_t s);
+hipError_t LaunchTraceWide(const WideTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,
+                           int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters,
+                           hipStream_t s);
+size_t TraceWideSpillBytes(int cu_count, int width, int height);
 hipError_t LaunchRender(bool adjoint, const TraceTree& tree, const dpl_render_params& rp, const float* basis,
                         const int32_t* mat_id, const float* albedo, const float* emission, const float* d_image,
                         float* image, float* grad_albedo, float* grad_emission, int cu_count, uint32_t* counter,

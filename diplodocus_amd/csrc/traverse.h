@@ -145,6 +145,7 @@ struct HitResult {
 // when the stack is deeper than S (the spill region holds height - S + 1 entries).
 template <int S, int BLOCK>
 struct LaneStack {
+    static constexpr int kBlock = BLOCK;
     int32_t* col;
     int32_t* spill;
     int sp;
@@ -163,18 +164,81 @@ struct LaneStack {
     }
 };
 
+// The same ring with the spill region interleaved across lanes: entry d of lane l is
+// spill[d * step + l] (step = lanes of the launch), so lanes of a wave that spill at the same
+// depth write the same cache lines.
+template <int S, int BLOCK>
+struct LaneStackInterleaved {
+    static constexpr int kBlock = BLOCK;
+    int32_t* col;
+    int32_t* spill;  // spill base + this lane's index
+    int64_t step;
+    int sp;
+    __device__ __forceinline__ void Push(int32_t x) {
+        int32_t* slot = &col[(sp & (S - 1)) * BLOCK];
+        if (sp >= S) spill[int64_t(sp - S) * step] = *slot;
+        *slot = x;
+        sp++;
+    }
+    __device__ __forceinline__ int32_t Pop() {
+        sp--;
+        int32_t* slot = &col[(sp & (S - 1)) * BLOCK];
+        const int32_t x = *slot;
+        if (sp >= S) *slot = spill[int64_t(sp - S) * step];
+        return x;
+    }
+};
+
 struct TraceCounters {
     uint32_t nodes;  // inner nodes visited (two box tests each)
     uint32_t tris;   // triangle tests
 };
 
+// One triangle test of the closest-hit (or any-hit) rule; true when an any-hit query is done.
+template <bool ANY>
+__device__ __forceinline__ bool TestTriangle(const float4* tris, int32_t k, const WtRay& w, bool cull, float& best_t,
+                                             HitResult& r) {
+    const float4 p0 = tris[3 * int64_t(k)], p1 = tris[3 * int64_t(k) + 1], p2 = tris[3 * int64_t(k) + 2];
+    float t, u1, u2;
+    if (!Watertight(w, p0, p1, p2, cull, t, u1, u2)) return false;
+    if (!(t > w.tmin)) return false;
+    const int32_t prim = __float_as_int(p0.w);
+    if (ANY) {
+        if (t < w.tmax) {
+            r = HitResult{t, u1, u2, prim, k};
+            return true;
+        }
+        return false;
+    }
+    if (t < best_t || (t == best_t && r.prim >= 0 && prim < r.prim)) {
+        best_t = t;
+        r = HitResult{t, u1, u2, prim, k};
+    }
+    return false;
+}
+
+// Tests every triangle of the leaf reference `ref` (packed (first << 4) | count); true when
+// an any-hit query is done.
+template <bool ANY>
+__device__ __forceinline__ bool TestLeaf(const float4* tris, int32_t prim_count, int32_t ref, const WtRay& w, bool cull,
+                                         float& best_t, HitResult& r, TraceCounters& cnt) {
+    const int32_t packed = ~ref;
+    const int32_t b0 = packed >> 4;
+    const int32_t np = b0 + (packed & 15) <= prim_count ? (packed & 15) : 0;
+    for (int32_t k = b0; k < b0 + np; k++) {
+        cnt.tris++;
+        if (TestTriangle<ANY>(tris, k, w, cull, best_t, r)) return true;
+    }
+    return false;
+}
+
 // Ordered DFS: nearer child first, the other pushed. ANY stops at the first hit inside
 // (tmin, tmax). Bounded: a tree visits each node at most once. SOBB selects the skewed
 // child boxes (12 floats = left (lo hi) x 3 slabs, right the same; d.z packs the slab
 // directions, 4 bits each, left in bits 0-11 and right in bits 12-23).
-template <bool ANY, bool SOBB, int S, int BLOCK>
+template <bool ANY, bool SOBB, class Stack>
 __device__ __forceinline__ HitResult Traverse(const TraceTree& tree, const WtRay& w, const SobbRay& sr, bool cull,
-                                              LaneStack<S, BLOCK>& st, TraceCounters& cnt) {
+                                              Stack& st, TraceCounters& cnt) {
     HitResult r{0.0f, 0.0f, 0.0f, -1, -1};
     float best_t = w.tmax;
     st.sp = 0;
@@ -191,8 +255,9 @@ __device__ __forceinline__ HitResult Traverse(const TraceTree& tree, const WtRay
             bool hl, hr;
             if (SOBB) {
                 const uint32_t dirs = uint32_t(d.z);
-                hl = SobbBox<BLOCK>(sr, w.tmin, best_t, a.x, a.y, a.z, a.w, b.x, b.y, dirs & 0xFFFu, tl);
-                hr = SobbBox<BLOCK>(sr, w.tmin, best_t, b.z, b.w, c.x, c.y, c.z, c.w, (dirs >> 12) & 0xFFFu, tr);
+                hl = SobbBox<Stack::kBlock>(sr, w.tmin, best_t, a.x, a.y, a.z, a.w, b.x, b.y, dirs & 0xFFFu, tl);
+                hr = SobbBox<Stack::kBlock>(sr, w.tmin, best_t, b.z, b.w, c.x, c.y, c.z, c.w, (dirs >> 12) & 0xFFFu,
+                                            tr);
             } else {
                 hl = Box(w, best_t, a.x, a.y, a.z, a.w, c.x, c.y, tl);
                 hr = Box(w, best_t, b.x, b.y, b.z, b.w, c.z, c.w, tr);
@@ -210,32 +275,7 @@ __device__ __forceinline__ HitResult Traverse(const TraceTree& tree, const WtRay
                 ref = st.Pop();
             }
         } else {
-            const int32_t packed = ~ref;
-            const int32_t b0 = packed >> 4;
-            const int32_t np = b0 + (packed & 15) <= tree.prim_count ? (packed & 15) : 0;
-            bool done = false;
-            for (int32_t k = b0; k < b0 + np; k++) {
-                cnt.tris++;
-                const float4 p0 = tree.tris[3 * int64_t(k)], p1 = tree.tris[3 * int64_t(k) + 1],
-                             p2 = tree.tris[3 * int64_t(k) + 2];
-                float t, u1, u2;
-                if (!Watertight(w, p0, p1, p2, cull, t, u1, u2)) continue;
-                if (!(t > w.tmin)) continue;
-                const int32_t prim = __float_as_int(p0.w);
-                if (ANY) {
-                    if (t < w.tmax) {
-                        r = HitResult{t, u1, u2, prim, k};
-                        done = true;
-                        break;
-                    }
-                    continue;
-                }
-                if (t < best_t || (t == best_t && r.prim >= 0 && prim < r.prim)) {
-                    best_t = t;
-                    r = HitResult{t, u1, u2, prim, k};
-                }
-            }
-            if (done || st.sp == 0) break;
+            if (TestLeaf<ANY>(tree.tris, tree.prim_count, ref, w, cull, best_t, r, cnt) || st.sp == 0) break;
             ref = st.Pop();
         }
     }

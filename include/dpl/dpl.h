@@ -114,6 +114,7 @@ typedef struct dpl_hit {
 
 #define DPL_TRACE_ANY 1u       /* any-hit (shadow) query: IntersectAny */
 #define DPL_TRACE_CULL_BACK 2u /* backface culling */
+#define DPL_TRACE_WIDE 4u      /* traverse the wide tree made by dpl_bvh_widen */
 
 /* Context: one device, one stream, scratch pools. The reference selects the device with
  * cudaSetDevice(0) (src/gpu/cuda_utils.h:41). */
@@ -198,6 +199,30 @@ dpl_status dpl_trace_graph_create(dpl_context* ctx, const dpl_bvh* bvh, const dp
                                   int64_t n, uint32_t flags, dpl_hit* hits, dpl_trace_graph** out);
 dpl_status dpl_trace_graph_launch(dpl_trace_graph* graph); /* on the context's stream */
 void dpl_trace_graph_destroy(dpl_trace_graph* graph);
+
+/* ---- Wide trees ----
+ * Converts a built single-scene BVH into a tree of `width` (4 or 8) children per node whose
+ * child boxes are quantized to 8 bits per bound (64-byte nodes for width 4, 96-byte for 8),
+ * kept beside the two-child tree until the next rebuild. dpl_trace, dpl_trace_counters and
+ * dpl_trace_graph_create with DPL_TRACE_WIDE traverse it; closest hits equal the two-child
+ * tree's (counters then count wide nodes). SOBB trees are refitted to axis-aligned boxes
+ * first. No counterpart in the reference, whose GPU intersector is brute force
+ * (src/gpu/renderer/gpu_renderer_util.h:29-94) and whose SBvh is two-child
+ * (src/acceleration/sbvh.h:25-31); it serves the same AccelerationStructure::Intersect /
+ * IntersectAny calls (acceleration_structure.h:17-18) as dpl_trace. Synchronises. */
+typedef struct dpl_wide_info {
+    int32_t width;       /* 0: no wide tree */
+    int32_t node_bytes;  /* 64 or 96 */
+    int64_t node_count;  /* the two-child tree's node count: wide node i converts node i */
+    double widen_ms;     /* device time of the conversion (refit included) */
+} dpl_wide_info;
+dpl_status dpl_bvh_widen(dpl_bvh* bvh, int32_t width, dpl_wide_info* info /* may be NULL */);
+dpl_status dpl_bvh_get_wide(const dpl_bvh* bvh, dpl_wide_info* info);
+/* Host copy of the wide nodes: node_count x node_bytes. Node layout: origin xyz (f32),
+ * meta (u32: biased exponent byte per axis, child count << 24), width child references
+ * (as the two-child nodes'), 6 planes of width bytes (lo x, hi x, lo y, hi y, lo z, hi z;
+ * byte k = child k), zero tail; child k's bound = fmaf(q, 2^(exponent - 127), origin). */
+dpl_status dpl_bvh_download_wide(const dpl_bvh* bvh, void* nodes);
 
 /* Whitted ray tracing: the reference's CpuRaytracer (src/renderer/cpu_raytracer.cc:50-288)
  * over the BVH — area-light next-event estimation with hash-sampled light points (seed 42),

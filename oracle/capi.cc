@@ -278,6 +278,46 @@ void orc_trace_final_counted(const void* fnodes, int64_t nnodes, const float* tr
     TraceFinalCounted(t, rays, n, flags, hits, threads, tc);
     counts[0] = tc.nodes, counts[1] = tc.tris;
 }
+// Axis-aligned child boxes of every node of a final tree (any bounding-volume type), from
+// its triangles; out: nnodes x 64 B.
+int orc_refit_aabb(const void* fnodes, int64_t nnodes, const float* tri_ordered, int64_t nprim, void* out) {
+    FinalTree t;
+    t.nodes.resize(nnodes);
+    std::memcpy(t.nodes.data(), fnodes, size_t(nnodes) * sizeof(FinalTree::Node));
+    t.tri_ordered.assign(tri_ordered, tri_ordered + 12 * nprim);
+    std::vector<FinalTree::Node> o;
+    try {
+        RefitAabb(t, o);
+    } catch (const std::exception&) {
+        return -1;
+    }
+    std::memcpy(out, o.data(), o.size() * sizeof(FinalTree::Node));
+    return 0;
+}
+// Wide nodes (width 4 or 8) from AABB final nodes; out: nnodes x orc_wide_node_bytes(width).
+int orc_widen(const void* aabb_nodes, int64_t nnodes, int32_t width, void* out) {
+    if (width != 4 && width != 8) return -1;
+    std::vector<FinalTree::Node> nodes(static_cast<size_t>(nnodes));
+    std::memcpy(nodes.data(), aabb_nodes, size_t(nnodes) * sizeof(FinalTree::Node));
+    std::vector<uint8_t> o;
+    Widen(nodes, width, o);
+    std::memcpy(out, o.data(), o.size());
+    return 0;
+}
+int32_t orc_wide_node_bytes(int32_t width) { return WideNodeBytes(width); }
+// Trace over wide nodes; counts[0] wide nodes visited, counts[1] triangle tests (may be NULL).
+void orc_trace_wide(const void* wnodes, int64_t nnodes, int32_t width, const float* tri_ordered, int64_t nprim,
+                    const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads, int64_t* counts) {
+    WideTree t;
+    t.width = width;
+    t.node_count = nnodes;
+    const uint8_t* b = static_cast<const uint8_t*>(wnodes);
+    t.nodes.assign(b, b + size_t(nnodes) * size_t(WideNodeBytes(width)));
+    t.tri_ordered.assign(tri_ordered, tri_ordered + 12 * nprim);
+    TraceCount tc;
+    TraceWideCounted(t, rays, n, flags, hits, threads, tc);
+    if (counts) counts[0] = tc.nodes, counts[1] = tc.tris;
+}
 // SAH of a downloaded tree; root_box6 (lo xyz, hi xyz) is the tree's axis-aligned box.
 double orc_sah_final(const void* fnodes, int64_t nnodes, const float* root_box6, int32_t sobb_kdop) {
     FinalTree t;

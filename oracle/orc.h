@@ -20,6 +20,9 @@
 //                                                 tiny inputs (no reference implementation
 //                                                 exists: parity vs the reference unpinned)
 //   watertight test, traversal                    pinned by FP64 intersection + brute force
+//   wide trees (refit, widen, traversal)          pinned by exact refit of AABB trees,
+//                                                 containment/tightness/greedy-rule checks
+//                                                 and hit agreement with the two-child trace
 //   path tracer forward/adjoint                   pinned by the furnace closed form and
 //                                                 FP64 central differences
 #pragma once
@@ -310,6 +313,26 @@ void TraceFinalCounted(const FinalTree& t, const RayRec* rays, int64_t n, uint32
                        TraceCount& total);
 void TraceBrute(const float* tri_xyz, int64_t ntri, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits,
                 int threads);
+
+// ---------------------------------------------------------------------------------------
+// Wide trees (wide_restate.cc): node i of width W is node i of the two-child tree opened
+// greedily to W children; bytes: origin xyz (f32), meta (exponent byte per axis | count << 24),
+// W child references, 6 planes of W bytes (lo x, hi x, lo y, hi y, lo z, hi z); 64 B for
+// W = 4 (8 B zero tail), 96 B for W = 8. Child k's bound = fmaf(q, 2^(e - 127), origin).
+// ---------------------------------------------------------------------------------------
+struct WideTree {
+    int width = 4;
+    int64_t node_count = 0;
+    std::vector<uint8_t> nodes;      // node_count * WideNodeBytes(width)
+    std::vector<float> tri_ordered;  // as FinalTree::tri_ordered
+};
+int WideNodeBytes(int width);
+void RefitAabb(const FinalTree& t, std::vector<FinalTree::Node>& out);
+void Widen(const std::vector<FinalTree::Node>& nodes, int width, std::vector<uint8_t>& out);
+void DecodeWideChild(const uint8_t* node, int width, int k, float lo[3], float hi[3]);
+void TraceWideOne(const WideTree& t, const RayRec& r, uint32_t flags, HitRec& h, TraceCount* count = nullptr);
+void TraceWideCounted(const WideTree& t, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads,
+                      TraceCount& total);
 double IntersectFp64(const float* tri_xyz9, const RayRec& r, double& b1, double& b2);
 
 // ---------------------------------------------------------------------------------------

@@ -65,6 +65,10 @@ def _load():
         "orc_trace_final": (None, [P, I64, P, I64, P, I64, U32, P, I32, I32]),
         "orc_trace_final_counted": (None, [P, I64, P, I64, P, I64, U32, P, I32, I32, P]),
         "orc_sah_final": (D, [P, I64, P, I32]),
+        "orc_refit_aabb": (I32, [P, I64, P, I64, P]),
+        "orc_widen": (I32, [P, I64, I32, P]),
+        "orc_wide_node_bytes": (I32, [I32]),
+        "orc_trace_wide": (None, [P, I64, I32, P, I64, P, I64, U32, P, I32, P]),
         "orc_trace_brute": (None, [P, I64, P, I64, U32, P, I32]),
         "orc_pcg_stream": (None, [U64, U64, I32, P]),
         "orc_sincos2pi": (None, [F, P, P]),
@@ -181,6 +185,41 @@ def trace_final_counted(fnodes, tri_ordered, rays, flags=0, threads=None, sobb_k
     counts = np.zeros(2, np.int64)
     lib.orc_trace_final_counted(_p(fn), fn.shape[0], _p(to), to.size // 12, _p(r), r.shape[0], flags, _p(hits),
                                 threads or nthreads(), sobb_kdop, _p(counts))
+    return hits, int(counts[0]), int(counts[1])
+
+
+def refit_aabb(fnodes, tri_ordered) -> np.ndarray:
+    """Axis-aligned child boxes of every node of a final tree (same references, zero pad)."""
+    fn = np.ascontiguousarray(fnodes)
+    to = _f32(tri_ordered)
+    out = np.zeros(fn.shape[0], FNODE_DTYPE)
+    if lib.orc_refit_aabb(_p(fn), fn.shape[0], _p(to), to.size // 12, _p(out)) != 0:
+        raise ValueError("tree too deep")
+    return out
+
+
+def wide_node_bytes(width: int) -> int:
+    return int(lib.orc_wide_node_bytes(width))
+
+
+def widen(aabb_fnodes, width: int) -> np.ndarray:
+    """Wide nodes (uint8 rows of wide_node_bytes(width)) from AABB final nodes."""
+    fn = np.ascontiguousarray(aabb_fnodes)
+    out = np.zeros((fn.shape[0], wide_node_bytes(width)), np.uint8)
+    if lib.orc_widen(_p(fn), fn.shape[0], width, _p(out)) != 0:
+        raise ValueError("width must be 4 or 8")
+    return out
+
+
+def trace_wide(wnodes, width, tri_ordered, rays, flags=0, threads=None):
+    """(hits, wide nodes visited, triangle tests) over a wide tree."""
+    wn = np.ascontiguousarray(wnodes, dtype=np.uint8)
+    to = _f32(tri_ordered)
+    r = _f32(rays)
+    hits = np.zeros(r.shape[0], HIT_DTYPE)
+    counts = np.zeros(2, np.int64)
+    lib.orc_trace_wide(_p(wn), wn.shape[0], width, _p(to), to.size // 12, _p(r), r.shape[0], flags, _p(hits),
+                       threads or nthreads(), _p(counts))
     return hits, int(counts[0]), int(counts[1])
 
 

@@ -209,6 +209,13 @@ void TraceFinalCounted(const FinalTree& T, const RayRec* rays, int64_t n, uint32
     for (const TraceCount& c : per) total.nodes += c.nodes, total.tris += c.tris;
 }
 
+void TraceWideCounted(const WideTree& T, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits, int threads,
+                      TraceCount& total) {
+    std::vector<TraceCount> per(static_cast<size_t>(n));
+    ParallelFor(n, threads, [&](int64_t i) { TraceWideOne(T, rays[i], flags, hits[i], &per[size_t(i)]); });
+    for (const TraceCount& c : per) total.nodes += c.nodes, total.tris += c.tris;
+}
+
 void TraceBrute(const float* tri_xyz, int64_t ntri, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits,
                 int threads) {
     const bool any = flags & 1u, cull = flags & 2u;

@@ -1,0 +1,123 @@
+"""GPU parity of the wide trees: the converted nodes are byte-identical to the oracle's
+conversion of the same two-child tree (SOBB trees through the axis-aligned refit); the wide
+trace gives the oracle's wide traversal bit for bit, counters included, and the two-child
+trace's closest hits; a captured graph replays it; a rebuild drops the wide form."""
+import sys
+
+import numpy as np
+import pytest
+
+from diplodocus_amd import scenes
+
+from .test_wide import SCENES, rays_for
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_bvh(ctx, tris, mt=1, radius=8, leaf=4, bv=0):
+    from diplodocus_amd import accel
+
+    return accel.GpuBvh(ctx).build(tris, accel.make_params(morton_type=mt, radius=radius, max_leaf=leaf, bv_type=bv))
+
+
+def same(a, b):
+    return a.view(np.uint32).tobytes() == b.view(np.uint32).tobytes()
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("bv", [0, 1])
+@pytest.mark.parametrize("width", [4, 8])
+def test_gpu_widen_matches_oracle(oracle, ctx, name, bv, width):
+    say(f"widen {name} bv={bv} width={width}")
+    g = gpu_bvh(ctx, SCENES[name](), bv=bv)
+    info = g.widen(width)
+    d = g.download()
+    assert info["width"] == width and info["node_count"] == len(d["nodes"])
+    assert info["node_bytes"] == oracle.wide_node_bytes(width)
+    w = g.download_wide()
+    ref = oracle.widen(oracle.refit_aabb(d["nodes"], d["tri_ordered"]), width)
+    assert w.tobytes() == ref.tobytes()
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("bv", [0, 1])
+@pytest.mark.parametrize("width", [4, 8])
+def test_gpu_wide_trace_matches_oracle(oracle, ctx, name, bv, width):
+    from diplodocus_amd import _lib
+
+    tris = SCENES[name]()
+    say(f"wide trace {name} bv={bv} width={width}")
+    g = gpu_bvh(ctx, tris, bv=bv)
+    g.widen(width)
+    d = g.download()
+    w = g.download_wide()
+    rays = rays_for(tris, 4000, seed=3 * width + bv)
+    for flags in (0, 1, 2):
+        h, nodes, tests = g.trace_counters(rays, flags | _lib.DPL_TRACE_WIDE)
+        ref, rn, rt = oracle.trace_wide(w, width, d["tri_ordered"], rays, flags)
+        assert same(h, ref) and (nodes, tests) == (rn, rt)
+        if not flags & 1:
+            assert same(h, g.trace(rays, flags))
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_gpu_wide_trace_mixed_scene(oracle, ctx, width):
+    """A 220k-triangle scene of clusters and thin triangles: wide closest hits equal the
+    two-child trace's and brute force; wide nodes visited stay below two-child nodes."""
+    from diplodocus_amd import _lib
+
+    s = scenes.mixed_2m(n_small=200_000, n_thin=20_000)
+    say(f"wide mixed width={width}")
+    for bv in (0, 1):
+        g = gpu_bvh(ctx, s.tris, radius=16, bv=bv)
+        g.widen(width)
+        rays = scenes.random_rays_towards(s.tris, 20_000, seed=width)
+        hw, nw, _ = g.trace_counters(rays, _lib.DPL_TRACE_WIDE)
+        hb, nb, _ = g.trace_counters(rays, 0)
+        assert same(hw, hb) and nw < nb
+        if bv == 1:
+            sub = slice(0, 3000)
+            assert same(hw[sub], oracle.trace_brute(s.tris, rays[sub]))
+
+
+def test_gpu_wide_graph_replays(ctx):
+    import torch
+
+    from diplodocus_amd import _lib, accel
+
+    s = scenes.mixed_2m(n_small=50_000, n_thin=5_000)
+    say("wide graph")
+    g = gpu_bvh(ctx, s.tris, bv=1)
+    g.widen(4)
+    rays = torch.from_numpy(scenes.random_rays_towards(s.tris, 30_000, seed=2)).cuda()
+    hits = torch.zeros((30_000, 4), dtype=torch.float32, device="cuda")
+    eager = g.trace(rays, _lib.DPL_TRACE_WIDE).cpu()
+    gr = accel.TraceGraph(g, rays, hits, flags=_lib.DPL_TRACE_WIDE)
+    for k in range(3):
+        hits.zero_()
+        torch.cuda.synchronize()
+        gr.launch()
+        ctx.synchronize()
+        assert torch.equal(hits.cpu().view(torch.int32), eager.view(torch.int32)), k
+    gr.close()
+
+
+def test_gpu_rebuild_drops_wide(ctx):
+    from diplodocus_amd import _lib, accel
+
+    tris = SCENES["soup"]()
+    say("rebuild drops wide")
+    g = gpu_bvh(ctx, tris)
+    g.widen(8)
+    assert g.wide_info()["width"] == 8
+    g.build(tris, accel.make_params(morton_type=1, radius=8, max_leaf=4))
+    assert g.wide_info()["width"] == 0
+    with pytest.raises(_lib.DplError):
+        g.trace(rays_for(tris, 10, 1), _lib.DPL_TRACE_WIDE)
+    with pytest.raises(_lib.DplError):
+        g.widen(6)
