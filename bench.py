@@ -3,9 +3,10 @@ against the reference's CPU SAH BVH and the CPU baseline timed on the same host.
 
 A step = one PLOC build of the 2,000,002-triangle mixed scene (device-resident input)
 followed by one closest-hit trace of its 4,194,304 rays. `value` is the build rate
-(Mtris/s). Beside it: primary-ray trace rate of the 1024x1024 camera of the 69,668-triangle
-icosphere room, traversal counters, the roofline of the PLOC iteration kernel (the build's
-dominant kernel) and the CPU baseline. Prints one JSON line.
+(Mtris/s). Beside it: the same trace over the BVH's wide forms (4 and 8 children),
+primary-ray trace rate of the 1024x1024 camera of the 69,668-triangle icosphere room,
+traversal counters, the roofline of the PLOC iteration kernel (the build's dominant kernel)
+and the CPU baseline. Prints one JSON line.
 """
 import argparse
 import json
@@ -127,6 +128,33 @@ def main() -> int:
     _, visits, tests = bvh.trace_counters(rays)
     touched = (visits * 64 + tests * 48) / nrays
 
+    # ---- wide trees of the same BVH (4 and 8 children, 8-bit quantized child boxes): the
+    #      conversion (SOBB trees refitted to axis-aligned boxes first) and the closest-hit
+    #      trace of the same rays, whose hits must equal the two-child trace's ----
+    from diplodocus_amd import _lib
+
+    two_child_hits = hits.clone()
+    wide = {}
+    for width in (4, 8):
+        bvh.widen(width)
+        widen_ms = float(np.median([bvh.widen(width)["widen_ms"] for _ in range(3)]))
+        bvh.trace(rays, _lib.DPL_TRACE_WIDE, hits)
+        w0, w1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        w0.record(stream)
+        for _ in range(5):
+            bvh.trace(rays, _lib.DPL_TRACE_WIDE, hits)
+        w1.record(stream)
+        torch.cuda.synchronize()
+        wide_ms = w0.elapsed_time(w1) / 5
+        equal = bool(torch.equal(hits.view(torch.int32), two_child_hits.view(torch.int32)))
+        _, wv, wt = bvh.trace_counters(rays, _lib.DPL_TRACE_WIDE)
+        wide[f"width{width}"] = {"node_bytes": bvh.wide_info()["node_bytes"], "widen_ms": round(widen_ms, 4),
+                                 "trace_ms": round(wide_ms, 4), "trace_mrays_s": round(nrays / (wide_ms * 1e-3) / 1e6, 3),
+                                 "nodes_per_ray": round(wv / nrays, 2), "tris_per_ray": round(wt / nrays, 2),
+                                 "hits_equal_two_child": equal}
+        log(f"wide{width}: widen {widen_ms:.3f} ms, trace {wide_ms:.3f} ms = {nrays / wide_ms / 1e3:.1f} Mrays/s, "
+            f"{wv / nrays:.1f} nodes and {wt / nrays:.1f} tris per ray, hits equal: {equal}")
+
     # ---- roofline of the PLOC iteration kernel: one profiled build (per-launch events) ----
     prof = accel.GpuBvh(ctx).build(tris, accel.make_params(morton_type=mt, radius=args.radius, max_leaf=4,
                                                              bv_type=bv, keep_intermediates=3))
@@ -164,6 +192,17 @@ def main() -> int:
     _, pv, pt = rb.trace_counters(cam_rays)
     log(f"primary rays: {prim_ms:.3f} ms per 1M rays = {prim_mrays:.1f} Mrays/s, {pv / cam_rays.shape[0]:.1f} "
         f"nodes and {pt / cam_rays.shape[0]:.1f} tris per ray")
+    rb.widen(4)
+    for _ in range(3):
+        rb.trace(cam_rays, _lib.DPL_TRACE_WIDE, cam_hits)
+    e0.record(stream)
+    for _ in range(reps):
+        rb.trace(cam_rays, _lib.DPL_TRACE_WIDE, cam_hits)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    prim_wide_mrays = cam_rays.shape[0] / (e0.elapsed_time(e1) / reps * 1e-3) / 1e6
+    wide["primary_width4_mrays_s"] = round(prim_wide_mrays, 3)
+    log(f"primary rays, wide4: {prim_wide_mrays:.1f} Mrays/s")
 
     # ---- path tracing on the same room: 1024x1024 primary + 1 diffuse bounce, and the
     #      differentiable case (512x512, 4 spp, depth 4): forward and adjoint path replay ----
@@ -343,6 +382,7 @@ def main() -> int:
                                "4,194,304 rays from a bounding sphere toward the centre",
                    "variant": args.variant, "radius": args.radius, "max_leaf": 4},
         "trace_mrays_s": round(mrays, 3),
+        "trace_wide4_mrays_s": wide["width4"]["trace_mrays_s"],
         "primary_mrays_s": round(prim_mrays, 3),
         "primary_workload": "icosphere_room 69,668 tris, 1024x1024 camera rays, PLOC Morton30 r=16",
         "primary_bounce_ms": round(bounce_ms, 4),
@@ -360,6 +400,8 @@ def main() -> int:
         "trace_nodes_per_ray": round(visits / nrays, 2),
         "trace_tris_per_ray": round(tests / nrays, 2),
         "trace_touched_bytes_per_ray": round(touched, 1),
+        "wide": dict(wide, workload="the step's BVH and rays; wide trees converted from it (SOBB trees refitted "
+                                    "to axis-aligned boxes), 4 or 8 children, 8-bit quantized child boxes"),
         "wall_s": round(wall, 3),
         "variant_sweep": sweep,
         "roofline": roof,

@@ -204,6 +204,12 @@ class GpuBvh final : public AccelerationStructure {
     void IntersectAnyBatch(Stats& stats, std::span<const Ray> rays, std::span<uint8_t> occluded,
                            bool backface_culling) const;
 
+    // Adds the wide form (4 or 8 children, quantized child boxes; dpl_bvh_widen): every trace
+    // until the next Build traverses it, with the same closest hits. No counterpart in the
+    // reference.
+    void Widen(int width = 4);
+    int wide_width() const { return wide_width_; }
+
     const dpl_build_stats& build_stats() const { return build_stats_; }
     dpl_bvh* handle() const { return bvh_; }
     const std::shared_ptr<Context>& context() const { return ctx_; }
@@ -216,6 +222,7 @@ class GpuBvh final : public AccelerationStructure {
     dpl_bvh* bvh_ = nullptr;
     std::span<const Triangle> triangles_;
     dpl_build_stats build_stats_{};
+    int wide_width_ = 0;
 };
 
 // The factory case a renderer adds for the PLOC types (kPloc, kPlocEmc, kPlocSobb,

@@ -69,6 +69,7 @@ void GpuBvh::Build(const AccelerationStructureConfig& config, Stats& stats, std:
     }
     triangles_ = triangles;
     build_stats_ = bs;
+    wide_width_ = 0;  // a rebuild drops the wide form
     AccelerationStats& a = stats.accel_stats;
     a.build_time = bs.build_ms;
     a.node_count = bs.node_count;
@@ -77,8 +78,16 @@ void GpuBvh::Build(const AccelerationStructureConfig& config, Stats& stats, std:
     a.memory_consumption = int(std::min<int64_t>(bs.memory_consumption, INT32_MAX));
 }
 
+void GpuBvh::Widen(int width) {
+    if (!bvh_) throw std::logic_error("GpuBvh: Build before Widen");
+    if (width != 4 && width != 8) throw std::invalid_argument("GpuBvh::Widen: width must be 4 or 8");
+    ctx_->Check(dpl_bvh_widen(bvh_, width, nullptr));
+    wide_width_ = width;
+}
+
 std::vector<dpl_hit> GpuBvh::Trace(Stats& stats, std::span<const Ray> rays, uint32_t flags) const {
     if (!bvh_) throw std::logic_error("GpuBvh: Build before tracing");
+    if (wide_width_) flags |= DPL_TRACE_WIDE;
     std::vector<dpl_ray> r(rays.size());
     for (size_t i = 0; i < rays.size(); i++) {
         const Ray& q = rays[i];

@@ -354,6 +354,21 @@ void GpuChecks() {
         gpu->IntersectAnyBatch(stats, rays, occ, false);
         for (size_t i = 0; i < rays.size(); i++) bad += occ[i] != (hits[i].triangle_id >= 0);
         Expect(bad == 0, std::string(name) + ": Intersect / IntersectAny agree with the batch");
+        // the wide form (4 and 8 children) gives the same closest hits; a rebuild drops it
+        for (int width : {4, 8}) {
+            gpu->Widen(width);
+            std::vector<RayHit> wh(rays.size());
+            const int64_t wn = gpu->IntersectBatch(stats, rays, wh, false);
+            int64_t wbad = wn != nhit;
+            for (size_t i = 0; i < rays.size(); i++)
+                wbad += wh[i].triangle_id != hits[i].triangle_id || (wh[i].triangle_id >= 0 && wh[i].t != hits[i].t);
+            gpu->IntersectAnyBatch(stats, rays, occ, false);
+            for (size_t i = 0; i < rays.size(); i++) wbad += occ[i] != (hits[i].triangle_id >= 0);
+            Expect(wbad == 0 && gpu->wide_width() == width,
+                   std::string(name) + ": wide" + std::to_string(width) + " hits equal the two-child trace");
+        }
+        as->Build(cfg, stats, tris);
+        Expect(gpu->wide_width() == 0, std::string(name) + ": a rebuild drops the wide form");
         std::printf("%s: %lld hits of %zu rays, build %.3f ms, %d nodes\n", name, (long long)nhit, rays.size(),
                     stats.accel_stats.build_time, stats.accel_stats.node_count);
         std::fflush(stdout);
