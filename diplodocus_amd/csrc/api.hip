@@ -291,8 +291,9 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     b.status[0] = C.status0.ensure<uint64_t>(size_t(status_words) * 8);
     b.status[1] = C.status1.ensure<uint64_t>(size_t(status_words) * 8);
     {
-        uint8_t* sm = C.small.ensure<uint8_t>(1024);
+        uint8_t* sm = C.small.ensure<uint8_t>(1024 + size_t(kBoundsBlocks) * 64);
         b.bounds = reinterpret_cast<uint32_t*>(sm);              // 13 words
+        b.bounds_part = reinterpret_cast<uint32_t*>(sm + 1024);  // kBoundsBlocks x 16
         b.tile_counter = reinterpret_cast<uint32_t*>(sm + 64);   // 2
         b.iter_done = reinterpret_cast<int32_t*>(sm + 96);       // 1
         b.counters = reinterpret_cast<int32_t*>(sm + 128);       // 8
@@ -1083,8 +1084,9 @@ dpl_status dpl_compute_codes(dpl_context* ctx, const float* tri, int64_t n, int3
         b.tri = tri;
         b.keys = codes;
         b.vals = ctx->vals.ensure<uint32_t>(size_t(n) * 4);
-        uint8_t* sm = ctx->small.ensure<uint8_t>(1024);
+        uint8_t* sm = ctx->small.ensure<uint8_t>(1024 + size_t(kBoundsBlocks) * 64);
         b.bounds = reinterpret_cast<uint32_t*>(sm);
+        b.bounds_part = reinterpret_cast<uint32_t*>(sm + 1024);
         // the init kernel touches these; give it valid scratch
         b.tile_counter = reinterpret_cast<uint32_t*>(sm + 64);
         b.iter_done = reinterpret_cast<int32_t*>(sm + 96);

@@ -54,7 +54,7 @@ hipError_t LaunchBuildInit(const BuildBuffers& b, int block, hipStream_t s) {
 // ---------------------------------------------------------------------------------------
 // bounds: vertex and centroid min/max, exact in any order; non-finite input flagged
 // ---------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void KBounds(const float* __restrict__ tri, int64_t n, uint32_t* bounds) {
+__global__ __launch_bounds__(256) void KBounds(const float* __restrict__ tri, int64_t n, uint32_t* __restrict__ part) {
     float mn[6], mx[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) {
@@ -93,6 +93,10 @@ __global__ __launch_bounds__(256) void KBounds(const float* __restrict__ tri, in
     __syncthreads();
     if (bad) atomicOr(&s_bad, 1);
     __syncthreads();
+    // the workgroup's partial (slots: vmin 0-2, vmax 3-5, cmin 6-8, cmax 9-11, flag 12) goes
+    // to its own row; KBoundsReduce combines the rows after the kernel boundary, which costs
+    // far less than thousands of same-address atomics
+    uint32_t* row = part + 16 * int64_t(blockIdx.x);
     if (threadIdx.x < 6) {
         int k = threadIdx.x;
         float a = s_mn[k][0], z = s_mx[k][0];
@@ -100,18 +104,42 @@ __global__ __launch_bounds__(256) void KBounds(const float* __restrict__ tri, in
             a = fminf(a, s_mn[k][w]);
             z = fmaxf(z, s_mx[k][w]);
         }
-        // slots: vmin 0-2, vmax 3-5, cmin 6-8, cmax 9-11
         int base = k < 3 ? 0 : 6, ax = k % 3;
-        atomicMin(&bounds[base + ax], FloatToOrdered(a));
-        atomicMax(&bounds[base + 3 + ax], FloatToOrdered(z));
+        row[base + ax] = FloatToOrdered(a);
+        row[base + 3 + ax] = FloatToOrdered(z);
     }
-    if (threadIdx.x == 0 && s_bad) atomicOr(&bounds[12], 1u);
+    if (threadIdx.x == 0) row[12] = s_bad ? 1u : 0u;
+}
+
+// One workgroup: min / max (ordered integers) and OR of the flag over the partial rows.
+__global__ __launch_bounds__(256) void KBoundsReduce(const uint32_t* __restrict__ part, int rows,
+                                                     uint32_t* __restrict__ bounds) {
+    uint32_t v[13];
+    for (int k = 0; k < 13; k++) v[k] = (k % 6) < 3 && k < 12 ? 0xFFFFFFFFu : 0u;
+    for (int r = threadIdx.x; r < rows; r += 256) {
+        const uint32_t* row = part + 16 * int64_t(r);
+        for (int k = 0; k < 12; k++) v[k] = (k % 6) < 3 ? min(v[k], row[k]) : max(v[k], row[k]);
+        v[12] |= row[12];
+    }
+    __shared__ uint32_t s_v[13][256];
+    for (int k = 0; k < 13; k++) s_v[k][threadIdx.x] = v[k];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (int(threadIdx.x) < h)
+            for (int k = 0; k < 13; k++) {
+                const uint32_t a = s_v[k][threadIdx.x], b = s_v[k][threadIdx.x + h];
+                s_v[k][threadIdx.x] = k == 12 ? (a | b) : ((k % 6) < 3 ? min(a, b) : max(a, b));
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x < 13) bounds[threadIdx.x] = s_v[threadIdx.x][0];
 }
 
 hipError_t LaunchBounds(const BuildBuffers& b, hipStream_t s) {
-    int grid = int(std::min<int64_t>(2048, (b.n + 255) / 256));
+    int grid = int(std::min<int64_t>(kBoundsBlocks, (b.n + 255) / 256));
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(KBounds, dim3(grid), dim3(256), 0, s, b.tri, b.n, b.bounds);
+    hipLaunchKernelGGL(KBounds, dim3(grid), dim3(256), 0, s, b.tri, b.n, b.bounds_part);
+    hipLaunchKernelGGL(KBoundsReduce, dim3(1), dim3(256), 0, s, b.bounds_part, grid, b.bounds);
     return hipGetLastError();
 }
 
@@ -139,7 +167,8 @@ __global__ __launch_bounds__(256) void KCodes(const float* __restrict__ tri, int
 }
 
 hipError_t LaunchCodes(const BuildBuffers& b, int morton_type, hipStream_t s) {
-    int grid = int(std::min<int64_t>(4096, (b.n + 255) / 256));
+    // EMC60 workgroups each build the code tables first: fewer, longer-lived workgroups
+    int grid = int(std::min<int64_t>(morton_type == 1 ? 1024 : 4096, (b.n + 255) / 256));
     if (grid < 1) grid = 1;
     if (morton_type == 0)
         hipLaunchKernelGGL(KCodes<0>, dim3(grid), dim3(256), 0, s, b.tri, b.n, b.bounds, b.keys, b.vals);

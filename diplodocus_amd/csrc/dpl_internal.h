@@ -206,6 +206,7 @@ struct BuildBuffers {
     int64_t n;
     const float* tri;      // n x 9 (device)
     uint32_t* bounds;      // 13: ordered vmin3 vmax3 cmin3 cmax3, error flag
+    uint32_t* bounds_part; // kBoundsBlocks x 16: per-workgroup partial bounds (same order)
     uint64_t* keys;        // n (u32 view for Morton30)
     uint32_t* vals;        // n
     uint64_t* keys_alt;    // n
@@ -238,6 +239,7 @@ struct BuildBuffers {
 };
 
 hipError_t LaunchBuildInit(const BuildBuffers& b, int block, hipStream_t s);
+constexpr int kBoundsBlocks = 2048;
 hipError_t LaunchBounds(const BuildBuffers& b, hipStream_t s);
 hipError_t LaunchCodes(const BuildBuffers& b, int morton_type, hipStream_t s);
 hipError_t LaunchInitClusters(const BuildBuffers& b, hipStream_t s);
