@@ -19,7 +19,7 @@
 
 namespace dpl {
 
-hipError_t LaunchSah(const FNode* nodes, const int32_t* counters, double* out, const float* sobb_dirs,
+hipError_t LaunchSah(const FNode* nodes, const int32_t* counters, double* part, const float* sobb_dirs,
                      hipStream_t s);
 
 // SOBB direction set (same construction as the oracle): the 13 directions of the 26-DOP or a
@@ -413,7 +413,8 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     DPL_HIP(hipEventRecord(C.ev[5], s));
     DPL_HIP(LaunchLayout(b, p.max_triangles_per_leaf, s));
     DPL_HIP(hipEventRecord(C.ev[6], s));
-    DPL_HIP(LaunchSah(b.fnodes, b.counters, b.sah, b.ndirs > 0 ? b.dirs : nullptr, s));
+    DPL_HIP(LaunchSah(b.fnodes, b.counters + 5, reinterpret_cast<double*>(b.bounds_part),
+                      b.ndirs > 0 ? b.dirs : nullptr, s));
     DPL_HIP(LaunchFinish(b, s));
     DPL_HIP(hipEventRecord(C.ev[7], s));
     if (p.keep_intermediates) {

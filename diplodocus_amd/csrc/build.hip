@@ -111,28 +111,40 @@ __global__ __launch_bounds__(256) void KBounds(const float* __restrict__ tri, in
     if (threadIdx.x == 0) row[12] = s_bad ? 1u : 0u;
 }
 
-// One workgroup: min / max (ordered integers) and OR of the flag over the partial rows.
+// One workgroup: min / max (ordered integers) and OR of the flag over the partial rows;
+// each thread folds its rows, then the waves by shuffles, then the four wave results.
+__device__ __forceinline__ uint32_t FoldBound(int k, uint32_t a, uint32_t b) {
+    return k == 12 ? (a | b) : ((k % 6) < 3 ? min(a, b) : max(a, b));
+}
+
 __global__ __launch_bounds__(256) void KBoundsReduce(const uint32_t* __restrict__ part, int rows,
                                                      uint32_t* __restrict__ bounds) {
     uint32_t v[13];
+#pragma unroll
     for (int k = 0; k < 13; k++) v[k] = (k % 6) < 3 && k < 12 ? 0xFFFFFFFFu : 0u;
     for (int r = threadIdx.x; r < rows; r += 256) {
-        const uint32_t* row = part + 16 * int64_t(r);
-        for (int k = 0; k < 12; k++) v[k] = (k % 6) < 3 ? min(v[k], row[k]) : max(v[k], row[k]);
-        v[12] |= row[12];
+        const uint4* row = reinterpret_cast<const uint4*>(part + 16 * int64_t(r));
+        const uint4 a = row[0], b = row[1], c = row[2], d = row[3];
+        const uint32_t x[13] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w, c.x, c.y, c.z, c.w, d.x};
+#pragma unroll
+        for (int k = 0; k < 13; k++) v[k] = FoldBound(k, v[k], x[k]);
     }
-    __shared__ uint32_t s_v[13][256];
-    for (int k = 0; k < 13; k++) s_v[k][threadIdx.x] = v[k];
+#pragma unroll
+    for (int k = 0; k < 13; k++)
+#pragma unroll
+        for (int d = kWave / 2; d > 0; d >>= 1) v[k] = FoldBound(k, v[k], uint32_t(__shfl_xor(int(v[k]), d, kWave)));
+    __shared__ uint32_t s_v[4][13];
+    const int w = threadIdx.x / kWave;
+    if (LaneId() == 0)
+#pragma unroll
+        for (int k = 0; k < 13; k++) s_v[w][k] = v[k];
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
-        if (int(threadIdx.x) < h)
-            for (int k = 0; k < 13; k++) {
-                const uint32_t a = s_v[k][threadIdx.x], b = s_v[k][threadIdx.x + h];
-                s_v[k][threadIdx.x] = k == 12 ? (a | b) : ((k % 6) < 3 ? min(a, b) : max(a, b));
-            }
-        __syncthreads();
+    if (threadIdx.x < 13) {
+        const int k = threadIdx.x;
+        uint32_t r = s_v[0][k];
+        for (int ww = 1; ww < 4; ww++) r = FoldBound(k, r, s_v[ww][k]);
+        bounds[k] = r;
     }
-    if (threadIdx.x < 13) bounds[threadIdx.x] = s_v[threadIdx.x][0];
 }
 
 hipError_t LaunchBounds(const BuildBuffers& b, hipStream_t s) {
@@ -428,22 +440,13 @@ hipError_t LaunchPlocTail(const BuildBuffers& b, int iter, int radius, int max_l
 }
 
 
+// The kept-node count K is the total of the keptflag scan that follows (counters[5]); the
+// final tree is a full binary tree, so it has K + 1 leaves. No counting atomics here.
 __global__ __launch_bounds__(256) void KClassify(BuildBuffers b, int max_leaf) {
     const int64_t n = b.n, nn = 2 * n - 1;
     int kept = 0, leaves = 0;
     for (int64_t id = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; id < nn; id += int64_t(gridDim.x) * blockDim.x)
         ClassifyNode(b, int32_t(id), max_leaf, kept, leaves);
-    // block-reduce the two counters
-    __shared__ int s_k, s_l;
-    if (threadIdx.x == 0) s_k = s_l = 0;
-    __syncthreads();
-    if (kept) atomicAdd(&s_k, kept);
-    if (leaves) atomicAdd(&s_l, leaves);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (s_k) atomicAdd(&b.counters[0], s_k);
-        if (s_l) atomicAdd(&b.counters[1], s_l);
-    }
 }
 
 // Places every triangle in its final leaf, and writes the node of every kept inner node
@@ -451,7 +454,7 @@ __global__ __launch_bounds__(256) void KClassify(BuildBuffers b, int max_leaf) {
 // in place into exclusive ranks and leafslot into begin offsets.
 __global__ __launch_bounds__(256) void KEmit(BuildBuffers b) {
     const int64_t n = b.n, nn = 2 * n - 1;
-    const int32_t K = b.counters[0];
+    const int32_t K = b.counters[5];
     for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x)
         EmitTriangle(b, int32_t(k));
     const int64_t first = n > 1 ? n : 0;
@@ -464,7 +467,7 @@ __global__ __launch_bounds__(256) void KEmit(BuildBuffers b) {
 // At least four waves per SIMD: the unrolled 26-DOP triple scan would otherwise be
 // scheduled for instruction-level parallelism at the cost of every register.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void KFitSobb(BuildBuffers b) {
-    const int32_t K = b.counters[0];
+    const int32_t K = b.counters[5];
     __shared__ int4 s_tri[kMaxTriples];
     __shared__ float s_w[kMaxDirs * 256];
     for (int t = threadIdx.x; t < b.ntriples; t += blockDim.x) s_tri[t] = b.triples[t];
@@ -477,7 +480,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void K
 
 // SAH of the final tree in FP64 (Ct = Ci = 1, divided by SA(root) on the host); the node
 // count is the number of kept inner nodes (0 when the whole scene is one leaf).
-__global__ __launch_bounds__(256) void KSah(const FNode* nodes, const int32_t* counters, double* out,
+__global__ __launch_bounds__(256) void KSah(const FNode* nodes, const int32_t* counters, double* part,
                                             const float* dirs) {
     const int32_t count = counters[0];
     double acc = 0.0;
@@ -490,7 +493,7 @@ __global__ __launch_bounds__(256) void KSah(const FNode* nodes, const int32_t* c
         if (int(threadIdx.x) < d) s[threadIdx.x] += s[threadIdx.x + d];
         __syncthreads();
     }
-    if (threadIdx.x == 0 && s[0] != 0.0) atomicAdd(out, s[0]);
+    if (threadIdx.x == 0) part[blockIdx.x] = s[0];
 }
 
 hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s) {
@@ -516,12 +519,23 @@ hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Copies the build's small results into host-mapped memory (one thread, after the SAH sum).
-__global__ void KFinish(BuildBuffers b) {
+// Sums the SAH partials of KSah in a fixed order (deterministic, no same-address atomics),
+// then copies the build's small results into host-mapped memory.
+__global__ __launch_bounds__(256) void KFinish(BuildBuffers b) {
+    const double* part = reinterpret_cast<const double*>(b.bounds_part);
+    double acc = 0.0;
+    for (int r = threadIdx.x; r < kSahBlocks; r += 256) acc += part[r];
+#pragma unroll
+    for (int d = kWave / 2; d > 0; d >>= 1) acc += __shfl_xor(acc, d, kWave);
+    __shared__ double s_w[4];
+    if (LaneId() == 0) s_w[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    b.sah[0] = (s_w[0] + s_w[1]) + (s_w[2] + s_w[3]);
     BuildResults* r = b.results;
     const int64_t root = 2 * b.n - 2;
-    r->kept = b.counters[0];
-    r->leaves = b.counters[1];
+    r->kept = b.counters[5];
+    r->leaves = b.counters[5] + 1;
     r->root_height = b.eval[root].height;
     r->sah = b.sah[0];
     r->flags = b.bounds[12];
@@ -533,13 +547,14 @@ __global__ void KFinish(BuildBuffers b) {
 }
 
 hipError_t LaunchFinish(const BuildBuffers& b, hipStream_t s) {
-    hipLaunchKernelGGL(KFinish, dim3(1), dim3(1), 0, s, b);
+    hipLaunchKernelGGL(KFinish, dim3(1), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 
-hipError_t LaunchSah(const FNode* nodes, const int32_t* counters, double* out, const float* sobb_dirs,
+// Partial SAH sums per workgroup into part[kSahBlocks]; KFinish adds them up.
+hipError_t LaunchSah(const FNode* nodes, const int32_t* counters, double* part, const float* sobb_dirs,
                      hipStream_t s) {
-    hipLaunchKernelGGL(KSah, dim3(512), dim3(256), 0, s, nodes, counters, out, sobb_dirs);
+    hipLaunchKernelGGL(KSah, dim3(kSahBlocks), dim3(256), 0, s, nodes, counters, part, sobb_dirs);
     return hipGetLastError();
 }
 

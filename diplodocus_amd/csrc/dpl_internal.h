@@ -224,7 +224,7 @@ struct BuildBuffers {
     int32_t* leafslot;     // n: count of the final leaf whose minimum sorted position is k
     int32_t* keptflag;     // n: internal node kept as a final inner node
     int32_t* scan_tmp;     // scan block sums
-    int32_t* counters;     // 8: [0] kept inner, [1] final leaves, [2] max height
+    int32_t* counters;     // 8: [4] triangles in leaves, [5] kept inner nodes (scan totals)
     double* sah;           // 1
     FNode* fnodes;         // n (capacity)
     int2* kept_child;      // n: PLOC ids of the children of each final inner node (SOBB trees; else null)
@@ -240,6 +240,7 @@ struct BuildBuffers {
 
 hipError_t LaunchBuildInit(const BuildBuffers& b, int block, hipStream_t s);
 constexpr int kBoundsBlocks = 2048;
+constexpr int kSahBlocks = 512;  // SAH partials reuse the bounds partial rows (kSahBlocks doubles fit)
 hipError_t LaunchBounds(const BuildBuffers& b, hipStream_t s);
 hipError_t LaunchCodes(const BuildBuffers& b, int morton_type, hipStream_t s);
 hipError_t LaunchInitClusters(const BuildBuffers& b, hipStream_t s);
