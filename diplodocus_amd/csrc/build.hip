@@ -14,7 +14,7 @@ namespace dpl {
 
 namespace {
 
-constexpr int kTail = 2048;        // clusters finished by the single-workgroup tail
+constexpr int kTail = 1024;        // clusters finished by the single-workgroup tail
 constexpr int kTailThreads = 1024;
 constexpr int kIpt = 2;            // at most this many clusters per thread in an iteration tile
 
