@@ -1,7 +1,7 @@
 // Ray-batch trace kernel for gfx950: persistent grid, rays fetched 64 at a time per wave
-// from a global atomic counter, a 16-entry LDS stack ring per lane with a global spill
-// region for deeper stacks (see traverse.h), AABB or SOBB child boxes, optional
-// traversal counters.
+// from a global atomic counter, an LDS stack ring per lane (16 entries for AABB trees, 4 for
+// SOBB trees) with a global spill region for deeper stacks (see traverse.h), AABB or SOBB
+// child boxes, optional traversal counters.
 #include <hip/hip_runtime.h>
 
 #include "dpl_internal.h"
@@ -11,6 +11,10 @@ namespace dpl {
 
 namespace {
 constexpr int kStackLds = 16;
+// SOBB kernels keep n.o and 1/(n.d) of all 13 directions per lane in LDS (104 B); a shorter
+// LDS stack ring keeps more workgroups resident per CU, which pays more than the extra spills.
+constexpr int kStackLdsSobb = 4;
+constexpr bool kSobbSpillInterleaved = true;
 constexpr int kTraceBlock = 256;
 constexpr int kTraceWgPerCu = 8;
 }  // namespace
@@ -23,7 +27,8 @@ __global__ __launch_bounds__(BLOCK) void KTrace(TraceTree tree, const float4* __
                                                 int32_t* spill, int32_t spill_stride,
                                                 unsigned long long* counters, const int32_t* __restrict__ ray_mesh,
                                                 const BatchTree* __restrict__ mt, int32_t mesh_count) {
-    __shared__ int32_t s_stack[kStackLds][BLOCK];
+    constexpr int S = SOBB ? kStackLdsSobb : kStackLds;
+    __shared__ int32_t s_stack[S][BLOCK];
     __shared__ uint32_t s_base[BLOCK / kWave];
     __shared__ float s_on[SOBB ? kMaxDirs : 1][BLOCK];
     __shared__ float s_idn[SOBB ? kMaxDirs : 1][BLOCK];
@@ -34,46 +39,55 @@ __global__ __launch_bounds__(BLOCK) void KTrace(TraceTree tree, const float4* __
     }
     const bool any = flags & DPL_TRACE_ANY, cull = flags & DPL_TRACE_CULL_BACK;
     const int lane = LaneId(), wid = threadIdx.x / kWave;
-    LaneStack<kStackLds, BLOCK> st{&s_stack[0][threadIdx.x],
-                                   spill + (int64_t(blockIdx.x) * BLOCK + threadIdx.x) * spill_stride, 0};
     SobbRay sr{&s_on[0][threadIdx.x], &s_idn[0][threadIdx.x], 0.0f};
     TraceCounters cnt{0u, 0u};
-    for (;;) {
-        if (lane == 0) s_base[wid] = atomicAdd(counter, uint32_t(kWave));
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int64_t base = s_base[wid];
-        if (base >= n) break;
-        const int64_t i = base + lane;
-        if (i < n) {
-            const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
-            const WtRay w = PrepareRay(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w);
-            if (SOBB) PrepareSobb<BLOCK>(sr, s_dirs, tree.ndirs, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z);
-            TraceTree t = tree;
-            bool valid = true;
-            if (BATCH) {
-                const int32_t m = ray_mesh[i];
-                valid = m >= 0 && m < mesh_count;
-                if (valid) {
-                    const BatchTree bt = mt[m];
-                    t.nodes = tree.nodes + bt.node_off;
-                    t.tris = tree.tris + 3 * bt.tri_off;
-                    t.node_count = bt.node_count;
-                    t.prim_count = bt.prim_count;
-                    t.height = bt.height;
+    auto run = [&](auto& st) {
+        for (;;) {
+            if (lane == 0) s_base[wid] = atomicAdd(counter, uint32_t(kWave));
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int64_t base = s_base[wid];
+            if (base >= n) break;
+            const int64_t i = base + lane;
+            if (i < n) {
+                const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+                const WtRay w = PrepareRay(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w);
+                if (SOBB) PrepareSobb<BLOCK>(sr, s_dirs, tree.ndirs, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z);
+                TraceTree t = tree;
+                bool valid = true;
+                if (BATCH) {
+                    const int32_t m = ray_mesh[i];
+                    valid = m >= 0 && m < mesh_count;
+                    if (valid) {
+                        const BatchTree bt = mt[m];
+                        t.nodes = tree.nodes + bt.node_off;
+                        t.tris = tree.tris + 3 * bt.tri_off;
+                        t.node_count = bt.node_count;
+                        t.prim_count = bt.prim_count;
+                        t.height = bt.height;
+                    }
                 }
+                HitResult h{0.0f, 0.0f, 0.0f, -1, -1};
+                if (valid)
+                    h = any ? Traverse<true, SOBB>(t, w, sr, cull, st, cnt) : Traverse<false, SOBB>(t, w, sr, cull, st, cnt);
+                dpl_hit o;
+                o.t = h.t;
+                o.b1 = h.b1;
+                o.b2 = h.b2;
+                o.prim = h.prim;
+                hits[i] = o;
             }
-            HitResult h{0.0f, 0.0f, 0.0f, -1, -1};
-            if (valid)
-                h = any ? Traverse<true, SOBB>(t, w, sr, cull, st, cnt) : Traverse<false, SOBB>(t, w, sr, cull, st, cnt);
-            dpl_hit o;
-            o.t = h.t;
-            o.b1 = h.b1;
-            o.b2 = h.b2;
-            o.prim = h.prim;
-            hits[i] = o;
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_wave_barrier();
+    };
+    if constexpr (SOBB && kSobbSpillInterleaved) {
+        LaneStackInterleaved<S, BLOCK> st{&s_stack[0][threadIdx.x], spill + int64_t(blockIdx.x) * BLOCK + threadIdx.x,
+                                          int64_t(gridDim.x) * BLOCK, 0};
+        run(st);
+    } else {
+        LaneStack<S, BLOCK> st{&s_stack[0][threadIdx.x],
+                               spill + (int64_t(blockIdx.x) * BLOCK + threadIdx.x) * spill_stride, 0};
+        run(st);
     }
     if (COUNT) {
         unsigned long long a = cnt.nodes, b = cnt.tris;
@@ -104,7 +118,9 @@ int TraceGrid(int cu_count, int64_t n) {
     return int(std::max<int64_t>(1, std::min<int64_t>(int64_t(cu_count) * kTraceWgPerCu, need)));
 }
 
-int32_t TraceSpillStride(int height) { return height >= kStackLds ? height - kStackLds + 1 : 0; }
+// Per-lane spill entries beyond the shortest LDS ring (the SOBB kernels'); the other kernels'
+// longer rings need fewer, so one size serves every trace and render kernel.
+int32_t TraceSpillStride(int height) { return height >= kStackLdsSobb ? height - kStackLdsSobb + 1 : 0; }
 
 size_t TraceSpillBytes(int cu_count, int height) {
     return size_t(cu_count) * kTraceWgPerCu * kTraceBlock * size_t(TraceSpillStride(height)) * 4;
