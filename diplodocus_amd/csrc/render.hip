@@ -101,7 +101,6 @@ __global__ __launch_bounds__(BLOCK) void KRender(RenderArgs a, uint32_t* counter
     __shared__ int32_t s_stack[kRenderStack][BLOCK];
     __shared__ uint32_t s_base[BLOCK / kWave];
     __shared__ float s_grad[ADJ ? 6 * kMaxMaterials : 1];
-    __shared__ float s_on[SOBB ? kMaxDirs : 1][BLOCK];
     __shared__ float s_idn[SOBB ? kMaxDirs : 1][BLOCK];
     __shared__ float s_dirs[SOBB ? 3 * kMaxDirs : 1];
     if (SOBB) {
@@ -111,7 +110,7 @@ __global__ __launch_bounds__(BLOCK) void KRender(RenderArgs a, uint32_t* counter
     const int lane = LaneId(), wid = threadIdx.x / kWave;
     LaneStack<kRenderStack, BLOCK> st{&s_stack[0][threadIdx.x],
                                       a.spill + (int64_t(blockIdx.x) * BLOCK + threadIdx.x) * a.spill_stride, 0};
-    SobbRay sr{&s_on[0][threadIdx.x], &s_idn[0][threadIdx.x], 0.0f};
+    SobbRay sr{s_dirs, &s_idn[0][threadIdx.x], 0.0f, 0.0f, 0.0f, 0.0f};
     TraceCounters cnt{0u, 0u};
     if (ADJ) {
         for (int i = threadIdx.x; i < 6 * a.M; i += BLOCK) s_grad[i] = 0.0f;

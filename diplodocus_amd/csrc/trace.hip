@@ -1,5 +1,5 @@
 // Ray-batch trace kernel for gfx950: persistent grid, rays fetched 64 at a time per wave
-// from a global atomic counter, an LDS stack ring per lane (16 entries for AABB trees, 4 for
+// from a global atomic counter, an LDS stack ring per lane (16 entries for AABB trees, 8 for
 // SOBB trees) with a global spill region for deeper stacks (see traverse.h), AABB or SOBB
 // child boxes, optional traversal counters.
 #include <hip/hip_runtime.h>
@@ -11,9 +11,10 @@ namespace dpl {
 
 namespace {
 constexpr int kStackLds = 16;
-// SOBB kernels keep n.o and 1/(n.d) of all 13 directions per lane in LDS (104 B); a shorter
-// LDS stack ring keeps more workgroups resident per CU, which pays more than the extra spills.
-constexpr int kStackLdsSobb = 4;
+// SOBB kernels keep 1/(n.d) of all 13 directions per lane in LDS (52 B, n.o is recomputed);
+// a shorter LDS stack ring keeps more workgroups resident per CU, which pays more than the
+// extra spills.
+constexpr int kStackLdsSobb = 8;
 constexpr bool kSobbSpillInterleaved = true;
 constexpr int kTraceBlock = 256;
 constexpr int kTraceWgPerCu = 8;
@@ -30,7 +31,6 @@ __global__ __launch_bounds__(BLOCK) void KTrace(TraceTree tree, const float4* __
     constexpr int S = SOBB ? kStackLdsSobb : kStackLds;
     __shared__ int32_t s_stack[S][BLOCK];
     __shared__ uint32_t s_base[BLOCK / kWave];
-    __shared__ float s_on[SOBB ? kMaxDirs : 1][BLOCK];
     __shared__ float s_idn[SOBB ? kMaxDirs : 1][BLOCK];
     __shared__ float s_dirs[SOBB ? 3 * kMaxDirs : 1];
     if (SOBB) {
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(BLOCK) void KTrace(TraceTree tree, const float4* __
     }
     const bool any = flags & DPL_TRACE_ANY, cull = flags & DPL_TRACE_CULL_BACK;
     const int lane = LaneId(), wid = threadIdx.x / kWave;
-    SobbRay sr{&s_on[0][threadIdx.x], &s_idn[0][threadIdx.x], 0.0f};
+    SobbRay sr{s_dirs, &s_idn[0][threadIdx.x], 0.0f, 0.0f, 0.0f, 0.0f};
     TraceCounters cnt{0u, 0u};
     auto run = [&](auto& st) {
         for (;;) {

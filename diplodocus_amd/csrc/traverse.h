@@ -98,13 +98,15 @@ __device__ __forceinline__ bool Box(const WtRay& w, float tfar_limit, float x0, 
     return tn <= tf * 1.0000004f;
 }
 
-// SOBB slab test. Per ray and direction k of the set: n_k . o and 1 / (n_k . d), kept in
-// the lane's LDS columns (on[k * BLOCK], idn[k * BLOCK]); po = 4e-7 |o|_1 bounds the
-// rounding of n_k . o and pads every slab.
+// SOBB slab test. Per ray and direction k of the set: 1 / (n_k . d) in the lane's LDS column
+// (idn[k * BLOCK]); n_k . o is recomputed from the workgroup's direction table (LDS, ndirs x
+// 3) and the ray origin at each test, left to right without FMA (the oracle's order), which
+// halves the per-lane LDS against keeping it. po = 4e-7 |o|_1 bounds the rounding of n_k . o
+// and pads every slab.
 struct SobbRay {
-    float* on;
+    const float* dirs;
     float* idn;
-    float po;
+    float ox, oy, oz, po;
 };
 
 template <int BLOCK>
@@ -112,19 +114,21 @@ __device__ __forceinline__ void PrepareSobb(SobbRay& s, const float* dirs, int n
                                             float dx, float dy, float dz) {
     for (int k = 0; k < ndirs; k++) {
         const float nx = dirs[3 * k], ny = dirs[3 * k + 1], nz = dirs[3 * k + 2];
-        s.on[k * BLOCK] = nx * ox + ny * oy + nz * oz;
         s.idn[k * BLOCK] = 1.0f / (nx * dx + ny * dy + nz * dz);
     }
+    s.dirs = dirs;
+    s.ox = ox, s.oy = oy, s.oz = oz;
     s.po = 4e-7f * (fabsf(ox) + fabsf(oy) + fabsf(oz));
 }
 
 template <int BLOCK>
-__device__ __forceinline__ bool SobbBox(const SobbRay& s, float tmin, float tfar_limit, float la, float ha, float lb,
-                                        float hb, float lc, float hc, uint32_t dirs, float& tnear) {
+__device__ __forceinline__ bool SobbBox(const SobbRay& s, float tmin, float tfar_limit, float la, float ha,
+                                        float lb, float hb, float lc, float hc, uint32_t dirs, float& tnear) {
     const int a = dirs & 15, b = (dirs >> 4) & 15, c = (dirs >> 8) & 15;
-    const float oa = s.on[a * BLOCK], ia = s.idn[a * BLOCK];
-    const float ob = s.on[b * BLOCK], ib = s.idn[b * BLOCK];
-    const float oc = s.on[c * BLOCK], ic = s.idn[c * BLOCK];
+    auto on = [&](int k) { return s.dirs[3 * k] * s.ox + s.dirs[3 * k + 1] * s.oy + s.dirs[3 * k + 2] * s.oz; };
+    const float oa = on(a), ia = s.idn[a * BLOCK];
+    const float ob = on(b), ib = s.idn[b * BLOCK];
+    const float oc = on(c), ic = s.idn[c * BLOCK];
     const float ta0 = ((la - s.po) - oa) * ia, ta1 = ((ha + s.po) - oa) * ia;
     const float tb0 = ((lb - s.po) - ob) * ib, tb1 = ((hb + s.po) - ob) * ib;
     const float tc0 = ((lc - s.po) - oc) * ic, tc1 = ((hc + s.po) - oc) * ic;
@@ -236,8 +240,8 @@ __device__ __forceinline__ bool TestLeaf(const float4* tris, int32_t prim_count,
 // (tmin, tmax). Bounded: a tree visits each node at most once. SOBB selects the skewed
 // child boxes (12 floats = left (lo hi) x 3 slabs, right the same; d.z packs the slab
 // directions, 4 bits each, left in bits 0-11 and right in bits 12-23).
-template <bool ANY, bool SOBB, class Stack>
-__device__ __forceinline__ HitResult Traverse(const TraceTree& tree, const WtRay& w, const SobbRay& sr, bool cull,
+template <bool ANY, bool SOBB, class Stack, class SR>
+__device__ __forceinline__ HitResult Traverse(const TraceTree& tree, const WtRay& w, const SR& sr, bool cull,
                                               Stack& st, TraceCounters& cnt) {
     HitResult r{0.0f, 0.0f, 0.0f, -1, -1};
     float best_t = w.tmax;

@@ -201,7 +201,6 @@ template <bool SOBB, int BLOCK, bool COUNT>
 __global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* counter, unsigned long long* totals) {
     __shared__ int32_t s_stack[kWhittedStack][BLOCK];
     __shared__ uint32_t s_base[BLOCK / kWave];
-    __shared__ float s_on[SOBB ? kMaxDirs : 1][BLOCK];
     __shared__ float s_idn[SOBB ? kMaxDirs : 1][BLOCK];
     __shared__ float s_dirs[SOBB ? 3 * kMaxDirs : 1];
     if (SOBB) {
@@ -211,7 +210,7 @@ __global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* count
     const int lane = LaneId(), wid = threadIdx.x / kWave;
     LaneStack<kWhittedStack, BLOCK> st{&s_stack[0][threadIdx.x],
                                        a.spill + (int64_t(blockIdx.x) * BLOCK + threadIdx.x) * a.spill_stride, 0};
-    SobbRay sr{&s_on[0][threadIdx.x], &s_idn[0][threadIdx.x], 0.0f};
+    SobbRay sr{s_dirs, &s_idn[0][threadIdx.x], 0.0f, 0.0f, 0.0f, 0.0f};
     Tracer<SOBB, BLOCK> T{a, st, sr, s_dirs};
     const int64_t pixels = int64_t(a.W) * a.H;
     const V3 cam{a.basis[0], a.basis[1], a.basis[2]}, p00{a.basis[3], a.basis[4], a.basis[5]};
