@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void KInitClusters(BuildBuffers b) {
     const int64_t n = b.n;
     const float pad = b.ndirs > 0 ? KdopPad(b.bounds) : 0.0f;
     for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x)
-        InitLeaf(b, k, pad);
+        InitLeaf<true>(b, k, pad);
 }
 
 hipError_t LaunchInitClusters(const BuildBuffers& b, hipStream_t s) {
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(256) void KEmit(BuildBuffers b) {
     const int64_t n = b.n, nn = 2 * n - 1;
     const int32_t K = b.counters[5];
     for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x)
-        EmitTriangle(b, int32_t(k));
+        EmitTriangle<true>(b, int32_t(k));
     const int64_t first = n > 1 ? n : 0;
     for (int64_t id = first + blockIdx.x * int64_t(blockDim.x) + threadIdx.x; id < nn;
          id += int64_t(gridDim.x) * blockDim.x)
