@@ -283,6 +283,13 @@ def main() -> int:
         graph = accel.TraceGraph(gb, brays, bhits, ray_mesh=bids)
         graph_ms = timed(graph.launch, 10)
         eager_ms = timed(lambda: gb.trace_batched(brays, bids, 0, bhits), 10)
+        # the same rays over the batch's wide form (4 children), also graph-captured
+        two_child = bhits.clone()
+        winfo = gb.widen(4)
+        wgraph = accel.TraceGraph(gb, brays, bhits, ray_mesh=bids, flags=_lib.DPL_TRACE_WIDE)
+        wgraph_ms = timed(wgraph.launch, 10)
+        wide_equal = bool(torch.equal(bhits.view(torch.int32), two_child.view(torch.int32)))
+        wgraph.close()
         nb = int(boff[-1])
         batch = {"workload": f"{len(boff) - 1} meshes, log-uniform 64..65536 tris (noise spheres / soups), {nb:,} tris; "
                              f"256 rays per mesh", "meshes": len(boff) - 1, "triangles": nb,
@@ -293,10 +300,14 @@ def main() -> int:
                  "sah_cost_mean": gb.stats.sah_cost,
                  "trace_rays": len(bids_np), "trace_graph_ms": round(graph_ms, 4),
                  "trace_eager_ms": round(eager_ms, 4),
-                 "trace_mrays_s": round(len(bids_np) / (graph_ms * 1e-3) / 1e6, 2)}
+                 "trace_mrays_s": round(len(bids_np) / (graph_ms * 1e-3) / 1e6, 2),
+                 "widen4_ms": round(winfo["widen_ms"], 4), "trace_wide4_graph_ms": round(wgraph_ms, 4),
+                 "trace_wide4_mrays_s": round(len(bids_np) / (wgraph_ms * 1e-3) / 1e6, 2),
+                 "wide_hits_equal_two_child": wide_equal}
         log(f"batch: build {batch['build_ms']:.2f} ms ({batch['build_mtris_s']:.0f} Mtris/s, r="
             f"{bparams.nn_search_radius} block={bparams.block_size} digit={bparams.sort_digit_bits}), trace graph "
-            f"{graph_ms:.3f} ms / eager {eager_ms:.3f} ms for {len(bids_np)} rays")
+            f"{graph_ms:.3f} ms / eager {eager_ms:.3f} ms for {len(bids_np)} rays; wide4 graph {wgraph_ms:.3f} ms "
+            f"(hits equal: {wide_equal})")
         graph.close()
         gb.close()
         del btris, brays, bids, bhits

@@ -908,10 +908,12 @@ dpl_status dpl_trace_batched(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray
     return Guard(ctx, [&] {
         Require(n >= 0, "n must be non-negative");
         Require(n == 0 || (rays && hits && ray_mesh), "rays, ray_mesh and hits must be non-null");
-        Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK)) == 0u, "unknown trace flags");
+        Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK | DPL_TRACE_WIDE)) == 0u, "unknown trace flags");
         Require(mem == DPL_MEM_HOST || mem == DPL_MEM_DEVICE, "mem must be DPL_MEM_HOST or DPL_MEM_DEVICE");
         Require(bvh->ctx == ctx, "bvh belongs to another context");
         Require(bvh->batched, "dpl_trace_batched needs a batched BVH");
+        const bool wide = flags & DPL_TRACE_WIDE;
+        Require(!wide || bvh->wide_width != 0, "DPL_TRACE_WIDE needs a wide tree (dpl_bvh_widen)");
         if (n == 0) return;
         hipStream_t s = ctx->stream;
         const dpl_ray* dr = rays;
@@ -924,10 +926,20 @@ dpl_status dpl_trace_batched(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray
             DPL_HIP(hipMemcpyAsync(const_cast<dpl_ray*>(dr), rays, size_t(n) * sizeof(dpl_ray), hipMemcpyHostToDevice, s));
             DPL_HIP(hipMemcpyAsync(const_cast<int32_t*>(dm), ray_mesh, size_t(n) * 4, hipMemcpyHostToDevice, s));
         }
-        int32_t* spill = ctx->spill.ensure<int32_t>(TraceSpillBytes(ctx->cu_count, bvh->tree_height));
-        DPL_HIP(LaunchTraceBatched(bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(),
-                                   bvh->batch_tree.as<BatchTree>(), bvh->mesh_count, bvh->tree_height, dr, dm, n, flags,
-                                   dh, ctx->cu_count, ctx->trace_counter.as<uint32_t>(), spill, s));
+        if (wide) {
+            int32_t* spill = ctx->spill.ensure<int32_t>(
+                std::max<size_t>(16, TraceWideSpillBytes(ctx->cu_count, bvh->wide_width, bvh->tree_height)));
+            const WideTree wt{bvh->wnodes.as<float4>(), bvh->tri_ordered.as<float4>(), 0, 0, bvh->wide_width,
+                              bvh->tree_height};
+            DPL_HIP(LaunchTraceWideBatched(wt, bvh->batch_tree.as<BatchTree>(), bvh->mesh_count, dr, dm, n,
+                                           flags & ~DPL_TRACE_WIDE, dh, ctx->cu_count,
+                                           ctx->trace_counter.as<uint32_t>(), spill, s));
+        } else {
+            int32_t* spill = ctx->spill.ensure<int32_t>(TraceSpillBytes(ctx->cu_count, bvh->tree_height));
+            DPL_HIP(LaunchTraceBatched(bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(),
+                                       bvh->batch_tree.as<BatchTree>(), bvh->mesh_count, bvh->tree_height, dr, dm, n,
+                                       flags, dh, ctx->cu_count, ctx->trace_counter.as<uint32_t>(), spill, s));
+        }
         if (mem == DPL_MEM_HOST) {
             DPL_HIP(hipMemcpyAsync(hits, dh, size_t(n) * sizeof(dpl_hit), hipMemcpyDeviceToHost, s));
             DPL_HIP(hipStreamSynchronize(s));
@@ -965,7 +977,12 @@ dpl_status dpl_trace_graph_create(dpl_context* ctx, const dpl_bvh* bvh, const dp
         DPL_HIP(hipStreamCreateWithFlags(&g->capture, hipStreamNonBlocking));
         DPL_HIP(hipStreamBeginCapture(g->capture, hipStreamCaptureModeThreadLocal));
         hipError_t e;
-        if (bvh->batched) {
+        if (bvh->batched && wide) {
+            const WideTree wt{bvh->wnodes.as<float4>(), bvh->tri_ordered.as<float4>(), 0, 0, bvh->wide_width,
+                              bvh->tree_height};
+            e = LaunchTraceWideBatched(wt, bvh->batch_tree.as<BatchTree>(), bvh->mesh_count, rays, ray_mesh, n,
+                                       flags & ~DPL_TRACE_WIDE, hits, ctx->cu_count, counter, spill, g->capture);
+        } else if (bvh->batched) {
             e = LaunchTraceBatched(bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(),
                                    bvh->batch_tree.as<BatchTree>(), bvh->mesh_count, bvh->tree_height, rays, ray_mesh,
                                    n, flags, hits, ctx->cu_count, counter, spill, g->capture);
@@ -1012,10 +1029,12 @@ void dpl_trace_graph_destroy(dpl_trace_graph* g) {
 }
 
 // ---- wide trees ----
+static int64_t NodeSlots(const dpl_bvh* bvh) { return bvh->batched ? bvh->fnode_slots : bvh->node_count; }
+
 static void WideInfo(const dpl_bvh* bvh, dpl_wide_info* info) {
     info->width = bvh->wide_width;
     info->node_bytes = bvh->wide_width ? WideNodeBytes(bvh->wide_width) : 0;
-    info->node_count = bvh->wide_width ? bvh->node_count : 0;
+    info->node_count = bvh->wide_width ? NodeSlots(bvh) : 0;
     info->widen_ms = bvh->wide_width ? bvh->widen_ms : 0.0;
 }
 
@@ -1024,9 +1043,9 @@ dpl_status dpl_bvh_widen(dpl_bvh* bvh, int32_t width, dpl_wide_info* info) {
     dpl_context* ctx = bvh->ctx;
     return Guard(ctx, [&] {
         Require(width == 4 || width == 8, "width must be 4 or 8");
-        Require(!bvh->batched, "a batched BVH has no wide form");
+        Require(NodeSlots(bvh) <= INT32_MAX, "too many nodes");
         hipStream_t s = ctx->stream;
-        const int32_t K = bvh->node_count;
+        const int32_t K = int32_t(NodeSlots(bvh));
         bvh->wide_width = 0;
         float4* out = bvh->wnodes.ensure<float4>(std::max<size_t>(16, size_t(K) * WideNodeBytes(width)));
         const FNode* src = bvh->fnodes.as<FNode>();
@@ -1045,7 +1064,8 @@ dpl_status dpl_bvh_widen(dpl_bvh* bvh, int32_t width, dpl_wide_info* info) {
                                     refit, box, done, s));
             src = refit;
         }
-        DPL_HIP(LaunchWiden(src, K, width, out, s));
+        DPL_HIP(LaunchWiden(src, K, width, out, bvh->batched ? bvh->batch_tree.as<BatchTree>() : nullptr,
+                            bvh->batched ? bvh->mesh_count : 0, s));
         DPL_HIP(hipEventRecord(ctx->ev[1], s));
         DPL_HIP(hipEventSynchronize(ctx->ev[1]));
         float ms = 0.0f;
@@ -1068,7 +1088,7 @@ dpl_status dpl_bvh_download_wide(const dpl_bvh* bvh, void* nodes) {
     return Guard(ctx, [&] {
         Require(bvh->wide_width != 0, "the BVH has no wide tree (dpl_bvh_widen)");
         DPL_HIP(hipStreamSynchronize(ctx->stream));
-        DPL_HIP(hipMemcpy(nodes, bvh->wnodes.p, size_t(bvh->node_count) * WideNodeBytes(bvh->wide_width),
+        DPL_HIP(hipMemcpy(nodes, bvh->wnodes.p, size_t(NodeSlots(bvh)) * WideNodeBytes(bvh->wide_width),
                           hipMemcpyDeviceToHost));
     });
 }

@@ -367,11 +367,18 @@ struct WideTree {
 // height: the tree's height (height + 1 launches). Scratch: box 6 floats and done 1 int per node.
 hipError_t LaunchRefitAabb(const FNode* nodes, int32_t node_count, const float4* tris, int32_t prim_count,
                            int32_t height, FNode* out, float* box, int32_t* done, hipStream_t s);
-hipError_t LaunchWiden(const FNode* aabb_nodes, int32_t node_count, int width, void* out, hipStream_t s);
+// mt / mesh_count: the per-mesh trees of a batched BVH (node_count = all slots), else null / 0.
+hipError_t LaunchWiden(const FNode* aabb_nodes, int32_t node_count, int width, void* out, const BatchTree* mt,
+                       int32_t mesh_count, hipStream_t s);
 hipError_t LaunchTraceWide(const WideTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,
                            int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters,
                            hipStream_t s);
 size_t TraceWideSpillBytes(int cu_count, int width, int height);
+// Batched BVH: ray i against mesh ray_mesh[i]; tree.nodes / tree.tris are the bases, tree.height
+// the largest mesh height.
+hipError_t LaunchTraceWideBatched(const WideTree& tree, const BatchTree* mt, int32_t mesh_count, const dpl_ray* rays,
+                                  const int32_t* ray_mesh, int64_t n, uint32_t flags, dpl_hit* hits, int cu_count,
+                                  uint32_t* counter, int32_t* spill, hipStream_t s);
 hipError_t LaunchRender(bool adjoint, const TraceTree& tree, const dpl_render_params& rp, const float* basis,
                         const int32_t* mat_id, const float* albedo, const float* emission, const float* d_image,
                         float* image, float* grad_albedo, float* grad_emission, int cu_count, uint32_t* counter,

@@ -164,11 +164,34 @@ hipError_t LaunchRefitAabb(const FNode* nodes, int32_t K, const float4* tris, in
 
 // ---- conversion ------------------------------------------------------------------------
 
+// mt (batched BVHs, else null): node slot i belongs to the mesh whose slots start at the
+// largest node_off <= i; that mesh's references are local to its slots. Slots past a mesh's
+// node count are never reached and get a zero node.
 template <int W>
-__global__ __launch_bounds__(kWideBlock) void KWiden(const FNode* __restrict__ nodes, int32_t K,
-                                                     float4* __restrict__ out) {
-    const int32_t i = int32_t(blockIdx.x) * kWideBlock + int32_t(threadIdx.x);
-    if (i >= K) return;
+__global__ __launch_bounds__(kWideBlock) void KWiden(const FNode* __restrict__ all_nodes, int32_t K_all,
+                                                     float4* __restrict__ out, const BatchTree* __restrict__ mt,
+                                                     int32_t mesh_count) {
+    const int32_t gi = int32_t(blockIdx.x) * kWideBlock + int32_t(threadIdx.x);
+    if (gi >= K_all) return;
+    const FNode* nodes = all_nodes;
+    int32_t K = K_all, i = gi;
+    if (mt) {
+        int lo = 0, hi = mesh_count - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) / 2;
+            if (mt[mid].node_off <= gi) lo = mid;
+            else hi = mid - 1;
+        }
+        const BatchTree bt = mt[lo];
+        i = int32_t(gi - bt.node_off);
+        if (i >= bt.node_count) {
+            float4* o = out + int64_t(gi) * (WideNodeBytes(W) / 16);
+            for (int j = 0; j < WideNodeBytes(W) / 16; j++) o[j] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            return;
+        }
+        nodes = all_nodes + bt.node_off;
+        K = bt.node_count;
+    }
     int32_t ref[W];
     float lo[W][3], hi[W][3];
     double area[W];
@@ -260,7 +283,7 @@ __global__ __launch_bounds__(kWideBlock) void KWiden(const FNode* __restrict__ n
         }
     }
     const uint32_t meta = bexp[0] | (bexp[1] << 8) | (bexp[2] << 16) | (uint32_t(c) << 24);
-    float4* o = out + int64_t(i) * (WideNodeBytes(W) / 16);
+    float4* o = out + int64_t(gi) * (WideNodeBytes(W) / 16);
     o[0] = make_float4(origin[0], origin[1], origin[2], __uint_as_float(meta));
 #pragma unroll
     for (int j = 0; j < W / 4; j++)
@@ -277,14 +300,15 @@ __global__ __launch_bounds__(kWideBlock) void KWiden(const FNode* __restrict__ n
                                        __uint_as_float(words[4 * j + 2]), __uint_as_float(words[4 * j + 3]));
 }
 
-hipError_t LaunchWiden(const FNode* nodes, int32_t K, int width, void* out, hipStream_t s) {
+hipError_t LaunchWiden(const FNode* nodes, int32_t K, int width, void* out, const BatchTree* mt, int32_t mesh_count,
+                       hipStream_t s) {
     if (K <= 0) return hipSuccess;
     const int grid = (K + kWideBlock - 1) / kWideBlock;
     float4* o = static_cast<float4*>(out);
     if (width == 4)
-        hipLaunchKernelGGL(KWiden<4>, dim3(grid), dim3(kWideBlock), 0, s, nodes, K, o);
+        hipLaunchKernelGGL(KWiden<4>, dim3(grid), dim3(kWideBlock), 0, s, nodes, K, o, mt, mesh_count);
     else if (width == 8)
-        hipLaunchKernelGGL(KWiden<8>, dim3(grid), dim3(kWideBlock), 0, s, nodes, K, o);
+        hipLaunchKernelGGL(KWiden<8>, dim3(grid), dim3(kWideBlock), 0, s, nodes, K, o, mt, mesh_count);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -389,11 +413,15 @@ __device__ __forceinline__ HitResult TraverseWide(const WideTree& tree, const Wt
     return r;
 }
 
-template <int W, bool COUNT>
+// BATCH: ray i is traced against mesh ray_mesh[i] (a mesh id out of range gives a miss);
+// tree.nodes / tree.tris are then the bases of all meshes' wide nodes and triangles.
+template <int W, bool COUNT, bool BATCH>
 __global__ __launch_bounds__(kWideBlock) void KTraceWide(WideTree tree, const float4* __restrict__ rays, int64_t n,
                                                          uint32_t flags, dpl_hit* __restrict__ hits,
                                                          uint32_t* counter, int32_t* spill,
-                                                         unsigned long long* counters) {
+                                                         unsigned long long* counters,
+                                                         const int32_t* __restrict__ ray_mesh,
+                                                         const BatchTree* __restrict__ mt, int32_t mesh_count) {
     __shared__ int32_t s_stack[kWideStackLds][kWideBlock];
     __shared__ uint32_t s_base[kWideBlock / kWave];
     const bool any = flags & DPL_TRACE_ANY, cull = flags & DPL_TRACE_CULL_BACK;
@@ -412,8 +440,22 @@ __global__ __launch_bounds__(kWideBlock) void KTraceWide(WideTree tree, const fl
         if (i < n) {
             const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
             const WtRay w = PrepareRay(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w);
-            const HitResult h = any ? TraverseWide<true, W>(tree, w, cull, st, cnt)
-                                    : TraverseWide<false, W>(tree, w, cull, st, cnt);
+            WideTree t = tree;
+            bool valid = true;
+            if (BATCH) {
+                const int32_t m = ray_mesh[i];
+                valid = m >= 0 && m < mesh_count;
+                if (valid) {
+                    const BatchTree bt = mt[m];
+                    t.nodes = tree.nodes + bt.node_off * (WideNodeBytes(W) / 16);
+                    t.tris = tree.tris + 3 * bt.tri_off;
+                    t.node_count = bt.node_count;
+                    t.prim_count = bt.prim_count;
+                }
+            }
+            HitResult h{0.0f, 0.0f, 0.0f, -1, -1};
+            if (valid)
+                h = any ? TraverseWide<true, W>(t, w, cull, st, cnt) : TraverseWide<false, W>(t, w, cull, st, cnt);
             dpl_hit o;
             o.t = h.t;
             o.b1 = h.b1;
@@ -468,8 +510,8 @@ hipError_t LaunchTraceWide(const WideTree& tree, const dpl_ray* rays, int64_t n,
     const float4* r4 = reinterpret_cast<const float4*>(rays);
     const int grid = WideGrid(cu_count, n);
 #define DPL_TRACE_W(W_, COUNT_)                                                                                 \
-    hipLaunchKernelGGL((KTraceWide<W_, COUNT_>), dim3(grid), dim3(kWideBlock), 0, s, tree, r4, n, flags, hits, \
-                       counter, spill, counters)
+    hipLaunchKernelGGL((KTraceWide<W_, COUNT_, false>), dim3(grid), dim3(kWideBlock), 0, s, tree, r4, n, flags,     \
+                       hits, counter, spill, counters, nullptr, nullptr, 0)
     if (tree.width == 4) {
         if (counters) DPL_TRACE_W(4, true);
         else DPL_TRACE_W(4, false);
@@ -478,6 +520,24 @@ hipError_t LaunchTraceWide(const WideTree& tree, const dpl_ray* rays, int64_t n,
         else DPL_TRACE_W(8, false);
     }
 #undef DPL_TRACE_W
+    return hipGetLastError();
+}
+
+hipError_t LaunchTraceWideBatched(const WideTree& tree, const BatchTree* mt, int32_t mesh_count, const dpl_ray* rays,
+                                  const int32_t* ray_mesh, int64_t n, uint32_t flags, dpl_hit* hits, int cu_count,
+                                  uint32_t* counter, int32_t* spill, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    if (n > int64_t(0xFFFFFFFFu) - 64) return hipErrorInvalidValue;
+    if (tree.width != 4 && tree.width != 8) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(KResetWideCounter, dim3(1), dim3(1), 0, s, counter, nullptr);
+    const float4* r4 = reinterpret_cast<const float4*>(rays);
+    const int grid = WideGrid(cu_count, n);
+    if (tree.width == 4)
+        hipLaunchKernelGGL((KTraceWide<4, false, true>), dim3(grid), dim3(kWideBlock), 0, s, tree, r4, n, flags, hits,
+                           counter, spill, nullptr, ray_mesh, mt, mesh_count);
+    else
+        hipLaunchKernelGGL((KTraceWide<8, false, true>), dim3(grid), dim3(kWideBlock), 0, s, tree, r4, n, flags, hits,
+                           counter, spill, nullptr, ray_mesh, mt, mesh_count);
     return hipGetLastError();
 }
 

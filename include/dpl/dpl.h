@@ -201,9 +201,10 @@ dpl_status dpl_trace_graph_launch(dpl_trace_graph* graph); /* on the context's s
 void dpl_trace_graph_destroy(dpl_trace_graph* graph);
 
 /* ---- Wide trees ----
- * Converts a built single-scene BVH into a tree of `width` (4 or 8) children per node whose
- * child boxes are quantized to 8 bits per bound (64-byte nodes for width 4, 96-byte for 8),
- * kept beside the two-child tree until the next rebuild. dpl_trace, dpl_trace_counters and
+ * Converts a built BVH into a tree of `width` (4 or 8) children per node whose child boxes
+ * are quantized to 8 bits per bound (64-byte nodes for width 4, 96-byte for 8), kept beside
+ * the two-child tree until the next rebuild; a batched BVH converts every mesh's tree in
+ * place of its node slots. dpl_trace, dpl_trace_counters, dpl_trace_batched and
  * dpl_trace_graph_create with DPL_TRACE_WIDE traverse it; closest hits equal the two-child
  * tree's (counters then count wide nodes). SOBB trees are refitted to axis-aligned boxes
  * first. No counterpart in the reference, whose GPU intersector is brute force
@@ -213,7 +214,7 @@ void dpl_trace_graph_destroy(dpl_trace_graph* graph);
 typedef struct dpl_wide_info {
     int32_t width;       /* 0: no wide tree */
     int32_t node_bytes;  /* 64 or 96 */
-    int64_t node_count;  /* the two-child tree's node count: wide node i converts node i */
+    int64_t node_count;  /* the two-child tree's node slots: wide node i converts node i */
     double widen_ms;     /* device time of the conversion (refit included) */
 } dpl_wide_info;
 dpl_status dpl_bvh_widen(dpl_bvh* bvh, int32_t width, dpl_wide_info* info /* may be NULL */);

@@ -121,3 +121,60 @@ def test_gpu_rebuild_drops_wide(ctx):
         g.trace(rays_for(tris, 10, 1), _lib.DPL_TRACE_WIDE)
     with pytest.raises(_lib.DplError):
         g.widen(6)
+
+
+def batch_of_meshes(count=48, seed=9):
+    """Meshes of 1 to 3000 triangles (soups and flat patches), concatenated."""
+    rng = np.random.default_rng(seed)
+    sizes = [1, 2, 3] + [int(x) for x in np.exp(rng.uniform(np.log(4), np.log(3000), count - 3))]
+    parts = [SCENES["flat"]()[: max(1, s)] if k % 5 == 4 else
+             (rng.uniform(-1, 1, (s, 1, 3)) + rng.normal(0, 0.05, (s, 3, 3))).reshape(s, 9).astype(np.float32)
+             for k, s in enumerate(sizes)]
+    off = np.concatenate([[0], np.cumsum([len(p) for p in parts])]).astype(np.int64)
+    return np.concatenate(parts).astype(np.float32), off
+
+
+@pytest.mark.parametrize("width", [4, 8])
+def test_gpu_batched_wide(oracle, ctx, width):
+    """Batched BVHs: every mesh's wide nodes equal the oracle's conversion of that mesh's
+    nodes (unused slots zero); wide hits equal the two-child batched trace's; a captured graph
+    replays the wide batched trace."""
+    import torch
+
+    from diplodocus_amd import _lib, accel
+
+    tris, off = batch_of_meshes()
+    say(f"batched wide width={width} meshes={len(off) - 1}")
+    g = accel.GpuBvh(ctx).build_batched(tris, off, accel.make_params(morton_type=1, radius=8, max_leaf=4,
+                                                                     block_size=256))
+    info = g.widen(width)
+    d = g.download()
+    ms = g.meshes()
+    w = g.download_wide()
+    assert info["node_count"] == len(d["nodes"]) == len(w)
+    for m in range(len(ms)):
+        a, k = int(ms[m]["node_offset"]), int(ms[m]["node_count"])
+        end = int(ms[m + 1]["node_offset"]) if m + 1 < len(ms) else len(w)
+        ref = oracle.widen(d["nodes"][a:a + k], width)
+        assert w[a:a + k].tobytes() == ref.tobytes(), m
+        assert not w[a + k:end].any(), m
+    rays, ray_mesh = scenes.batched_rays(tris, off, 64, seed=width)
+    for flags in (0, 1, 2):
+        hw = g.trace_batched(rays, ray_mesh, flags | _lib.DPL_TRACE_WIDE)
+        hb = g.trace_batched(rays, ray_mesh, flags)
+        if flags & 1:
+            assert ((hw["prim"] >= 0) == (hb["prim"] >= 0)).all()
+        else:
+            assert same(hw, hb)
+    rd = torch.from_numpy(rays).cuda()
+    md = torch.from_numpy(ray_mesh).cuda()
+    hits = torch.zeros((len(rays), 4), dtype=torch.float32, device="cuda")
+    eager = g.trace_batched(rd, md, _lib.DPL_TRACE_WIDE).cpu()
+    gr = accel.TraceGraph(g, rd, hits, ray_mesh=md, flags=_lib.DPL_TRACE_WIDE)
+    for k in range(3):
+        hits.zero_()
+        torch.cuda.synchronize()
+        gr.launch()
+        ctx.synchronize()
+        assert torch.equal(hits.cpu().view(torch.int32), eager.view(torch.int32)), k
+    gr.close()
