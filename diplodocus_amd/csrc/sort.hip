@@ -175,18 +175,34 @@ __global__ __launch_bounds__(kSortBlock) void KOnesweep(const K* __restrict__ ki
             __hip_atomic_store(&st[r], kStInc | acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
             __hip_atomic_store(&st[size_t(tile) * RADIX + r], kStAgg | acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // eight predecessors per step (their loads in flight together), consumed in order
+            // up to the first inclusive prefix or the first not yet published
+            constexpr int kLb = 8;
             int64_t p = tile - 1;
             for (;;) {
-                const uint32_t v = __hip_atomic_load(&st[size_t(p) * RADIX + r], __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-                const uint32_t flag = v & ~kStMask;
-                if (flag == 0u) {
-                    __builtin_amdgcn_s_sleep(1);
-                    continue;
+                uint32_t v[kLb];
+#pragma unroll
+                for (int q = 0; q < kLb; q++)
+                    v[q] = p - q >= 0 ? __hip_atomic_load(&st[size_t(p - q) * RADIX + r], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                      : kStInc;
+                bool done = false, wait = false;
+                int used = 0;
+#pragma unroll
+                for (int q = 0; q < kLb; q++) {
+                    if (done || wait) continue;
+                    const uint32_t flag = v[q] & ~kStMask;
+                    if (flag == 0u) {
+                        wait = true;
+                        continue;
+                    }
+                    excl += v[q] & kStMask;
+                    used = q + 1;
+                    done = flag == kStInc;
                 }
-                excl += v & kStMask;
-                if (flag == kStInc) break;
-                p--;
+                if (done) break;
+                p -= used;
+                if (wait) __builtin_amdgcn_s_sleep(1);
             }
             __hip_atomic_store(&st[size_t(tile) * RADIX + r], kStInc | (excl + acc), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
