@@ -3,13 +3,20 @@
 //
 //   1. one pass over the keys builds the digit histograms of every pass in LDS;
 //   2. one workgroup per pass turns them into exclusive digit bases;
-//   3. per pass, one kernel: each workgroup takes a tile id from an atomic counter, ranks
-//      its keys stably with 64-lane ballot matching (lanes holding the same digit form a
-//      peer mask; rank = lower peers + earlier items of the wave + earlier waves), learns
-//      the tile's global digit offsets by a decoupled look-back on per-(tile, digit)
-//      status words, and scatters keys and values.
+//   3. per pass, one kernel: workgroup b sorts tile b; it ranks its keys stably with
+//      64-lane ballot matching (lanes holding the same digit form a peer mask; rank = lower
+//      peers + earlier items of the wave + earlier waves), learns the tile's global digit
+//      offsets by a decoupled look-back on per-(tile, digit) status words, and scatters
+//      keys and values.
+// The look-back waits on earlier tiles only for a bounded time: a predecessor that has not
+// published its count within `spin_ticks` of the 100 MHz real-time clock (for instance,
+// one not yet resident) is counted from its keys by the waiting lane. Every wait therefore
+// ends, whatever order the hardware starts workgroups in, and no tile counter (a
+// contended atomic per workgroup) is needed.
 // Stable: equal keys keep their input order, so the result equals std::stable_sort by key.
 #include <hip/hip_runtime.h>
+
+#include <cstdlib>
 
 #include "dpl_internal.h"
 
@@ -30,11 +37,13 @@ struct SortTraits {
 };
 
 struct SortTemp {
-    uint32_t* hist;          // P x RADIX
-    uint32_t* base;          // P x RADIX
-    uint32_t* tile_counter;  // P
-    uint32_t* status;        // P x T x RADIX
+    uint32_t* hist;    // P x RADIX
+    uint32_t* base;    // P x RADIX
+    uint32_t* status;  // P x T x RADIX
 };
+
+// Look-back wait before a lane counts an unpublished predecessor itself: 100 us.
+constexpr int64_t kSpinTicks = 10000;
 
 int Passes(int key_bits, int digit_bits) { return (key_bits + digit_bits - 1) / digit_bits; }
 
@@ -45,8 +54,6 @@ SortTemp CarveTemp(void* temp, int passes, int radix, int64_t tiles) {
     p += size_t(passes) * radix;
     t.base = p;
     p += size_t(passes) * radix;
-    t.tile_counter = p;
-    p += 64;
     t.status = p;
     return t;
 }
@@ -88,10 +95,23 @@ __global__ __launch_bounds__(kSortBlock) void KSortBases(const uint32_t* hist, u
     if (threadIdx.x < RADIX) base[p * RADIX + threadIdx.x] = ex;
 }
 
+// Occurrences of digit r in full tile p (the look-back's fallback for a predecessor that has
+// not published).
+template <typename K, int DB>
+__device__ uint32_t CountDigit(const K* __restrict__ kin, int64_t p, int shift, uint32_t r) {
+    constexpr int TILE = SortTraits<K>::kTile;
+    const K* src = kin + p * TILE;
+    uint32_t c = 0;
+#pragma unroll 8
+    for (int i = 0; i < TILE; i++) c += uint32_t((src[i] >> shift) & K((1 << DB) - 1)) == r ? 1u : 0u;
+    return c;
+}
+
 template <typename K, int DB>
 __global__ __launch_bounds__(kSortBlock) void KOnesweep(const K* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                         K* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n,
-                                                        int pass, int shift, SortTemp tmp, int64_t tiles) {
+                                                        int pass, int shift, SortTemp tmp, int64_t tiles,
+                                                        int64_t spin_ticks) {
     constexpr int RADIX = 1 << DB;
     constexpr int IPT = SortTraits<K>::kIpt;
     constexpr int TILE = SortTraits<K>::kTile;
@@ -101,13 +121,10 @@ __global__ __launch_bounds__(kSortBlock) void KOnesweep(const K* __restrict__ ki
     __shared__ uint32_t s_scan[kSortBlock / kWave + 1];
     __shared__ K s_key[TILE];
     __shared__ uint32_t s_val[TILE];
-    __shared__ int s_tile;
 
-    if (threadIdx.x == 0) s_tile = int(atomicAdd(&tmp.tile_counter[pass], 1u));
     for (int i = threadIdx.x; i < kSortWaves * RADIX; i += kSortBlock) (&s_wh[0][0])[i] = 0u;
     __syncthreads();
-    const int64_t tile = s_tile;
-    if (tile >= tiles) return;
+    const int64_t tile = blockIdx.x;
 
     const int lane = LaneId(), w = threadIdx.x / kWave;
     const uint64_t lt_mask = (1ull << lane) - 1ull;
@@ -176,9 +193,13 @@ __global__ __launch_bounds__(kSortBlock) void KOnesweep(const K* __restrict__ ki
         } else {
             __hip_atomic_store(&st[size_t(tile) * RADIX + r], kStAgg | acc, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // eight predecessors per step (their loads in flight together), consumed in order
-            // up to the first inclusive prefix or the first not yet published
+            // up to the first inclusive prefix or the first not yet published; a predecessor
+            // still unpublished after spin_ticks is counted from its keys (predecessors are
+            // full tiles: tile < tiles)
             constexpr int kLb = 8;
             int64_t p = tile - 1;
+            bool waiting = false;
+            uint64_t since = 0;
             for (;;) {
                 uint32_t v[kLb];
 #pragma unroll
@@ -202,7 +223,18 @@ __global__ __launch_bounds__(kSortBlock) void KOnesweep(const K* __restrict__ ki
                 }
                 if (done) break;
                 p -= used;
-                if (wait) __builtin_amdgcn_s_sleep(1);
+                if (!wait) continue;
+                const uint64_t now = __builtin_amdgcn_s_memrealtime();
+                if (used > 0 || !waiting) {
+                    waiting = true;
+                    since = now;
+                } else if (int64_t(now - since) >= spin_ticks) {
+                    excl += CountDigit<K, DB>(kin, p, shift, uint32_t(r));
+                    p -= 1;
+                    waiting = false;
+                    continue;
+                }
+                __builtin_amdgcn_s_sleep(1);
             }
             __hip_atomic_store(&st[size_t(tile) * RADIX + r], kStInc | (excl + acc), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -227,6 +259,7 @@ __global__ __launch_bounds__(kSortBlock) void KOnesweep(const K* __restrict__ ki
         const K kk = s_key[k];
         const uint32_t d = uint32_t((kk >> shift) & K(RADIX - 1));
         const uint32_t dst = s_off[d] + (uint32_t(k) - s_start[d]);
+        if (dst >= n) continue;  // unreachable: the offsets are exact
         kout[dst] = kk;
         vout[dst] = s_val[k];
     }
@@ -237,7 +270,7 @@ size_t RadixSortTempBytes(int64_t n, int key_bits, int digit_bits, bool key64) {
     const int radix = 1 << digit_bits;
     const int64_t tile = key64 ? SortTraits<uint64_t>::kTile : SortTraits<uint32_t>::kTile;
     const int64_t tiles = (n + tile - 1) / tile;
-    return (size_t(passes) * radix * 2 + 64 + size_t(passes) * size_t(tiles) * radix) * sizeof(uint32_t);
+    return (size_t(passes) * radix * 2 + size_t(passes) * size_t(tiles) * radix) * sizeof(uint32_t);
 }
 
 template <typename K, int DB>
@@ -265,9 +298,13 @@ static hipError_t SortImpl(K* keys, uint32_t* vals, K* keys_alt, uint32_t* vals_
     uint32_t* vi = vals;
     K* ko = keys_alt;
     uint32_t* vo = vals_alt;
+    // DPL_SORT_LOOKBACK_SPIN_TICKS overrides the look-back wait (tests set 0 to force the
+    // counting fallback on every wait)
+    int64_t spin = kSpinTicks;
+    if (const char* e = std::getenv("DPL_SORT_LOOKBACK_SPIN_TICKS")) spin = std::max<int64_t>(0, std::atoll(e));
     for (int p = 0; p < passes; p++) {
         hipLaunchKernelGGL((KOnesweep<K, DB>), dim3(unsigned(tiles)), dim3(kSortBlock), 0, s, ki, vi, ko, vo, n, p,
-                           p * DB, t, tiles);
+                           p * DB, t, tiles, spin);
         std::swap(ki, ko);
         std::swap(vi, vo);
     }

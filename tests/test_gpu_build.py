@@ -108,7 +108,8 @@ def test_codes_and_sort_entry_points(oracle, ctx):
         assert (b.view(np.uint32) == oracle.bounds(tris).view(np.uint32)).all()
     rng = np.random.default_rng(8)
     for n, bits, digit in [(1, 30, 8), (4095, 30, 8), (4096, 30, 4), (4097, 60, 8), (1_000_003, 60, 8),
-                           (2_000_000, 30, 8), (300_001, 60, 4)]:
+                           (2_000_000, 30, 8), (300_001, 60, 4), (6_000_001, 60, 8)]:
+        # 6M 64-bit keys: 1954 tiles, more workgroups than can be resident at once
         say(f"sort n={n} bits={bits} digit={digit}")
         k = rng.integers(0, 1 << min(bits, 20), n, dtype=np.uint64) << np.uint64(max(0, bits - 20))
         if bits <= 32:
@@ -123,6 +124,23 @@ def test_codes_and_sort_entry_points(oracle, ctx):
         got = keys.cpu().numpy()
         got = got.view(np.uint32).astype(np.uint64) if bits <= 32 else got.view(np.uint64)
         assert (got == k[order]).all()
+
+
+def test_sort_lookback_fallback(ctx, monkeypatch):
+    """A zero look-back wait makes every lane that finds a predecessor tile unpublished count
+    that tile's digit from its keys: the sort is unchanged."""
+    import torch
+
+    monkeypatch.setenv("DPL_SORT_LOOKBACK_SPIN_TICKS", "0")
+    rng = np.random.default_rng(12)
+    for n, bits, digit in [(300_001, 60, 8), (200_003, 30, 4), (3072, 60, 8), (250_000, 30, 8)]:
+        say(f"sort fallback n={n} bits={bits} digit={digit}")
+        k = rng.integers(0, 1 << 12, n, dtype=np.uint64) << np.uint64(bits - 12)
+        keys = torch.from_numpy(k.astype(np.uint32).view(np.int32) if bits <= 32 else k.view(np.int64)).cuda()
+        vals = torch.arange(n, dtype=torch.int32, device="cuda")
+        ctx.radix_sort_pairs(keys, vals, bits, digit)
+        ctx.synchronize()
+        assert (vals.cpu().numpy().astype(np.int64) == np.argsort(k, kind="stable")).all()
 
 
 def test_build_is_deterministic(ctx):
