@@ -164,7 +164,8 @@ def main() -> int:
         per_launch_bytes = ps.ploc_bytes / ps.ploc_launches
         per_launch_ms = ps.ploc_launch_ms / ps.ploc_launches
         achieved = per_launch_bytes / (per_launch_ms * 1e-3) / 1e9
-        roof = {"kernel": "KPlocIteration<256> (one PLOC iteration: NN search, merge, compaction)",
+        roof = {"kernel": "KPlocIteration<256, LANES> (one PLOC iteration: NN search, merge, compaction; "
+                          "LANES = 4 lanes per search once at most two tiles per CU)",
                 "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic(args.variant, args.radius),
                 "launches": ps.ploc_launches, "avg_launch_us": round(per_launch_ms * 1e3, 3),

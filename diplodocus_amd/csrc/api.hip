@@ -394,10 +394,18 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
         int est = int(std::ceil(std::log(double(c_known) / tail) / std::log(rate)));
         est = std::max(1, std::min(est, 96));
         const int64_t T = (c_known + p.block_size - 1) / p.block_size;  // tiles at one cluster per thread
+        const int64_t per_tile = p.block_size - 2 * p.nn_search_radius;
         for (int g = 0; g < est; g++) {
+            // once the modelled count (c_known / rate^g) needs at most two tiles per CU (two
+            // 1024-thread workgroups are resident per CU): four lanes per neighbour search, twice
+            // the modelled tiles up to 2 per CU (surplus workgroups leave at once; more tiles
+            // than workgroups take further rounds)
+            const int64_t t4 = int64_t(std::ceil(double(c_known) / std::pow(rate, g) / double(per_tile)));
+            const bool four = p.block_size == 256 && t4 <= 2 * int64_t(C.cu_count);
+            const int grid = int(four ? std::min<int64_t>(2 * t4, 2 * int64_t(C.cu_count)) : std::min<int64_t>(T, cap));
             if (time_iters) DPL_HIP(hipEventRecord(C.IterEvent(2 * size_t(iter)), s));
-            DPL_HIP(LaunchPlocIteration(b, iter, p.nn_search_radius, p.block_size, tail, p.max_triangles_per_leaf,
-                                        int(std::min<int64_t>(T, cap)), cap, s));
+            DPL_HIP(LaunchPlocIteration(b, iter, p.nn_search_radius, p.block_size, four ? 4 : 1, tail,
+                                        p.max_triangles_per_leaf, grid, cap, s));
             if (time_iters) DPL_HIP(hipEventRecord(C.IterEvent(2 * size_t(iter) + 1), s));
             iter++;
         }

@@ -223,15 +223,19 @@ constexpr uint64_t kMask31 = 0x7FFFFFFFull;
 // Tiles hold BLOCK * ipt clusters, ipt in {1, .., kIpt} chosen per iteration so that there are at
 // least `min_tiles` tiles: small iterations spread over more workgroups, each thread then
 // computing about one nearest neighbour instead of four.
-template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter, int R, int tail, int max_leaf,
-                                                        int min_tiles) {
+// LANES > 1: BLOCK * LANES threads, the tile's nearest-neighbour searches split over LANES
+// adjacent lanes each (same tiles, a quarter of the search latency for LANES = 4); the
+// other phases run on threads below BLOCK, threads above own no cluster.
+template <int BLOCK, int LANES>
+__global__ __launch_bounds__(BLOCK * LANES) void KPlocIteration(BuildBuffers b, int iter, int R, int tail,
+                                                                int max_leaf, int min_tiles) {
+    constexpr int NT = BLOCK * LANES;
     constexpr int CHUNK = BLOCK * kIpt;
     constexpr int WMAX = CHUNK + 4 * kMaxRadius;
     __shared__ float s_lx[WMAX], s_ly[WMAX], s_lz[WMAX], s_hx[WMAX], s_hy[WMAX], s_hz[WMAX];
     __shared__ int32_t s_id[WMAX];
     __shared__ int32_t s_nn[CHUNK + 2 * kMaxRadius];
-    __shared__ uint32_t s_scan[BLOCK / kWave + 1];
+    __shared__ uint32_t s_scan[NT / kWave + 1];
     __shared__ int s_tile;
     __shared__ uint64_t s_prefix;
 
@@ -242,7 +246,7 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
 
     if (c <= tail) {  // done with multi-workgroup iterations: pass the clusters through
         if (blockIdx.x == 0) {
-            for (int k = threadIdx.x; k < c; k += BLOCK) out[k] = in[k];
+            for (int k = threadIdx.x; k < c; k += NT) out[k] = in[k];
             if (threadIdx.x == 0) {
                 b.ccount[iter + 1] = c;
                 b.tile_counter[(iter + 1) & 1] = 0u;
@@ -280,7 +284,7 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
         const int e = a + chunk < c ? a + chunk : c;
         const int w0 = a - 2 * R;
         const int wlen = (e - a) + 4 * R;
-        for (int t = threadIdx.x; t < wlen; t += BLOCK) {
+        for (int t = threadIdx.x; t < wlen; t += NT) {
             int j = w0 + t;
             if (j >= 0 && j < c) {
                 const float4* p = reinterpret_cast<const float4*>(in + j);
@@ -292,7 +296,17 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
         __syncthreads();
         const int n0 = a - R < 0 ? 0 : a - R;
         const int n1 = e + R > c ? c : e + R;
-        for (int i = n0 + threadIdx.x; i < n1; i += BLOCK) s_nn[i - (a - R)] = NearestNeighbour(w, i, w0, c, R);
+        if (LANES == 1) {
+            for (int i = n0 + threadIdx.x; i < n1; i += NT) s_nn[i - (a - R)] = NearestNeighbour(w, i, w0, c, R);
+        } else {
+            // every lane of a wave takes part in the shuffles: the loop bound is block-uniform
+            const int g = int(threadIdx.x) / LANES;
+            for (int i0 = n0; i0 < n1; i0 += NT / LANES) {
+                const int i = i0 + g;
+                const int32_t j = NearestNeighbourLanes(w, i < n1 ? i : n0, w0, c, R, LANES, i < n1);
+                if (i < n1 && int(threadIdx.x) % LANES == 0) s_nn[i - (a - R)] = j;
+            }
+        }
         __syncthreads();
 
         // merge decisions for the tile's clusters (blocked: thread t owns kIpt in a row)
@@ -312,7 +326,7 @@ __global__ __launch_bounds__(BLOCK) void KPlocIteration(BuildBuffers b, int iter
         }
         const uint32_t mine = uint32_t(__popc(sflags)) | (uint32_t(__popc(mflags)) << 16);
         uint32_t total;
-        const uint32_t before = BlockExclusiveScan<BLOCK>(mine, s_scan, total);
+        const uint32_t before = BlockExclusiveScan<NT>(mine, s_scan, total);
         const uint64_t aggS = total & 0xFFFFu, aggM = total >> 16;
 
         // decoupled look-back over earlier tiles (S in bits 31..61, M in bits 0..30)
@@ -391,21 +405,26 @@ int64_t PlocStatusWords(int64_t n, int block) {
 }
 int PlocTail() { return kTail; }
 
-hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int tail, int max_leaf, int grid,
-                               int min_tiles, hipStream_t s) {
+hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int lanes, int tail,
+                               int max_leaf, int grid, int min_tiles, hipStream_t s) {
     if (grid < 1) grid = 1;
     if (min_tiles < 1) min_tiles = 1;
+    if (lanes == 4 && block == 256) {
+        hipLaunchKernelGGL((KPlocIteration<256, 4>), dim3(grid), dim3(1024), 0, s, b, iter, radius, tail, max_leaf,
+                           min_tiles);
+        return hipGetLastError();
+    }
     switch (block) {
         case 128:
-            hipLaunchKernelGGL(KPlocIteration<128>, dim3(grid), dim3(128), 0, s, b, iter, radius, tail, max_leaf,
+            hipLaunchKernelGGL((KPlocIteration<128, 1>), dim3(grid), dim3(128), 0, s, b, iter, radius, tail, max_leaf,
                                min_tiles);
             break;
         case 512:
-            hipLaunchKernelGGL(KPlocIteration<512>, dim3(grid), dim3(512), 0, s, b, iter, radius, tail, max_leaf,
+            hipLaunchKernelGGL((KPlocIteration<512, 1>), dim3(grid), dim3(512), 0, s, b, iter, radius, tail, max_leaf,
                                min_tiles);
             break;
         default:
-            hipLaunchKernelGGL(KPlocIteration<256>, dim3(grid), dim3(256), 0, s, b, iter, radius, tail, max_leaf,
+            hipLaunchKernelGGL((KPlocIteration<256, 1>), dim3(grid), dim3(256), 0, s, b, iter, radius, tail, max_leaf,
                                min_tiles);
             break;
     }

@@ -244,8 +244,10 @@ constexpr int kSahBlocks = 512;  // SAH partials reuse the bounds partial rows (
 hipError_t LaunchBounds(const BuildBuffers& b, hipStream_t s);
 hipError_t LaunchCodes(const BuildBuffers& b, int morton_type, hipStream_t s);
 hipError_t LaunchInitClusters(const BuildBuffers& b, hipStream_t s);
-hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int tail, int max_leaf,
-                               int grid, int min_tiles, hipStream_t s);
+// lanes: 1, or 4 with block 256 (1024-thread workgroups splitting each nearest-neighbour
+// search over four lanes; for iterations with about one tile per CU)
+hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int lanes, int tail,
+                               int max_leaf, int grid, int min_tiles, hipStream_t s);
 hipError_t LaunchPlocTail(const BuildBuffers& b, int iter, int radius, int max_leaf, hipStream_t s);
 int PlocChunk(int block);
 int64_t PlocStatusWords(int64_t n, int block);  // look-back states per parity
