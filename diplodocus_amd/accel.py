@@ -138,7 +138,10 @@ class Context:
             self.h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module globals (lib) may already be cleared; the
+        # process's device memory goes with it
+        if lib is not None:
+            self.close()
 
     # ---- stand-alone stages (parity checks / autotuner) ----
     def compute_codes(self, tris_dev, morton_type: int, codes_dev):
@@ -230,7 +233,10 @@ class GpuBvh:
             self.h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module globals (lib) may already be cleared; the
+        # process's device memory goes with it
+        if lib is not None:
+            self.close()
 
     def layout(self) -> BvhLayout:
         lay = BvhLayout()
@@ -343,7 +349,10 @@ class TraceGraph:
             self.h = None
 
     def __del__(self):
-        self.close()
+        # at interpreter exit the module globals (lib) may already be cleared; the
+        # process's device memory goes with it
+        if lib is not None:
+            self.close()
 
 
 def autotune(ctx: Context, tris, base: BuildParams, offsets=None, sah_tolerance: float = 0.05,

@@ -93,3 +93,17 @@ def test_context_create_reports_missing_device():
     h = C.c_void_p()
     assert _lib.lib.dpl_context_create(0, C.byref(h)) == _lib.DPL_ERR_DEVICE
     assert not h.value
+
+
+def test_destructors_tolerate_interpreter_exit(monkeypatch):
+    """At interpreter exit the module global holding the library can be cleared before the
+    last handles are collected: the destructors then do nothing instead of raising."""
+    from diplodocus_amd import accel
+
+    objs = [cls.__new__(cls) for cls in (accel.Context, accel.GpuBvh, accel.TraceGraph)]
+    for o in objs:
+        o.h = 12345  # never passed to the library: lib is cleared below
+    monkeypatch.setattr(accel, "lib", None)
+    for o in objs:
+        o.__del__()
+        o.h = None
