@@ -390,7 +390,8 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     double rate = 1.15;  // clusters shrink by about this factor per iteration
     const bool time_iters = (p.keep_intermediates & 2) != 0;
     // workgroups per launch, also the tile count below which an iteration takes one cluster per
-    // thread (2 above it): 4 per CU for 256 threads (8 per CU: 3 % slower PLOC, 16: 9 %)
+    // thread (2 above it): 4 per CU for 256 threads (PLOC iterations 3 % slower at 3 or 8 per CU,
+    // 12 % at 2, 13 % at 16)
     const int cap = std::max(1, C.cu_count * (p.block_size == 512 ? 4 : (p.block_size == 128 ? 16 : 4)));
     while (c_known > tail) {
         int est = int(std::ceil(std::log(double(c_known) / tail) / std::log(rate)));
