@@ -252,17 +252,9 @@ def test_collapse_is_cost_reducing(oracle):
     p4 = oracle.Ploc(tris, 0, 16, 4)
     assert (p1.nodes == p4.nodes).all()
     assert p4.sah() <= p1.sah()
-    # collapse decision: leaf iff count <= 4 and SA*count <= SA + cost(l) + cost(r)
-    n = len(tris)
-    nd = p4.nodes
-    for x in range(n, 2 * n - 1):
-        l, r = nd["left"][x], nd["right"][x]
-        s = (nd["hi"][x] - nd["lo"][x]).astype(np.float32)
-        sa = np.float32(2) * (s[0] * s[1] + s[1] * s[2] + s[0] * s[2])
-        inner = np.float32(np.float32(sa + p4.cost[l]) + p4.cost[r])
-        cnt = p4.count[l] + p4.count[r]
-        exp = cnt <= 4 and np.float32(sa * np.float32(cnt)) <= inner
-        assert bool(p4.collapsed[x]) == exp
+    # the collapse itself is pinned against brute-force enumeration of every leaf cut in
+    # tests/test_oracle_pins.py; here: collapsed nodes hold at most max_leaf triangles
+    assert (p4.count[p4.collapsed.astype(bool)] <= 4).all()
 
 
 def test_final_layout_sah_closed_form(oracle):
