@@ -49,6 +49,45 @@ def measured_traffic(variant: str, radius: int):
     return None
 
 
+def build_algorithmic_bytes(n: int, morton_type: int, digit_bits: int, iter_counts, kept_nodes: int) -> dict:
+    """Algorithmic bytes of one build, counted from sizes (never from the profiler): scene
+    bounds 36 n; codes 36 n read + key n + id 4 n + leaf box 24 n written; the radix sort's
+    passes read and write (key, id) pairs plus one histogram read of the keys; PLOC iteration
+    k with c_k clusters and m_k = c_k - c_k+1 merges: 28 c_k window read + 4 c_k neighbour
+    write + 8 c_k mutual check + 32 m_k node write + 28 c_k compaction read + 28 c_k+1
+    compaction write + 4 c_k scan; layout: the final nodes once (64 B each) and the ordered
+    triangles (36 n gathered, 48 n written)."""
+    kb = 8 if morton_type == 1 else 4
+    bits = 60 if morton_type == 1 else 30
+    passes = -(-bits // digit_bits)
+    c = [int(x) for x in iter_counts] + [1]
+    ploc = sum(28 * c[k] + 4 * c[k] + 8 * c[k] + 32 * (c[k] - c[k + 1]) + 28 * c[k] + 28 * c[k + 1] + 4 * c[k]
+               for k in range(len(c) - 1))
+    st = {"bounds": 36 * n, "codes": (36 + kb + 4 + 24) * n, "sort": passes * 2 * (kb + 4) * n + kb * n,
+          "ploc": ploc, "layout": 64 * kept_nodes + (36 + 48) * n}
+    st["total"] = sum(st.values())
+    return st
+
+
+def measured_build_traffic(variant: str, radius: int, build_ms: float):
+    """HBM bytes of one whole build (every kernel between the build's first and last event), as
+    measured by rocprofv3 --pmc passes and summarised in the profile below, with the rate they
+    give over this run's build time; None when no measurement is recorded there."""
+    path = os.path.join(ROOT, "profiles", "build_mixed2m_pmc.md")
+    try:
+        for line in open(path):
+            if line.startswith("build_traffic_bytes:"):
+                key, _, val = line.split(":", 1)[1].partition("=")
+                if key.strip() == f"{variant} r{radius}":
+                    b = int(val.strip())
+                    gbs = b / (build_ms * 1e-3) / 1e9
+                    return {"bytes": b, "gbs": round(gbs, 2), "frac": round(gbs / HBM_PEAK_GBS, 5),
+                            "source": "profiles/build_mixed2m_pmc.md"}
+    except OSError:
+        pass
+    return None
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -124,6 +163,10 @@ def main() -> int:
         f"{trace_ms.mean():.3f} ms, hit rate {(h >= 0).mean():.3f}, iterations {last['ploc_iterations']}, "
         f"stages {last['stage_ms']}")
 
+    # SAH with axis-aligned child boxes (a SOBB tree's boxes refitted): the same cost formula
+    # as the CPU SAH BVH's, for the ratio that gates tree quality
+    sah_aabb = bvh.sah_aabb()
+
     # ---- traversal counters of the same trace (outside the timed region) ----
     _, visits, tests = bvh.trace_counters(rays)
     touched = (visits * 64 + tests * 48) / nrays
@@ -159,6 +202,13 @@ def main() -> int:
     prof = accel.GpuBvh(ctx).build(tris, accel.make_params(morton_type=mt, radius=args.radius, max_leaf=4,
                                                              bv_type=bv, keep_intermediates=3))
     ps = prof.stats
+    pd = prof.download(intermediates=True)
+    alg = build_algorithmic_bytes(n, mt, 8, pd["iter_counts"], ps.inner_node_count)
+    build_gbs = alg["total"] / (build_ms.mean() * 1e-3) / 1e9
+    build_roof = {"algorithmic_bytes": alg, "build_ms": round(float(build_ms.mean()), 4),
+                  "achieved_gbs": round(build_gbs, 2), "frac": round(build_gbs / HBM_PEAK_GBS, 5),
+                  "measured": measured_build_traffic(args.variant, args.radius, float(build_ms.mean()))}
+    log(f"whole build: {alg['total'] / 1e6:.1f} MB algorithmic in {build_ms.mean():.3f} ms = {build_gbs:.1f} GB/s")
     roof = None
     if ps.ploc_launches > 0 and ps.ploc_launch_ms > 0:
         per_launch_bytes = ps.ploc_bytes / ps.ploc_launches
@@ -331,7 +381,8 @@ def main() -> int:
                 torch.cuda.synchronize()
                 sweep.append({"variant": name, "radius": r, "build_ms": round(sb_ev[0].elapsed_time(sb_ev[1]), 4),
                               "trace_ms": round(sb_ev[1].elapsed_time(sb_ev[2]), 4),
-                              "sah_cost": g.stats.sah_cost, "iterations": g.stats.ploc_iterations})
+                              "sah_cost": g.stats.sah_cost, "sah_cost_aabb": g.sah_aabb(),
+                              "iterations": g.stats.ploc_iterations})
         g.close()
         log("sweep: " + ", ".join(f"{s['variant']}/r{s['radius']}: {s['build_ms']:.2f} ms sah {s['sah_cost']:.1f} "
                                   f"trace {s['trace_ms']:.1f} ms" for s in sweep))
@@ -340,20 +391,26 @@ def main() -> int:
     cpu = None
     sah_ratio = None
     sah_ratio_fixed = None
+    sah_ratio_own = None
     if not args.no_cpu:
         from oracle import pyoracle
 
         sb = pyoracle.SBvh(scene.tris, fix_child_bounds=False)
         sah_cpu = sb.sah()
         for s in sweep:
-            s["sah_ratio_vs_cpu"] = round(s["sah_cost"] / sah_cpu, 4)
+            s["sah_ratio_vs_cpu"] = round(s["sah_cost_aabb"] / sah_cpu, 4)
+            s["sah_ratio_vs_cpu_own_bounds"] = round(s["sah_cost"] / sah_cpu, 4)
         sah_fixed = pyoracle.SBvh(scene.tris, fix_child_bounds=True).sah()
-        sah_ratio = last["sah_cost"] / sah_cpu
-        sah_ratio_fixed = last["sah_cost"] / sah_fixed
+        sah_ratio = sah_aabb / sah_cpu
+        sah_ratio_fixed = sah_aabb / sah_fixed
+        sah_ratio_own = last["sah_cost"] / sah_cpu
         sample = rays_np[:131072]
         threads = pyoracle.nthreads()
         sb.trace(sample, flags=0, mode=0, threads=threads)
         cpu_trace_s = sb.last_trace_s
+        sample1 = rays_np[:16384]
+        sb.trace(sample1, flags=0, mode=0, threads=1)
+        cpu_trace1_s = sb.last_trace_s
         t0 = time.perf_counter()
         pyoracle.whitted(pyoracle.Ploc(cs.tris, 0, 16, 4), cs.mat_id, cmats, 256, 256, 3, 8, cs.camera,
                          threads=threads)
@@ -370,12 +427,16 @@ def main() -> int:
                        f"rules on {threads} threads: {len(sample) / cpu_trace_s / 1e6:.4f} Mrays/s"),
             "trace_mrays_s": round(len(sample) / cpu_trace_s / 1e6, 5),
             "trace_threads": threads,
+            "trace_1core_mrays_s": round(len(sample1) / cpu_trace1_s / 1e6, 5),
+            "trace_1core_sample": f"the first {len(sample1)} rays on 1 thread",
             "cpu_model": cpu_model(),
             "host_cores": os.cpu_count(),
             "sah_faithful": sah_cpu,
             "sah_fixed": sah_fixed,
         }
-        log(f"cpu build {sb.build_ms:.0f} ms, sah cpu {sah_cpu:.3f} (fixed {sah_fixed:.3f}) gpu {last['sah_cost']:.3f}")
+        log(f"cpu build {sb.build_ms:.0f} ms, sah cpu {sah_cpu:.3f} (fixed {sah_fixed:.3f}) gpu {sah_aabb:.3f} in "
+            f"axis-aligned areas ({last['sah_cost']:.3f} in its own bounds); cpu trace {threads} threads "
+            f"{len(sample) / cpu_trace_s / 1e6:.4f} Mrays/s, 1 thread {len(sample1) / cpu_trace1_s / 1e6:.4f} Mrays/s")
 
     out = {
         "metric": "build Mtris/s + trace Mrays/s at SAH <=1.10x CPU; fwd+adjoint vs fwd time",
@@ -405,8 +466,12 @@ def main() -> int:
         "build_ms": round(float(build_ms.mean()), 4),
         "trace_ms": round(float(trace_ms.mean()), 4),
         "sah_cost": last["sah_cost"],
+        "sah_cost_aabb": sah_aabb,
         "sah_ratio_vs_cpu": sah_ratio,
         "sah_ratio_vs_cpu_fixed": sah_ratio_fixed,
+        "sah_ratio_vs_cpu_own_bounds": sah_ratio_own,
+        "sah_note": "sah_ratio_vs_cpu: both trees costed with axis-aligned child boxes (the SOBB tree's boxes "
+                    "refitted); _own_bounds: the SOBB tree costed with its SOBB child areas",
         "ploc_iterations": last["ploc_iterations"],
         "stage_ms": last["stage_ms"],
         "trace_nodes_per_ray": round(visits / nrays, 2),
@@ -417,6 +482,7 @@ def main() -> int:
         "wall_s": round(wall, 3),
         "variant_sweep": sweep,
         "roofline": roof,
+        "build_roofline": build_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

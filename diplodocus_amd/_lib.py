@@ -113,6 +113,7 @@ def _load() -> C.CDLL:
                                         C.POINTER(BuildStats)]),
         "dpl_bvh_destroy": (None, [P]),
         "dpl_bvh_get_stats": (I32, [P, C.POINTER(BuildStats)]),
+        "dpl_bvh_sah_aabb": (I32, [P, C.POINTER(C.c_double)]),
         "dpl_bvh_get_layout": (I32, [P, C.POINTER(BvhLayout)]),
         "dpl_bvh_download": (I32, [P, P, P, P, P, P, P, P]),
         "dpl_trace": (I32, [P, P, P, I64, U32, P, I32]),
@@ -153,5 +154,5 @@ EXPORTED = [
     "dpl_bvh_get_layout", "dpl_bvh_download", "dpl_compute_codes", "dpl_radix_sort_pairs", "dpl_render_forward",
     "dpl_render_adjoint", "dpl_bvh_get_meshes", "dpl_trace_graph_create", "dpl_trace_graph_launch",
     "dpl_trace_graph_destroy", "dpl_autotune", "dpl_render_whitted", "dpl_render_whitted_stats", "dpl_bvh_widen",
-    "dpl_bvh_get_wide", "dpl_bvh_download_wide",
+    "dpl_bvh_get_wide", "dpl_bvh_download_wide", "dpl_bvh_sah_aabb",
 ]

@@ -238,6 +238,13 @@ class GpuBvh:
         if lib is not None:
             self.close()
 
+    def sah_aabb(self) -> float:
+        """SAH cost with axis-aligned child boxes (a SOBB tree's boxes refitted); equals
+        stats.sah_cost for AABB trees."""
+        v = C.c_double()
+        self.ctx.check(lib.dpl_bvh_sah_aabb(self.h, C.byref(v)))
+        return v.value
+
     def layout(self) -> BvhLayout:
         lay = BvhLayout()
         self.ctx.check(lib.dpl_bvh_get_layout(self.h, C.byref(lay)))

@@ -13,6 +13,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "dpl_internal.h"
@@ -163,8 +164,15 @@ struct dpl_context {
     int sobb_kdop = -1;  // kdop_size the uploaded direction set was made for
     void* pinned = nullptr;          // host staging for small results (device-visible)
     int32_t* pinned_dev = nullptr;   // device address of `pinned`
-    hipEvent_t ev[10] = {};
+    hipEvent_t ev[11] = {};
     std::vector<hipEvent_t> iter_ev;  // per-launch timing of PLOC iterations (on request)
+    // Live BVH handles of this context and the generation of their device buffers. A build,
+    // rebuild, batched build or widen gives the handle a new generation (from a context-wide
+    // sequence, so a handle later allocated at a freed handle's address never matches it);
+    // destroy removes it. A captured trace graph replays only while its BVH keeps the
+    // generation it was captured with.
+    uint64_t gen_seq = 0;
+    std::unordered_map<const dpl_bvh*, uint64_t> bvh_gen;
     hipEvent_t IterEvent(size_t k) {
         while (iter_ev.size() <= k) {
             hipEvent_t e;
@@ -201,7 +209,13 @@ struct dpl_bvh {
     DevBuf wnodes;
     int32_t wide_width = 0;
     double widen_ms = 0.0;
+    float root_lo[3] = {}, root_hi[3] = {};  // scene box (single-scene BVHs)
 };
+
+namespace {
+// New device buffers (or new contents) behind the handle: invalidates captured graphs.
+void BumpGeneration(dpl_bvh* bvh) { bvh->ctx->bvh_gen[bvh] = ++bvh->ctx->gen_seq; }
+}  // namespace
 
 namespace {
 
@@ -266,6 +280,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     Require(n >= 1, "at least one triangle is required");
     Require(n < (int64_t(1) << 27), "at most 2^27 - 1 triangles per BVH");
     Require(mem == DPL_MEM_HOST || mem == DPL_MEM_DEVICE, "mem must be DPL_MEM_HOST or DPL_MEM_DEVICE");
+    BumpGeneration(bvh);
     hipStream_t s = ctx->stream;
     dpl_context& C = *ctx;
     const int64_t nn = 2 * n - 1;
@@ -424,6 +439,8 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     DPL_HIP(hipEventRecord(C.ev[5], s));
     DPL_HIP(LaunchLayout(b, p.max_triangles_per_leaf, s));
     DPL_HIP(hipEventRecord(C.ev[6], s));
+    DPL_HIP(LaunchFitSobb(b, s));
+    DPL_HIP(hipEventRecord(C.ev[10], s));
     DPL_HIP(LaunchSah(b.fnodes, b.counters + 5, reinterpret_cast<double*>(b.bounds_part),
                       b.ndirs > 0 ? b.dirs : nullptr, s));
     DPL_HIP(LaunchFinish(b, s));
@@ -475,8 +492,8 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     st.stage_ms[3] = ElapsedMs(C.ev[3], C.ev[4]);
     st.stage_ms[4] = ElapsedMs(C.ev[4], C.ev[5]);
     st.stage_ms[5] = ElapsedMs(C.ev[5], C.ev[6]);
-    st.stage_ms[6] = 0.0;
-    st.stage_ms[7] = ElapsedMs(C.ev[6], C.ev[7]);
+    st.stage_ms[6] = ElapsedMs(C.ev[6], C.ev[10]);
+    st.stage_ms[7] = ElapsedMs(C.ev[10], C.ev[7]);
     const int32_t fcount = K > 0 ? K : 1;
     st.inner_node_count = fcount;
     st.leaf_node_count = leaves;
@@ -496,6 +513,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     bvh->node_count = fcount;
     bvh->tree_height = root_height;
     bvh->ndirs = b.ndirs;
+    for (int a = 0; a < 3; a++) bvh->root_lo[a] = lo[a], bvh->root_hi[a] = hi[a];
     bvh->params = p;
     bvh->stats = st;
     bvh->iter_counts = iters;
@@ -632,6 +650,7 @@ void BatchImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, const int64_
     Require(N < (int64_t(1) << 31), "at most 2^31 - 1 triangles per batch");
     if (p.bv_type != DPL_BV_AABB) throw Error(DPL_ERR_UNSUPPORTED, "batched builds support AABB bounds only");
     Require(p.block_size == 256 || p.block_size == 512, "batched block_size must be 256 or 512");
+    BumpGeneration(bvh);
     hipStream_t s = ctx->stream;
     dpl_context& C = *ctx;
 
@@ -793,6 +812,7 @@ void dpl_bvh_destroy(dpl_bvh* bvh) {
     if (bvh->ctx) {
         (void)hipSetDevice(bvh->ctx->device);
         (void)hipStreamSynchronize(bvh->ctx->stream);
+        bvh->ctx->bvh_gen.erase(bvh);
     }
     delete bvh;
 }
@@ -801,6 +821,43 @@ dpl_status dpl_bvh_get_stats(const dpl_bvh* bvh, dpl_build_stats* stats) {
     if (!bvh || !stats) return DPL_ERR_INVALID_ARG;
     *stats = bvh->stats;
     return DPL_OK;
+}
+
+dpl_status dpl_bvh_sah_aabb(const dpl_bvh* bvh, double* sah) {
+    if (!bvh || !sah) return DPL_ERR_INVALID_ARG;
+    dpl_context* ctx = bvh->ctx;
+    return Guard(ctx, [&] {
+        // AABB trees (and batched BVHs, which are AABB only): the build's own SAH; a
+        // single-leaf tree costs n either way
+        if (bvh->ndirs == 0 || bvh->batched || bvh->stats.leaf_node_count <= 1) {
+            *sah = bvh->stats.sah_cost;
+            return;
+        }
+        hipStream_t s = ctx->stream;
+        const int32_t K = bvh->node_count;
+        // refit (as for wide trees) into the context's scratch, then the same SAH kernel with
+        // axis-aligned areas: partial sums per workgroup, added here in workgroup order
+        const size_t refit_bytes = size_t(K) * (sizeof(FNode) + 28);
+        uint8_t* t = ctx->wide_tmp.ensure<uint8_t>(refit_bytes + 256 + size_t(kSahBlocks) * 8);
+        FNode* refit = reinterpret_cast<FNode*>(t);
+        float* box = reinterpret_cast<float*>(t + size_t(K) * sizeof(FNode));
+        int32_t* done = reinterpret_cast<int32_t*>(box + 6 * size_t(K));
+        int32_t* dK = reinterpret_cast<int32_t*>(t + refit_bytes);
+        double* part = reinterpret_cast<double*>(t + refit_bytes + 256);
+        DPL_HIP(hipMemcpyAsync(dK, &K, 4, hipMemcpyHostToDevice, s));
+        DPL_HIP(LaunchRefitAabb(bvh->fnodes.as<FNode>(), K, bvh->tri_ordered.as<float4>(), int32_t(bvh->prim_count),
+                                bvh->tree_height, refit, box, done, s));
+        DPL_HIP(LaunchSah(refit, dK, part, nullptr, s));
+        std::vector<double> h(kSahBlocks);
+        DPL_HIP(hipMemcpyAsync(h.data(), part, size_t(kSahBlocks) * 8, hipMemcpyDeviceToHost, s));
+        DPL_HIP(hipStreamSynchronize(s));
+        double sum = 0.0;
+        for (double v : h) sum += v;
+        const double sx = double(bvh->root_hi[0]) - bvh->root_lo[0], sy = double(bvh->root_hi[1]) - bvh->root_lo[1],
+                     sz = double(bvh->root_hi[2]) - bvh->root_lo[2];
+        const double root_sa = 2.0 * (sx * sy + sy * sz + sx * sz);
+        *sah = root_sa > 0 ? (root_sa + sum) / root_sa : 0.0;
+    });
 }
 
 dpl_status dpl_bvh_get_layout(const dpl_bvh* bvh, dpl_bvh_layout* l) {
@@ -961,6 +1018,8 @@ dpl_status dpl_trace_batched(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray
 // ---- trace passes captured in a hipGraph ----
 struct dpl_trace_graph {
     dpl_context* ctx = nullptr;
+    const dpl_bvh* bvh = nullptr;  // the traced BVH and its generation at capture
+    uint64_t generation = 0;
     hipStream_t capture = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -981,6 +1040,10 @@ dpl_status dpl_trace_graph_create(dpl_context* ctx, const dpl_bvh* bvh, const dp
         Require(bvh->batched == (ray_mesh != nullptr), "ray_mesh is required exactly for a batched BVH");
         const bool wide = flags & DPL_TRACE_WIDE;
         Require(!wide || bvh->wide_width != 0, "DPL_TRACE_WIDE needs a wide tree (dpl_bvh_widen)");
+        const auto live = ctx->bvh_gen.find(bvh);
+        Require(live != ctx->bvh_gen.end(), "the BVH has no built tree");
+        g->bvh = bvh;
+        g->generation = live->second;
         uint32_t* counter = g->counter.ensure<uint32_t>(256);
         const size_t spill_bytes = wide ? TraceWideSpillBytes(ctx->cu_count, bvh->wide_width, bvh->tree_height)
                                         : TraceSpillBytes(ctx->cu_count, bvh->tree_height);
@@ -1024,7 +1087,14 @@ dpl_status dpl_trace_graph_create(dpl_context* ctx, const dpl_bvh* bvh, const dp
 
 dpl_status dpl_trace_graph_launch(dpl_trace_graph* g) {
     if (!g || !g->exec) return DPL_ERR_INVALID_ARG;
-    return Guard(g->ctx, [&] { DPL_HIP(hipGraphLaunch(g->exec, g->ctx->stream)); });
+    return Guard(g->ctx, [&] {
+        // the graph holds the BVH's device addresses, node counts and a spill buffer sized
+        // for its height at capture: replay only the tree it was captured from
+        const auto live = g->ctx->bvh_gen.find(g->bvh);
+        Require(live != g->ctx->bvh_gen.end() && live->second == g->generation,
+                "the BVH was rebuilt, widened or destroyed after the graph was captured");
+        DPL_HIP(hipGraphLaunch(g->exec, g->ctx->stream));
+    });
 }
 
 void dpl_trace_graph_destroy(dpl_trace_graph* g) {
@@ -1055,6 +1125,7 @@ dpl_status dpl_bvh_widen(dpl_bvh* bvh, int32_t width, dpl_wide_info* info) {
     return Guard(ctx, [&] {
         Require(width == 4 || width == 8, "width must be 4 or 8");
         Require(NodeSlots(bvh) <= INT32_MAX, "too many nodes");
+        BumpGeneration(bvh);
         hipStream_t s = ctx->stream;
         const int32_t K = int32_t(NodeSlots(bvh));
         bvh->wide_width = 0;

@@ -529,12 +529,14 @@ hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s) {
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL(KEmit, dim3(grid), dim3(256), 0, s, b);
-    if (b.ndirs > 0) {
-        e = hipGetLastError();
-        if (e != hipSuccess) return e;
-        const int fgrid = int(std::min<int64_t>(4096, (b.n + 255) / 256));
-        hipLaunchKernelGGL(KFitSobb, dim3(fgrid), dim3(256), 0, s, b);
-    }
+    return hipGetLastError();
+}
+
+// SOBB trees: fit every kept child's slab triple (after LaunchLayout); nothing for AABB trees.
+hipError_t LaunchFitSobb(const BuildBuffers& b, hipStream_t s) {
+    if (b.ndirs == 0) return hipSuccess;
+    const int fgrid = int(std::min<int64_t>(4096, (b.n + 255) / 256));
+    hipLaunchKernelGGL(KFitSobb, dim3(fgrid > 0 ? fgrid : 1), dim3(256), 0, s, b);
     return hipGetLastError();
 }
 

@@ -253,6 +253,7 @@ int PlocChunk(int block);
 int64_t PlocStatusWords(int64_t n, int block);  // look-back states per parity
 int PlocTail();
 hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s);
+hipError_t LaunchFitSobb(const BuildBuffers& b, hipStream_t s);
 hipError_t LaunchFinish(const BuildBuffers& b, hipStream_t s);
 hipError_t LaunchExclusiveScan(const int32_t* in, int32_t* out, int64_t n, int32_t* tmp, int32_t* total,
                                hipStream_t s);

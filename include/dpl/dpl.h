@@ -171,6 +171,13 @@ dpl_status dpl_bvh_get_meshes(const dpl_bvh* bvh, dpl_mesh_info* out /* mesh_cou
 void dpl_bvh_destroy(dpl_bvh* bvh);
 dpl_status dpl_bvh_get_stats(const dpl_bvh* bvh, dpl_build_stats* stats);
 
+/* SAH cost of the tree with every child bounded by its axis-aligned box: for a SOBB tree the
+ * boxes are refitted from the triangles below each child (the SOBB-area cost is
+ * dpl_build_stats.sah_cost); for an AABB tree it is sah_cost itself. Gives the SOBB tree's
+ * cost in the same formula as an AABB tree, e.g. the reference's SBvh (src/acceleration/
+ * sbvh.cc:343-367, extended with SA as dpl_build_stats.sah_cost states). Synchronises. */
+dpl_status dpl_bvh_sah_aabb(const dpl_bvh* bvh, double* sah);
+
 /* Closest-hit (or any-hit with DPL_TRACE_ANY) for n rays. Replaces
  * AccelerationStructure::Intersect / IntersectAny (acceleration_structure.h:17-18) and the
  * brute-force DummyIntersect (src/gpu/renderer/gpu_renderer_util.h:29-94).
@@ -192,8 +199,12 @@ dpl_status dpl_trace_batched(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray
 /* A trace pass captured once in a hipGraph and replayed with one launch: rays, ray_mesh
  * (NULL for a single-scene BVH) and hits are device buffers whose addresses the graph keeps;
  * their contents may change between launches. The graph owns its ray counter and spill
- * stacks, so the context's other calls may run between launches. Replaces the per-frame
- * LaunchRaytracingKernel call of GpuRenderer::StartRender (src/renderer/gpu_renderer.cc:53). */
+ * stacks, so the context's other calls may run between launches. The graph also keeps the
+ * BVH's device addresses, sizes and height: a dpl_bvh_rebuild, dpl_bvh_widen or
+ * dpl_bvh_destroy of that BVH after the capture invalidates it, and dpl_trace_graph_launch
+ * then returns DPL_ERR_INVALID_ARG without launching (capture a new graph). Replaces the
+ * per-frame LaunchRaytracingKernel call of GpuRenderer::StartRender
+ * (src/renderer/gpu_renderer.cc:53). */
 typedef struct dpl_trace_graph dpl_trace_graph;
 dpl_status dpl_trace_graph_create(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, const int32_t* ray_mesh,
                                   int64_t n, uint32_t flags, dpl_hit* hits, dpl_trace_graph** out);

@@ -162,6 +162,43 @@ def test_trace_graph_replays_match_eager(ctx):
     g.close()
 
 
+def test_trace_graph_invalidated_by_rebuild_widen_destroy(ctx):
+    """A graph keeps the BVH's device addresses and sizes: after a rebuild, a widen or a
+    destroy of its BVH, launching it is refused without running anything."""
+    import torch
+
+    from diplodocus_amd import _lib, accel
+
+    tris = scenes.icosphere_room().tris
+    g = accel.GpuBvh(ctx).build(tris, accel.make_params(0, 16, 4))
+    cam = torch.from_numpy(scenes.camera_rays(scenes.Camera(), 32, 32)).cuda()
+    hits = torch.zeros((cam.shape[0], 4), dtype=torch.float32, device="cuda")
+    gr = accel.TraceGraph(g, cam, hits)
+    gr.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(hits.view(torch.int32), g.trace(cam).view(torch.int32))
+    say("graph after rebuild")
+    g.build(tris[: len(tris) // 2].copy(), accel.make_params(0, 16, 4))  # rebuild in place, smaller
+    with pytest.raises(accel.DplError) as e:
+        gr.launch()
+    assert e.value.status == 1 and "rebuilt" in str(e.value)
+    gr2 = accel.TraceGraph(g, cam, hits)  # a new capture of the new tree replays
+    gr2.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(hits.view(torch.int32), g.trace(cam).view(torch.int32))
+    g.widen(4)
+    with pytest.raises(accel.DplError):
+        gr2.launch()
+    gr3 = accel.TraceGraph(g, cam, hits, flags=_lib.DPL_TRACE_WIDE)
+    gr3.launch()
+    torch.cuda.synchronize()
+    g.close()
+    with pytest.raises(accel.DplError):
+        gr3.launch()
+    for x in (gr, gr2, gr3):
+        x.close()
+
+
 def test_batched_error_paths(ctx):
     from diplodocus_amd import accel
 

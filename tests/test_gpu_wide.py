@@ -178,3 +178,27 @@ def test_gpu_batched_wide(oracle, ctx, width):
         ctx.synchronize()
         assert torch.equal(hits.cpu().view(torch.int32), eager.view(torch.int32)), k
     gr.close()
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("kdop", [32, 14])
+def test_sah_in_axis_aligned_areas(oracle, ctx, name, kdop):
+    """dpl_bvh_sah_aabb: a SOBB tree's SAH with its children's axis-aligned boxes equals the
+    oracle's SAH of the oracle's refit of the same tree (FP64 sums in another order: 1e-9); an
+    AABB tree's is its own SAH."""
+    from diplodocus_amd import accel
+
+    tris = SCENES[name]()
+    say(f"sah aabb {name} kdop={kdop}")
+    g = accel.GpuBvh(ctx).build(tris, accel.make_params(morton_type=1, radius=16, max_leaf=4, bv_type=1,
+                                                        kdop_size=kdop))
+    d = g.download()
+    v = tris.reshape(-1, 3)
+    root6 = np.concatenate([v.min(0), v.max(0)]).astype(np.float32)
+    ref = oracle.sah_final(oracle.refit_aabb(d["nodes"], d["tri_ordered"]), root6, 0)
+    assert g.sah_aabb() == pytest.approx(ref, rel=1e-9)
+    assert g.stats.sah_cost == pytest.approx(oracle.sah_final(d["nodes"], root6, kdop), rel=1e-9)
+    a = gpu_bvh(ctx, tris, radius=16)
+    assert a.sah_aabb() == a.stats.sah_cost
+    g.close()
+    a.close()
