@@ -4,6 +4,8 @@
 // child boxes, optional traversal counters.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+
 #include "dpl_internal.h"
 #include "traverse.h"
 
@@ -18,6 +20,7 @@ constexpr int kStackLdsSobb = 8;
 constexpr bool kSobbSpillInterleaved = true;
 constexpr int kTraceBlock = 256;
 constexpr int kTraceWgPerCu = 8;
+constexpr int kTraceRefillDefault = 8;  // 2M-triangle random rays: 8 -> 26.3 ms, 16 -> 26.7, 32 -> 28.5, wave-at-a-time 28.1
 }  // namespace
 
 // BATCH: ray i is traced against mesh ray_mesh[i], whose tree starts at the per-mesh offsets
@@ -103,6 +106,175 @@ __global__ __launch_bounds__(BLOCK) void KTrace(TraceTree tree, const float4* __
     }
 }
 
+// The same trace with each lane replacing its ray as soon as it terminates and leaves
+// postponed while the wave is still among inner nodes (persistent threads, "while-while" with
+// speculative traversal and replacement of terminated rays: Aila & Laine, HPG 2009). The
+// wave fetches rays for all its idle lanes with one atomic once at least `refill` of them are
+// idle. Every lane runs the node loop while any lane is still looking for its first leaf; a
+// lane that meets a leaf keeps it pending and goes on with the next stack entry, and the
+// pending leaves are tested together. Closest hits are the smallest t in (tmin, tmax), ties
+// to the lower primitive id, and culling by the best t so far is conservative, so the result
+// does not depend on the visiting order: it equals the ordered traversal's bit for bit.
+template <int BLOCK, bool SOBB, bool BATCH, bool ANY>
+__global__ __launch_bounds__(BLOCK) void KTraceRefill(TraceTree tree, const float4* __restrict__ rays, int64_t n,
+                                                      bool cull, dpl_hit* __restrict__ hits, uint32_t* counter,
+                                                      int32_t* spill, int32_t spill_stride,
+                                                      const int32_t* __restrict__ ray_mesh,
+                                                      const BatchTree* __restrict__ mt, int32_t mesh_count,
+                                                      int refill) {
+    constexpr int S = SOBB ? kStackLdsSobb : kStackLds;
+    constexpr int32_t kEnd = 0x7FFFFFFF;  // no next node: the ray is done once no leaf is pending
+    __shared__ int32_t s_stack[S][BLOCK];
+    __shared__ uint32_t s_base[BLOCK / kWave];
+    __shared__ float s_idn[SOBB ? kMaxDirs : 1][BLOCK];
+    __shared__ float s_dirs[SOBB ? 3 * kMaxDirs : 1];
+    if (SOBB) {
+        for (int k = threadIdx.x; k < 3 * tree.ndirs; k += BLOCK) s_dirs[k] = tree.dirs[k];
+        __syncthreads();
+    }
+    const int lane = LaneId(), wid = threadIdx.x / kWave;
+    const uint64_t below = (1ull << lane) - 1ull;
+    SobbRay sr{s_dirs, &s_idn[0][threadIdx.x], 0.0f, 0.0f, 0.0f, 0.0f};
+    TraceCounters none{0u, 0u};
+    auto run = [&](auto& st) {
+        int64_t ray = -1;  // this lane's ray, -1 while idle
+        WtRay w{};
+        TraceTree t = tree;
+        float best_t = 0.0f;
+        HitResult r{0.0f, 0.0f, 0.0f, -1, -1};
+        int32_t ref = kEnd, pend = 0;  // pend < 0: a leaf reference waiting for its test
+        int64_t budget = 0;
+        bool more = true;  // wave-uniform: the ray counter may still hand out rays
+        for (;;) {
+            const uint64_t idle = __ballot(ray < 0);
+            const int nidle = __popcll(idle);
+            if (more && nidle >= refill) {
+                if (lane == 0) s_base[wid] = atomicAdd(counter, uint32_t(nidle));
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const int64_t base = s_base[wid];
+                __builtin_amdgcn_wave_barrier();
+                if (base + nidle >= n) more = false;
+                if (ray < 0) {
+                    const int64_t i = base + __popcll(idle & below);
+                    if (i < n) {
+                        const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+                        w = PrepareRay(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w);
+                        if (SOBB) PrepareSobb<BLOCK>(sr, s_dirs, tree.ndirs, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z);
+                        t = tree;
+                        ref = 0;
+                        if (BATCH) {
+                            const int32_t m = ray_mesh[i];
+                            if (m >= 0 && m < mesh_count) {
+                                const BatchTree bt = mt[m];
+                                t.nodes = tree.nodes + bt.node_off;
+                                t.tris = tree.tris + 3 * bt.tri_off;
+                                t.node_count = bt.node_count;
+                                t.prim_count = bt.prim_count;
+                                t.height = bt.height;
+                            } else {
+                                ref = kEnd;  // a mesh id out of range: a miss
+                            }
+                        }
+                        best_t = w.tmax;
+                        r = HitResult{0.0f, 0.0f, 0.0f, -1, -1};
+                        pend = 0;
+                        st.sp = 0;
+                        budget = 2 * int64_t(t.node_count) + 2;
+                        ray = i;
+                    }
+                }
+            }
+            if (__ballot(ray >= 0) == 0ull) {
+                if (!more) break;
+                continue;
+            }
+            if (ray >= 0) {
+                // inner nodes, while any lane has no pending leaf yet
+                for (;;) {
+                    if (ref < 0 && pend >= 0) {
+                        pend = ref;
+                        ref = st.sp > 0 ? st.Pop() : kEnd;
+                    }
+                    const bool node = ref >= 0 && ref < t.node_count;
+                    if (!__any(node && pend >= 0)) break;
+                    if (node) {
+                        if (--budget < 0) {  // only a malformed tree gets here
+                            ref = kEnd;
+                            pend = 0;
+                            st.sp = 0;
+                            continue;
+                        }
+                        const float4* np = reinterpret_cast<const float4*>(t.nodes + ref);
+                        const float4 a = np[0], b = np[1], c = np[2];
+                        const int4 d = reinterpret_cast<const int4*>(np)[3];
+                        float tl, tr;
+                        bool hl, hr;
+                        if (SOBB) {
+                            const uint32_t dirs = uint32_t(d.z);
+                            hl = SobbBox<BLOCK>(sr, w.tmin, best_t, a.x, a.y, a.z, a.w, b.x, b.y, dirs & 0xFFFu, tl);
+                            hr = SobbBox<BLOCK>(sr, w.tmin, best_t, b.z, b.w, c.x, c.y, c.z, c.w,
+                                                (dirs >> 12) & 0xFFFu, tr);
+                        } else {
+                            hl = Box(w, best_t, a.x, a.y, a.z, a.w, c.x, c.y, tl);
+                            hr = Box(w, best_t, b.x, b.y, b.z, b.w, c.z, c.w, tr);
+                        }
+                        if (hl && hr) {
+                            const bool rfirst = tr < tl;
+                            st.Push(rfirst ? d.x : d.y);
+                            ref = rfirst ? d.y : d.x;
+                        } else if (hl) {
+                            ref = d.x;
+                        } else if (hr) {
+                            ref = d.y;
+                        } else {
+                            ref = st.sp > 0 ? st.Pop() : kEnd;
+                        }
+                    }
+                }
+                // the pending leaves
+                if (pend < 0) {
+                    const bool done = TestLeaf<ANY>(t.tris, t.prim_count, pend, w, cull, best_t, r, none);
+                    pend = 0;
+                    if (ANY && done) {
+                        ref = kEnd;
+                        st.sp = 0;
+                    }
+                }
+                if (ref >= t.node_count && pend >= 0) {
+                    dpl_hit o;
+                    o.t = r.t;
+                    o.b1 = r.b1;
+                    o.b2 = r.b2;
+                    o.prim = r.prim;
+                    hits[ray] = o;
+                    ray = -1;
+                }
+            }
+        }
+    };
+    if constexpr (SOBB && kSobbSpillInterleaved) {
+        LaneStackInterleaved<S, BLOCK> st{&s_stack[0][threadIdx.x], spill + int64_t(blockIdx.x) * BLOCK + threadIdx.x,
+                                          int64_t(gridDim.x) * BLOCK, 0};
+        run(st);
+    } else {
+        LaneStack<S, BLOCK> st{&s_stack[0][threadIdx.x],
+                               spill + (int64_t(blockIdx.x) * BLOCK + threadIdx.x) * spill_stride, 0};
+        run(st);
+    }
+}
+
+// Idle lanes per wave before the wave fetches new rays (1..64); 0 selects the wave-at-a-time
+// kernel (each wave fetches 64 rays and fetches again once all 64 are done).
+int TraceRefillLanes() {
+    static const int v = [] {
+        const char* e = std::getenv("DPL_TRACE_REFILL");
+        const int x = e ? std::atoi(e) : kTraceRefillDefault;
+        return x < 0 ? 0 : (x > kWave ? kWave : x);
+    }();
+    return v;
+}
+
 __global__ void KResetCounter(uint32_t* c, unsigned long long* counters) {
     c[0] = 0u;
     if (counters) counters[0] = counters[1] = 0ull;
@@ -129,7 +301,7 @@ size_t TraceSpillBytes(int cu_count, int height) {
 hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,
                        int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    if (n > int64_t(0xFFFFFFFFu) - 64) return hipErrorInvalidValue;
+    if (n > int64_t(0x7FFFFFFF)) return hipErrorInvalidValue;  // the 32-bit ray counter never wraps
     hipLaunchKernelGGL(KResetCounter, dim3(1), dim3(1), 0, s, counter, counters);
     const float4* r4 = reinterpret_cast<const float4*>(rays);
     const int grid = TraceGrid(cu_count, n);
@@ -138,12 +310,26 @@ hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, ui
     hipLaunchKernelGGL((KTrace<kTraceBlock, COUNT_, SOBB_, false>), dim3(grid), dim3(kTraceBlock), 0, s, tree, r4, n, \
                        flags, hits, counter, spill, stride, counters, nullptr, nullptr, 0)
     const bool sobb = tree.ndirs > 0;
-    if (counters) {
+    const int refill = TraceRefillLanes();
+    if (counters) {  // the counters count the ordered traversal, as the oracle does
         if (sobb) DPL_TRACE(true, true);
         else DPL_TRACE(true, false);
-    } else {
+    } else if (refill == 0) {
         if (sobb) DPL_TRACE(false, true);
         else DPL_TRACE(false, false);
+    } else {
+        const bool any = flags & DPL_TRACE_ANY, cull = flags & DPL_TRACE_CULL_BACK;
+#define DPL_TRACE_R(SOBB_, ANY_)                                                                                    \
+    hipLaunchKernelGGL((KTraceRefill<kTraceBlock, SOBB_, false, ANY_>), dim3(grid), dim3(kTraceBlock), 0, s, tree, r4, \
+                       n, cull, hits, counter, spill, stride, nullptr, nullptr, 0, refill)
+        if (sobb) {
+            if (any) DPL_TRACE_R(true, true);
+            else DPL_TRACE_R(true, false);
+        } else {
+            if (any) DPL_TRACE_R(false, true);
+            else DPL_TRACE_R(false, false);
+        }
+#undef DPL_TRACE_R
     }
 #undef DPL_TRACE
     return hipGetLastError();
@@ -154,13 +340,25 @@ hipError_t LaunchTraceBatched(const FNode* nodes, const float4* tris, const Batc
                               uint32_t flags, dpl_hit* hits, int cu_count, uint32_t* counter, int32_t* spill,
                               hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    if (n > int64_t(0xFFFFFFFFu) - 64) return hipErrorInvalidValue;
+    if (n > int64_t(0x7FFFFFFF)) return hipErrorInvalidValue;  // the 32-bit ray counter never wraps
     hipLaunchKernelGGL(KResetCounter, dim3(1), dim3(1), 0, s, counter, nullptr);
     TraceTree tree{nodes, tris, 0, 0, max_height, 0, nullptr};
     const int grid = TraceGrid(cu_count, n);
-    hipLaunchKernelGGL((KTrace<kTraceBlock, false, false, true>), dim3(grid), dim3(kTraceBlock), 0, s, tree,
-                       reinterpret_cast<const float4*>(rays), n, flags, hits, counter, spill,
-                       TraceSpillStride(max_height), nullptr, ray_mesh, mt, mesh_count);
+    const int refill = TraceRefillLanes();
+    const float4* r4 = reinterpret_cast<const float4*>(rays);
+    const int32_t stride = TraceSpillStride(max_height);
+    if (refill == 0) {
+        hipLaunchKernelGGL((KTrace<kTraceBlock, false, false, true>), dim3(grid), dim3(kTraceBlock), 0, s, tree, r4, n,
+                           flags, hits, counter, spill, stride, nullptr, ray_mesh, mt, mesh_count);
+    } else {
+        const bool cull = flags & DPL_TRACE_CULL_BACK;
+        if (flags & DPL_TRACE_ANY)
+            hipLaunchKernelGGL((KTraceRefill<kTraceBlock, false, true, true>), dim3(grid), dim3(kTraceBlock), 0, s,
+                               tree, r4, n, cull, hits, counter, spill, stride, ray_mesh, mt, mesh_count, refill);
+        else
+            hipLaunchKernelGGL((KTraceRefill<kTraceBlock, false, true, false>), dim3(grid), dim3(kTraceBlock), 0, s,
+                               tree, r4, n, cull, hits, counter, spill, stride, ray_mesh, mt, mesh_count, refill);
+    }
     return hipGetLastError();
 }
 

@@ -504,7 +504,7 @@ hipError_t LaunchTraceWide(const WideTree& tree, const dpl_ray* rays, int64_t n,
                            int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters,
                            hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    if (n > int64_t(0xFFFFFFFFu) - 64) return hipErrorInvalidValue;
+    if (n > int64_t(0x7FFFFFFF)) return hipErrorInvalidValue;  // the 32-bit ray counter never wraps
     if (tree.width != 4 && tree.width != 8) return hipErrorInvalidValue;
     hipLaunchKernelGGL(KResetWideCounter, dim3(1), dim3(1), 0, s, counter, counters);
     const float4* r4 = reinterpret_cast<const float4*>(rays);
@@ -527,7 +527,7 @@ hipError_t LaunchTraceWideBatched(const WideTree& tree, const BatchTree* mt, int
                                   const int32_t* ray_mesh, int64_t n, uint32_t flags, dpl_hit* hits, int cu_count,
                                   uint32_t* counter, int32_t* spill, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    if (n > int64_t(0xFFFFFFFFu) - 64) return hipErrorInvalidValue;
+    if (n > int64_t(0x7FFFFFFF)) return hipErrorInvalidValue;  // the 32-bit ray counter never wraps
     if (tree.width != 4 && tree.width != 8) return hipErrorInvalidValue;
     hipLaunchKernelGGL(KResetWideCounter, dim3(1), dim3(1), 0, s, counter, nullptr);
     const float4* r4 = reinterpret_cast<const float4*>(rays);
