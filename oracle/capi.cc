@@ -334,6 +334,10 @@ void orc_trace_brute(const float* tri, int64_t ntri, const RayRec* rays, int64_t
                      int threads) {
     TraceBrute(tri, ntri, rays, n, flags, hits, threads);
 }
+void orc_trace_dummy(const float* tri, int64_t ntri, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits,
+                     int threads) {
+    TraceDummy(tri, ntri, rays, n, flags, hits, threads);
+}
 
 // Whitted ray tracing over the PLOC tree of `ploc` (closest hits are tree-independent).
 // mats: M x 16 floats (diffuse3 specular3 transmittance3 emission3 shininess ior r_ior pad).

@@ -313,6 +313,11 @@ void TraceFinalCounted(const FinalTree& t, const RayRec* rays, int64_t n, uint32
                        TraceCount& total);
 void TraceBrute(const float* tri_xyz, int64_t ntri, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits,
                 int threads);
+// The reference's brute force (src/acceleration/dummy_acceleration_structure.cc:21-79):
+// Moller-Trumbore with the absolute |det| < 1e-6 rejection, closest hit t in (1e-6, t_max)
+// with the first triangle winning a tie (strict <), any hit at t <= t_max (not <).
+void TraceDummy(const float* tri_xyz, int64_t ntri, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits,
+                int threads);
 
 // ---------------------------------------------------------------------------------------
 // Wide trees (wide_restate.cc): node i of width W is node i of the two-child tree opened

@@ -70,6 +70,7 @@ def _load():
         "orc_wide_node_bytes": (I32, [I32]),
         "orc_trace_wide": (None, [P, I64, I32, P, I64, P, I64, U32, P, I32, P]),
         "orc_trace_brute": (None, [P, I64, P, I64, U32, P, I32]),
+        "orc_trace_dummy": (None, [P, I64, P, I64, U32, P, I32]),
         "orc_pcg_stream": (None, [U64, U64, I32, P]),
         "orc_sincos2pi": (None, [F, P, P]),
     }
@@ -161,6 +162,16 @@ def trace_brute(tris, rays, flags=0, threads=None) -> np.ndarray:
     t, r = _f32(tris), _f32(rays)
     hits = np.zeros(r.shape[0], HIT_DTYPE)
     lib.orc_trace_brute(_p(t), t.shape[0], _p(r), r.shape[0], flags, _p(hits), threads or nthreads())
+    return hits
+
+
+def trace_dummy(tris, rays, flags=0, threads=None) -> np.ndarray:
+    """The reference's brute force (dummy_acceleration_structure.cc:21-79): MT with the
+    absolute |det| < 1e-6 rejection, closest hit strict (first of equal t wins), any-hit at
+    t <= t_max. prim = input index, -1 on a miss; any-hit results carry the hit found."""
+    t, r = _f32(tris), _f32(rays)
+    hits = np.zeros(r.shape[0], HIT_DTYPE)
+    lib.orc_trace_dummy(_p(t), t.shape[0], _p(r), r.shape[0], flags, _p(hits), threads or nthreads())
     return hits
 
 

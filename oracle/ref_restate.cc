@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <chrono>
 #include <numeric>
+#include <thread>
 #include <utility>
 #include <vector>
 
@@ -269,6 +270,45 @@ void SBvh::CalculateStats() {
 }
 
 static Ray ToRay(const RayRec& r) { return Ray{{r.ox, r.oy, r.oz}, {r.dx, r.dy, r.dz}, r.tmax}; }
+
+// dummy_acceleration_structure.cc:21-56 (Intersect) and :58-79 (IntersectAny): every
+// triangle in input order; closest keeps the first of equal t (strict <, from t_max), any-hit
+// accepts t <= t_max. The reference's implicit t_min is kEpsilon.
+static HitRec DummyOne(const std::vector<Tri>& tris, const RayRec& rr, bool any, bool cull) {
+    const Ray ray = ToRay(rr);
+    HitRec h{0.0f, 0.0f, 0.0f, -1};
+    float t_hit = ray.t_max;
+    for (size_t k = 0; k < tris.size(); k++) {
+        float b1 = 0.0f, b2 = 0.0f;
+        const float t = IntersectRayMT(tris[k], ray, b1, b2, cull);
+        if (any) {
+            if (t > kEpsilon && t <= ray.t_max) return HitRec{t, b1, b2, int32_t(k)};
+            continue;
+        }
+        if (t > kEpsilon && t < t_hit) {
+            t_hit = t;
+            h = HitRec{t, b1, b2, int32_t(k)};
+        }
+    }
+    return h;
+}
+
+void TraceDummy(const float* tri_xyz, int64_t ntri, const RayRec* rays, int64_t n, uint32_t flags, HitRec* hits,
+                int threads) {
+    std::vector<Tri> tris(ntri);
+    for (int64_t k = 0; k < ntri; k++) tris[k] = LoadTri(tri_xyz + 9 * k);
+    const bool any = flags & 1u, cull = flags & 2u;
+    auto body = [&](int64_t lo, int64_t hi) {
+        for (int64_t i = lo; i < hi; i++) hits[i] = DummyOne(tris, rays[i], any, cull);
+    };
+    if (threads <= 1) {
+        body(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < threads; t++) th.emplace_back(body, n * t / threads, n * (t + 1) / threads);
+    for (auto& x : th) x.join();
+}
 
 // sbvh.cc:60-145 (mode 0) and the engine's closest-hit rules over the same tree (mode 1)
 bool SBvh::Intersect(const RayRec& rr, HitRec& h, bool cull, int mode, int64_t* visits) const {

@@ -157,6 +157,47 @@ def test_reference_rules_classification(oracle, ctx):
     assert len(diff) < 0.01 * len(rays)
 
 
+def test_reference_brute_force_classification(oracle, ctx):
+    """Compares the engine's hits with the reference's own brute force (MT with |det| < 1e-6
+    rejected, strict closest hit: src/acceleration/dummy_acceleration_structure.cc:21-56) on
+    the icosphere room and the mixed scene's small triangles, and classifies every
+    disagreement as a tie, a grazing hit or a determinant rejection."""
+    for name in ("icosphere", "mixed"):
+        if name == "icosphere":
+            s = scenes.icosphere_room()
+            rays = scenes.camera_rays(s.camera, 128, 128)
+        else:
+            s = scenes.mixed_2m(n_small=20_000, n_thin=3_000)
+            rays = scenes.random_rays_towards(s.tris, 8192, seed=21)
+        say(f"reference brute force vs engine: {name}")
+        h = gpu_bvh(ctx, s.tris).trace(rays)
+        ref = oracle.trace_dummy(s.tris, rays)
+        diff = np.nonzero(h["prim"] != ref["prim"])[0]
+        kinds = {"tie": 0, "grazing": 0, "det": 0}
+        for i in diff:
+            a, b = h[i], ref[i]
+            if a["prim"] >= 0 and b["prim"] >= 0 and abs(a["t"] - b["t"]) <= 1e-6 * max(1.0, abs(a["t"])):
+                kinds["tie"] += 1
+                continue
+            graze = False
+            for x, cond in ((a, a["prim"] >= 0), (b, b["prim"] >= 0)):
+                if cond and min(x["b1"], x["b2"], 1 - x["b1"] - x["b2"]) < 1e-6:
+                    graze = True
+            if graze:
+                kinds["grazing"] += 1
+                continue
+            # the engine's triangle is rejected by the reference's test: |det| < 1e-6
+            assert a["prim"] >= 0, (name, i)
+            tri = s.tris[a["prim"]].astype(np.float64).reshape(3, 3)
+            d = rays[i, 4:7].astype(np.float64)
+            det = np.dot(tri[1] - tri[0], np.cross(d, tri[2] - tri[0]))
+            assert abs(det) < 1e-6 * (1 + 1e-5), (name, i, det)
+            assert oracle.intersect_mt(s.tris[a["prim"]], rays[i])[0] == np.finfo(np.float32).max
+            kinds["det"] += 1
+        say(f"  {len(diff)} of {len(rays)} differ: {kinds}")
+        assert len(diff) < 0.02 * len(rays)
+
+
 def test_trace_error_paths(ctx):
     from diplodocus_amd import accel
 

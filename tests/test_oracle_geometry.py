@@ -132,3 +132,63 @@ def test_box_test_is_conservative(oracle):
         r = ray(o, d)
         tn = C.c_float()
         assert lib.orc_intersect_box(oracle._p(r), oracle._p(lo), oracle._p(hi), np.float32(1e30), C.byref(tn))
+
+
+# ---------------------------------------------------------------------------------------
+# the reference's brute force (src/acceleration/dummy_acceleration_structure.cc:21-79)
+# ---------------------------------------------------------------------------------------
+def test_dummy_any_hit_accepts_t_equal_to_tmax(oracle):
+    """IntersectAny accepts t <= t_max (:73) where SBvh::IntersectAny needs t < t_max
+    (sbvh.cc:168), and Intersect needs t < t_max (strict, from t_hit = t_max, :35)."""
+    big = np.array([-1, -1, 1, 1, -1, 1, 0, 1, 1], np.float32)
+    r = ray((0.1, 0.2, 0.0), (0, 0, 1), tmax=1.0)  # hits the z = 1 plane at exactly t = 1
+    assert oracle.intersect_mt(big, r)[0] == 1.0
+    rays = r[None]
+    assert oracle.trace_dummy(big[None], rays, flags=1)["prim"][0] == 0
+    assert oracle.trace_dummy(big[None], rays, flags=0)["prim"][0] == -1
+    sb = oracle.SBvh(big[None])
+    assert sb.trace(rays, flags=1, mode=0)["prim"][0] == -1
+    r2 = ray((0.1, 0.2, 0.0), (0, 0, 1), tmax=np.nextafter(np.float32(1), np.float32(2)))
+    assert oracle.trace_dummy(big[None], r2[None], flags=0)["prim"][0] == 0
+    r3 = ray((0.1, 0.2, 0.0), (0, 0, 1), tmax=np.nextafter(np.float32(1), np.float32(0)))
+    assert oracle.trace_dummy(big[None], r3[None], flags=1)["prim"][0] == -1
+
+
+def test_dummy_rejects_small_determinants(oracle):
+    """|det| < 1e-6 is rejected absolutely (triangle.cc:18-24): a 1e-4-sized triangle hit
+    head-on (det = |e1 x e2| = 1e-8) is missed, where the watertight test hits it."""
+    s = np.float32(1e-4)
+    tiny = np.array([0, 0, 0, s, 0, 0, 0, s, 0], np.float32)
+    r = ray((s / 4, s / 4, 1.0), (0, 0, -1))[None]
+    assert oracle.trace_dummy(tiny[None], r)["prim"][0] == -1
+    assert oracle.trace_brute(tiny[None], r)["prim"][0] == 0
+    big = (tiny * np.float32(1e3)).astype(np.float32)  # det = 1e-2: hit
+    r = ray((0.025, 0.025, 1.0), (0, 0, -1))[None]
+    assert oracle.trace_dummy(big[None], r)["prim"][0] == 0
+
+
+def test_dummy_closest_first_of_equal_t_and_nearest(oracle):
+    a = np.array([0, 0, 2, 1, 0, 2, 0, 1, 2], np.float32)
+    b = np.array([0, 0, 1, 1, 0, 1, 0, 1, 1], np.float32)
+    r = ray((0.2, 0.2, 0.0), (0, 0, 1))[None]
+    h = oracle.trace_dummy(np.stack([a, b, b]), r)
+    assert h["prim"][0] == 1 and h["t"][0] == 1.0  # nearest, the first of two at equal t
+    assert (h["b1"][0], h["b2"][0]) == (np.float32(0.2), np.float32(0.2))
+    assert oracle.trace_dummy(np.stack([b, a, b]), r)["prim"][0] == 0
+
+
+def test_dummy_matches_the_reference_sbvh_on_the_cornell_room(oracle):
+    """The same MT test and rules over all triangles and over the SBvh (fixed child bounds)
+    give the same nearest t; prims differ only at equal t (first found wins in each)."""
+    from diplodocus_amd import scenes
+
+    s = scenes.cornell_room()
+    rays = scenes.camera_rays(s.camera, 96, 96)
+    d = oracle.trace_dummy(s.tris, rays)
+    sb = oracle.SBvh(s.tris, fix_child_bounds=True).trace(rays, mode=0)
+    assert ((d["prim"] >= 0) == (sb["prim"] >= 0)).all()
+    hit = d["prim"] >= 0
+    assert hit.mean() > 0.5
+    assert (d["t"][hit] == sb["t"][hit]).all()
+    same = d["prim"] == sb["prim"]
+    assert same.mean() > 0.99
