@@ -21,6 +21,7 @@
 
 #include "config.h"
 #include "dpl_host.h"
+#include "renderer.h"
 #include "scene_loader.h"
 
 extern "C" void orc_trace_brute(const float* tri, int64_t ntri, const void* rays, int64_t n, uint32_t flags, void* hits,
@@ -264,11 +265,9 @@ void CpuChecks() {
     std::printf("cpu checks: %s\n", g_failures ? "failed" : "ok");
 }
 
-// Renders scene with the GPU Whitted renderer and with the oracle's restatement; true when
-// every value is bit-identical.
-bool WhittedMatchesOracle(const GpuBvh& bvh, const RenderConfig& rc, const Camera& cam, std::span<const Triangle> tris,
-                          std::span<const Material> mats, double* sum) {
-    const std::vector<float> img = GpuWhittedRenderer(bvh).Render(rc, cam, tris, mats);
+// The oracle's restatement of the reference's Whitted renderer over the same scene.
+std::vector<float> OracleWhitted(const RenderConfig& rc, const Camera& cam, std::span<const Triangle> tris,
+                                 std::span<const Material> mats) {
     std::vector<float> mrows(16 * mats.size(), 0.0f), vn(tris.size() * 9, 0.0f), xyz(tris.size() * 9);
     std::vector<int32_t> mid(tris.size());
     for (size_t m = 0; m < mats.size(); m++) {
@@ -286,13 +285,22 @@ bool WhittedMatchesOracle(const GpuBvh& bvh, const RenderConfig& rc, const Camer
         }
     }
     void* ploc = orc_ploc_build(xyz.data(), int64_t(tris.size()), 0, 16, 4, 0, 32, nullptr);
-    std::vector<float> ref(img.size());
+    std::vector<float> ref(size_t(rc.width) * rc.height * 3);
     const float bg[3] = {rc.background_color.x, rc.background_color.y, rc.background_color.z};
     const float camv[11] = {cam.pos.x, cam.pos.y, cam.pos.z, cam.dir.x, cam.dir.y, cam.dir.z,
                             cam.up.x,  cam.up.y,  cam.up.z,  cam.fov,   cam.far};
     orc_whitted(ploc, xyz.data(), vn.data(), int64_t(tris.size()), mid.data(), mrows.data(), rc.width, rc.height,
                 rc.max_depth, rc.area_light_sample_cnt, rc.backface_culling ? 1 : 0, bg, camv, 0, ref.data(), 8, nullptr);
     orc_ploc_free(ploc);
+    return ref;
+}
+
+// Renders scene with the GPU Whitted renderer and with the oracle's restatement; true when
+// every value is bit-identical.
+bool WhittedMatchesOracle(const GpuBvh& bvh, const RenderConfig& rc, const Camera& cam, std::span<const Triangle> tris,
+                          std::span<const Material> mats, double* sum) {
+    const std::vector<float> img = GpuWhittedRenderer(bvh).Render(rc, cam, tris, mats);
+    const std::vector<float> ref = OracleWhitted(rc, cam, tris, mats);
     size_t diff = 0;
     *sum = 0.0;
     for (size_t i = 0; i < ref.size(); i++) {
@@ -486,6 +494,60 @@ void GpuChecks() {
                         r.secondary_ray_count, r.shadow_ray_count, r.raytracing_time * 1e3);
             std::printf("loaded scene: %zu triangles, %d nodes, whitted image sum %.4f\n", scene->Triangles().size(),
                         st.accel_stats.node_count, ssum);
+        }
+        // the reference's Renderer interface: load -> GpuRenderer::StartRender(RenderConfig,
+        // AccelerationStructureConfig, Scene, Framebuffer, Stats), as main() drives a renderer
+        for (const char* name : {"CornellBox-Original", "CornellBox-Sphere"}) {
+            sc.name = name;
+            const auto sc2 = CreateSceneLoader(sc)->Load(sc);
+            Expect(sc2.has_value(), std::string("scene loader (renderer): ") + ObjSceneLoader::LastError());
+            if (!sc2) continue;
+            for (AccelerationStructureType type : {AccelerationStructureType::kPlocEmcSobb, AccelerationStructureType::kSbvh}) {
+                RenderConfig rc2;
+                rc2.renderer_type = RendererType::kGpu;
+                rc2.width = 80, rc2.height = 64, rc2.max_depth = 8, rc2.area_light_sample_cnt = 8;
+                rc2.background_color = Vec3{0.1f, 0.2f, 0.3f};
+                AccelerationStructureConfig ac2;
+                ac2.acceleration_structure_type = type;
+                GpuRenderer renderer(ctx);
+                Framebuffer fb;
+                Stats rs;
+                Expect(std::string(renderer.GetName()) == "GPU Raytracer" && renderer.GetProgress() == 0.0f,
+                       "renderer: name and initial progress");
+                const RenderResult res = renderer.StartRender(rc2, ac2, *sc2, fb, rs);
+                Expect(res == RenderResult::kDone, std::string("renderer: StartRender done ") + renderer.LastError());
+                const std::vector<float> ref = OracleWhitted(rc2, sc2->GetCamera(), sc2->Triangles(), sc2->Materials());
+                size_t diff = 0;
+                for (size_t i = 0; i < ref.size() && fb.GetData().size() == ref.size(); i++)
+                    diff += std::memcmp(&fb.GetData()[i], &ref[i], 4) != 0;
+                Expect(fb.GetWidth() == 80 && fb.GetHeight() == 64 && fb.GetData().size() == ref.size() && diff == 0,
+                       std::string("renderer: ") + name + " framebuffer bit-identical to the oracle");
+                const Vec3 p = fb.GetPixel(3, 5);
+                Expect(p.x == ref[(5 * 80 + 3) * 3] && p.z == ref[(5 * 80 + 3) * 3 + 2], "renderer: GetPixel layout");
+                const RaytracingStats& r = rs.rt_stats;
+                Expect(r.primary_ray_count == 80 * 64 && r.secondary_ray_count > 0 && r.shadow_ray_count > 0 &&
+                           r.raytracing_time > 0.0 && renderer.GetProgress() == 1.0f,
+                       "renderer: ray counters and progress");
+                Expect(rs.accel_stats.node_count > 0 && rs.accel_stats.build_time > 0.0 &&
+                           rs.accel_stats.query_count == r.primary_ray_count + r.secondary_ray_count + r.shadow_ray_count,
+                       "renderer: acceleration stats");
+                std::printf("renderer %s: %d primary, %d secondary, %d shadow rays, %.3f ms, %zu values differ\n", name,
+                            r.primary_ray_count, r.secondary_ray_count, r.shadow_ray_count, r.raytracing_time * 1e3,
+                            diff);
+            }
+            GpuRenderer canceled(ctx);
+            canceled.Cancel();
+            Framebuffer fb;
+            Stats rs;
+            Expect(canceled.StartRender(RenderConfig{}, AccelerationStructureConfig{}, *sc2, fb, rs) ==
+                       RenderResult::kCanceled && canceled.GetProgress() == 0.0f,
+                   "renderer: canceled before the launch");
+            canceled.Reset();
+            RenderConfig bad;
+            bad.width = 0;
+            Expect(canceled.StartRender(bad, AccelerationStructureConfig{}, *sc2, fb, rs) == RenderResult::kError &&
+                       !canceled.LastError().empty(),
+                   "renderer: an invalid size is an error");
         }
     }
 }
