@@ -12,12 +12,15 @@
 
 namespace dpl {
 
+// bn / bf: the origin shifted so that (plane - bn) and (plane - bf) are the distances to a
+// box's near and far planes moved outward by po = 4e-7 |o|_1 (the pad of the SOBB slabs).
 struct WtRay {
     int kx, ky, kz;
     float Sx, Sy, Sz;
     float ox, oy, oz;
     float idx, idy, idz;
     float tmin, tmax;
+    float bnx, bny, bnz, bfx, bfy, bfz;
 };
 
 __device__ __forceinline__ float Sel3(float x, float y, float z, int k) { return k == 0 ? x : (k == 1 ? y : z); }
@@ -45,6 +48,10 @@ __device__ __forceinline__ WtRay PrepareRay(float ox, float oy, float oz, float 
     w.idz = 1.0f / dz;
     w.tmin = tmin;
     w.tmax = tmax;
+    const float po = 4e-7f * (fabsf(ox) + fabsf(oy) + fabsf(oz));
+    const float px = w.idx >= 0.0f ? po : -po, py = w.idy >= 0.0f ? po : -po, pz = w.idz >= 0.0f ? po : -po;
+    w.bnx = ox + px, w.bny = oy + py, w.bnz = oz + pz;
+    w.bfx = ox - px, w.bfy = oy - py, w.bfz = oz - pz;
     return w;
 }
 
@@ -87,13 +94,24 @@ __device__ __forceinline__ bool Watertight(const WtRay& w, const float4 p0, cons
     return true;
 }
 
+// Near and far planes are chosen per axis by the sign of 1/d (the same values as the min / max
+// of the two plane distances, since both bounds move monotonically), each moved outward by the
+// ray's pad po = 4e-7 |o|_1: the watertight test's t of a triangle the ray meets right at its
+// origin while nearly parallel to it carries rounding that the box's exact planes do not, and
+// without the pad the box is left before tmin where the triangle is hit after it. A ray
+// parallel to an axis (d = +-0, 1/d = +-inf) whose origin lies exactly on a (padded) plane
+// gives 0 * inf = NaN for that plane's distance; fmaxf / fminf drop a NaN operand, so that
+// plane does not bound the ray, as for a ray running inside the slab (the reference's slab
+// test drops such NaNs by its std::max / std::min argument order, src/acceleration/aabb.h:40-55).
+// A min / max of the pair first would instead let the NaN discard the valid plane.
 __device__ __forceinline__ bool Box(const WtRay& w, float tfar_limit, float x0, float x1, float y0, float y1, float z0,
                                     float z1, float& tnear) {
-    const float tx0 = (x0 - w.ox) * w.idx, tx1 = (x1 - w.ox) * w.idx;
-    const float ty0 = (y0 - w.oy) * w.idy, ty1 = (y1 - w.oy) * w.idy;
-    const float tz0 = (z0 - w.oz) * w.idz, tz1 = (z1 - w.oz) * w.idz;
-    const float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), w.tmin));
-    const float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fminf(fmaxf(tz0, tz1), tfar_limit));
+    const bool px = w.idx >= 0.0f, py = w.idy >= 0.0f, pz = w.idz >= 0.0f;
+    const float tnx = ((px ? x0 : x1) - w.bnx) * w.idx, tfx = ((px ? x1 : x0) - w.bfx) * w.idx;
+    const float tny = ((py ? y0 : y1) - w.bny) * w.idy, tfy = ((py ? y1 : y0) - w.bfy) * w.idy;
+    const float tnz = ((pz ? z0 : z1) - w.bnz) * w.idz, tfz = ((pz ? z1 : z0) - w.bfz) * w.idz;
+    const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, w.tmin));
+    const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tfar_limit));
     tnear = tn;
     return tn <= tf * 1.0000004f;
 }
@@ -129,11 +147,14 @@ __device__ __forceinline__ bool SobbBox(const SobbRay& s, float tmin, float tfar
     const float oa = on(a), ia = s.idn[a * BLOCK];
     const float ob = on(b), ib = s.idn[b * BLOCK];
     const float oc = on(c), ic = s.idn[c * BLOCK];
-    const float ta0 = ((la - s.po) - oa) * ia, ta1 = ((ha + s.po) - oa) * ia;
-    const float tb0 = ((lb - s.po) - ob) * ib, tb1 = ((hb + s.po) - ob) * ib;
-    const float tc0 = ((lc - s.po) - oc) * ic, tc1 = ((hc + s.po) - oc) * ic;
-    const float tn = fmaxf(fmaxf(fminf(ta0, ta1), fminf(tb0, tb1)), fmaxf(fminf(tc0, tc1), tmin));
-    const float tf = fminf(fminf(fmaxf(ta0, ta1), fmaxf(tb0, tb1)), fminf(fmaxf(tc0, tc1), tfar_limit));
+    // near / far slab planes by the sign of 1 / (n . d), NaN distances dropped (as in Box)
+    const float la_ = la - s.po, ha_ = ha + s.po, lb_ = lb - s.po, hb_ = hb + s.po, lc_ = lc - s.po, hc_ = hc + s.po;
+    const bool pa = ia >= 0.0f, pb = ib >= 0.0f, pc = ic >= 0.0f;
+    const float tna = ((pa ? la_ : ha_) - oa) * ia, tfa = ((pa ? ha_ : la_) - oa) * ia;
+    const float tnb = ((pb ? lb_ : hb_) - ob) * ib, tfb = ((pb ? hb_ : lb_) - ob) * ib;
+    const float tnc = ((pc ? lc_ : hc_) - oc) * ic, tfc = ((pc ? hc_ : lc_) - oc) * ic;
+    const float tn = fmaxf(fmaxf(tna, tnb), fmaxf(tnc, tmin));
+    const float tf = fminf(fminf(tfa, tfb), fminf(tfc, tfar_limit));
     tnear = tn;
     return tn <= tf * 1.0000004f;
 }

@@ -292,6 +292,8 @@ struct WtRay {
     float Sx, Sy, Sz;
     float ox, oy, oz;
     float idx, idy, idz;
+    // origins for the box's near / far plane distances, planes padded by po = 4e-7 |o|_1
+    float bn[3], bf[3];
 };
 WtRay PrepareRay(const RayRec& r);
 // Returns true and (t, b1, b2) when the ray hits the triangle; no t-range check.

@@ -29,6 +29,13 @@ WtRay PrepareRay(const RayRec& r) {
     w.idx = 1.0f / r.dx;
     w.idy = 1.0f / r.dy;
     w.idz = 1.0f / r.dz;
+    const float po = 4e-7f * (std::fabs(r.ox) + std::fabs(r.oy) + std::fabs(r.oz));
+    const float o[3] = {r.ox, r.oy, r.oz}, id[3] = {w.idx, w.idy, w.idz};
+    for (int a = 0; a < 3; a++) {
+        const float p = id[a] >= 0.0f ? po : -po;
+        w.bn[a] = o[a] + p;
+        w.bf[a] = o[a] - p;
+    }
     return w;
 }
 
@@ -72,12 +79,20 @@ bool IntersectWatertight(const WtRay& w, const float* v0, const float* v1, const
     return true;
 }
 
+// Near / far plane per axis by the sign of 1/d, both moved outward by po = 4e-7 |o|_1 (through
+// the shifted origins bn / bf); a NaN distance (origin on a plane, d = +-0 on that axis:
+// 0 * inf) is dropped by std::fmax / std::fmin, so the plane does not bound the ray (the
+// reference's slab test drops it by its std::max / std::min argument order, aabb.h:40-55).
 bool IntersectBoxRobust(const WtRay& w, float tmin, float tmax, const float lo[3], const float hi[3], float& tnear) {
-    float tx0 = (lo[0] - w.ox) * w.idx, tx1 = (hi[0] - w.ox) * w.idx;
-    float ty0 = (lo[1] - w.oy) * w.idy, ty1 = (hi[1] - w.oy) * w.idy;
-    float tz0 = (lo[2] - w.oz) * w.idz, tz1 = (hi[2] - w.oz) * w.idz;
-    float tn = std::fmax(std::fmax(std::fmin(tx0, tx1), std::fmin(ty0, ty1)), std::fmax(std::fmin(tz0, tz1), tmin));
-    float tf = std::fmin(std::fmin(std::fmax(tx0, tx1), std::fmax(ty0, ty1)), std::fmin(std::fmax(tz0, tz1), tmax));
+    const float id[3] = {w.idx, w.idy, w.idz};
+    float tn_[3], tf_[3];
+    for (int a = 0; a < 3; a++) {
+        const bool pos = id[a] >= 0.0f;
+        tn_[a] = ((pos ? lo[a] : hi[a]) - w.bn[a]) * id[a];
+        tf_[a] = ((pos ? hi[a] : lo[a]) - w.bf[a]) * id[a];
+    }
+    const float tn = std::fmax(std::fmax(tn_[0], tn_[1]), std::fmax(tn_[2], tmin));
+    const float tf = std::fmin(std::fmin(tf_[0], tf_[1]), std::fmin(tf_[2], tmax));
     tnear = tn;
     return tn <= tf * 1.0000004f;
 }
@@ -102,11 +117,16 @@ static SobbRay PrepareSobb(const SobbSet& S, const RayRec& r) {
 
 static bool IntersectSobb(const SobbRay& s, float tmin, float tmax, const float* f, uint32_t dirs, float& tnear) {
     const int a = dirs & 15, b = (dirs >> 4) & 15, c = (dirs >> 8) & 15;
-    const float ta0 = ((f[0] - s.po) - s.on[a]) * s.idn[a], ta1 = ((f[1] + s.po) - s.on[a]) * s.idn[a];
-    const float tb0 = ((f[2] - s.po) - s.on[b]) * s.idn[b], tb1 = ((f[3] + s.po) - s.on[b]) * s.idn[b];
-    const float tc0 = ((f[4] - s.po) - s.on[c]) * s.idn[c], tc1 = ((f[5] + s.po) - s.on[c]) * s.idn[c];
-    const float tn = std::fmax(std::fmax(std::fmin(ta0, ta1), std::fmin(tb0, tb1)), std::fmax(std::fmin(tc0, tc1), tmin));
-    const float tf = std::fmin(std::fmin(std::fmax(ta0, ta1), std::fmax(tb0, tb1)), std::fmin(std::fmax(tc0, tc1), tmax));
+    const int k[3] = {a, b, c};
+    float tn_[3], tf_[3];
+    for (int q = 0; q < 3; q++) {  // near / far slab plane by the sign of 1 / (n . d), NaNs dropped
+        const float lo = f[2 * q] - s.po, hi = f[2 * q + 1] + s.po, id = s.idn[k[q]], on = s.on[k[q]];
+        const bool pos = id >= 0.0f;
+        tn_[q] = ((pos ? lo : hi) - on) * id;
+        tf_[q] = ((pos ? hi : lo) - on) * id;
+    }
+    const float tn = std::fmax(std::fmax(tn_[0], tn_[1]), std::fmax(tn_[2], tmin));
+    const float tf = std::fmin(std::fmin(tf_[0], tf_[1]), std::fmin(tf_[2], tmax));
     tnear = tn;
     return tn <= tf * 1.0000004f;
 }

@@ -198,6 +198,28 @@ def test_reference_brute_force_classification(oracle, ctx):
         assert len(diff) < 0.02 * len(rays)
 
 
+@pytest.mark.parametrize("bv", [0, 1])
+def test_trace_conservative_for_in_plane_and_grazing_rays(oracle, ctx, bv):
+    """Rays that run in a wall's plane or graze a surface at their origin: the two-child, wide
+    and counting traces all give the brute-force hits."""
+    from diplodocus_amd import _lib
+
+    from .test_oracle_sbvh import cornell_surface_rays
+
+    tris, rays, shadow = cornell_surface_rays(60000, 3)
+    say(f"in-plane rays bv={bv}")
+    brute = oracle.trace_brute(tris, rays)
+    bany = oracle.trace_brute(tris, shadow, 1)
+    g = gpu_bvh(ctx, tris, bv=bv)
+    assert hits_equal(g.trace(rays), brute).all()
+    assert ((g.trace(shadow, 1)["prim"] >= 0) == (bany["prim"] >= 0)).all()
+    h, _, _ = g.trace_counters(rays)
+    assert hits_equal(h, brute).all()
+    g.widen(4)
+    assert hits_equal(g.trace(rays, _lib.DPL_TRACE_WIDE), brute).all()
+    g.close()
+
+
 def test_trace_error_paths(ctx):
     from diplodocus_amd import accel
 

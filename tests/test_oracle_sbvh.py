@@ -99,3 +99,52 @@ def test_sah_cost_closed_form(oracle):
     for k, (beg, cnt) in enumerate(bc):
         tot += sa(mm[k].astype(np.float64)) * (cnt if cnt else 1)
     assert sb.sah() == pytest.approx(tot / sa(mm[0].astype(np.float64)), rel=1e-12)
+
+
+def cornell_surface_rays(n, seed):
+    """Rays between random points on the Cornell room's axis-aligned surfaces: many run exactly
+    in a wall's plane (a zero direction component, the origin on a box face) or leave a surface
+    nearly parallel to it, the cases a slab test must not cull."""
+    from diplodocus_amd import scenes
+
+    tris = scenes.cornell_room().tris
+    rng = np.random.default_rng(seed)
+
+    def surf(idx):
+        t = tris[idx].reshape(-1, 3, 3).astype(np.float64)
+        u, v = rng.random(len(idx)), rng.random(len(idx))
+        m = u + v > 1
+        u[m], v[m] = 1 - u[m], 1 - v[m]
+        return t[:, 0] + (t[:, 1] - t[:, 0]) * u[:, None] + (t[:, 2] - t[:, 0]) * v[:, None]
+
+    a, b = surf(rng.integers(0, len(tris), n)), surf(rng.integers(0, len(tris), n))
+    d = b - a
+    L = np.linalg.norm(d, axis=1)
+    ok = L > 1e-3
+    a, d, L = a[ok], d[ok] / L[ok, None], L[ok]
+    rays = np.zeros((len(a), 8), np.float32)
+    rays[:, 0:3] = a + d * 1e-3
+    rays[:, 3] = 1e-6
+    rays[:, 4:7] = d
+    rays[:, 7] = np.float32(3e38)
+    shadow = rays.copy()
+    shadow[:, 7] = (L * 0.999).astype(np.float32)
+    return tris, rays, shadow
+
+
+def test_traversal_is_conservative_for_in_plane_and_grazing_rays(oracle):
+    """Every tree form (AABB, SOBB, wide 4 / 8 of both) finds exactly the brute-force hits of
+    rays that run in a box face plane or graze a surface at their origin."""
+    tris, rays, shadow = cornell_surface_rays(60000, 3)
+    assert (rays[:, 4:7] == 0).any(axis=1).mean() > 0.05  # axis-parallel directions are common
+    brute = oracle.trace_brute(tris, rays)
+    bany = oracle.trace_brute(tris, shadow, 1)
+    for bv in (0, 1):
+        p = oracle.Ploc(tris, 0, 16, 4, bv_type=bv)
+        assert (p.trace(rays).view(np.uint32) == brute.view(np.uint32)).all(), bv
+        assert ((p.trace(shadow, 1)["prim"] >= 0) == (bany["prim"] >= 0)).all(), bv
+        a = oracle.refit_aabb(p.fnodes, p.tri_ordered)
+        for width in (4, 8):
+            wn = oracle.widen(a, width)
+            h, _, _ = oracle.trace_wide(wn, width, p.tri_ordered, rays)
+            assert (h.view(np.uint32) == brute.view(np.uint32)).all(), (bv, width)
