@@ -88,6 +88,106 @@ def measured_build_traffic(variant: str, radius: int, build_ms: float):
     return None
 
 
+def _kname(full: str) -> str:
+    """Kernel name without return type, namespace and argument list (rocprofv3 reports the
+    demangled signature, e.g. `void dpl::KCodes<1>(float const*, ...)`)."""
+    full = full.strip()
+    if full.startswith("void "):
+        full = full[5:]
+    if full.startswith("dpl::"):
+        full = full[5:]
+    depth = 0
+    for i, ch in enumerate(full):
+        if ch == "<":
+            depth += 1
+        elif ch == ">":
+            depth -= 1
+        elif ch == "(" and depth == 0:
+            return full[:i].strip()
+    return full.strip()
+
+
+def pmc_summary(dirs, variant: str, radius: int, n_tris: int = 2_000_002, n_rays: int = 4_194_304) -> str:
+    """Markdown summary of rocprofv3 --pmc passes over `bench.py --minimal --calibrate-copy`
+    (counter_collection.csv files under `dirs`, one pass each). HBM-side bytes per dispatch
+    come from the L2-to-fabric request counters by request size: reads = 32 RDREQ_32B +
+    64 RDREQ_64B + 128 RDREQ_128B, writes = 32 (WRREQ - WRREQ_64B) + 64 WRREQ_64B. The
+    build is the first run of dispatches from KBuildInit to KFinish; the trace is the first
+    KTrace dispatch after it."""
+    import csv
+    import glob
+
+    disp = {}
+    for d in dirs:
+        for f in sorted(glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)):
+            for row in csv.DictReader(open(f)):
+                k = int(row["Dispatch_Id"])
+                e = disp.setdefault(k, {"kernel": _kname(row["Kernel_Name"]), "c": {}})
+                if e["kernel"] != _kname(row["Kernel_Name"]):
+                    raise ValueError(f"dispatch {k} differs between passes: {e['kernel']} / {row['Kernel_Name']}")
+                e["c"][row["Counter_Name"]] = e["c"].get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+    order = [disp[k] for k in sorted(disp)]
+
+    def rd(c):
+        return 32 * c.get("TCC_EA0_RDREQ_32B_sum", 0) + 64 * c.get("TCC_EA0_RDREQ_64B_sum", 0) + \
+            128 * c.get("TCC_EA0_RDREQ_128B_sum", 0)
+
+    def wr(c):
+        w, w64 = c.get("TCC_EA0_WRREQ_sum", 0), c.get("TCC_EA0_WRREQ_64B_sum", 0)
+        return 32 * (w - w64) + 64 * w64
+
+    start = next(i for i, e in enumerate(order) if e["kernel"] == "KBuildInit")
+    end = next(i for i in range(start, len(order)) if order[i]["kernel"] == "KFinish")
+    build = order[start:end + 1]
+    per = {}
+    for e in build:
+        t = per.setdefault(e["kernel"], [0, 0.0, 0.0])
+        t[0] += 1
+        t[1] += rd(e["c"])
+        t[2] += wr(e["c"])
+    tot_r = sum(v[1] for v in per.values())
+    tot_w = sum(v[2] for v in per.values())
+    out = [f"Dispatches of the first {n_tris:,}-triangle build ({variant} r{radius}): {len(build)}.", "",
+           "| kernel | dispatches | read MB | written MB | read+written per dispatch (MB) |", "|---|---|---|---|---|"]
+    for k, (cnt, r, w) in sorted(per.items(), key=lambda kv: -(kv[1][1] + kv[1][2])):
+        out.append(f"| `{k}` | {cnt} | {r / 1e6:.2f} | {w / 1e6:.2f} | {(r + w) / cnt / 1e6:.3f} |")
+    out += ["", f"Whole build: {tot_r / 1e6:.1f} MB read + {tot_w / 1e6:.1f} MB written = "
+                f"{(tot_r + tot_w) / 1e6:.1f} MB.", ""]
+    merging = [e for e in build if e["kernel"].startswith("KPlocIteration")]
+    # pass-through launches (clusters already at the tail size) move almost nothing
+    merging = [e for e in merging if rd(e["c"]) + wr(e["c"]) > 256 * 1024]
+    if merging:
+        pl = sum(rd(e["c"]) + wr(e["c"]) for e in merging) / len(merging)
+        out.append(f"`KPlocIteration` merging launches: {len(merging)}, {pl / 1e6:.2f} MB per launch.")
+        out.append("")
+        out.append(f"traffic_bytes_per_launch: {variant} r{radius} = {int(round(pl))}")
+    out.append(f"build_traffic_bytes: {variant} r{radius} = {int(round(tot_r + tot_w))}")
+    tr = next((e for e in order[end:] if e["kernel"].startswith("KTrace<")), None)
+    if tr is not None:
+        c = tr["c"]
+        out += ["", f"## `{tr['kernel']}` ({n_rays:,} rays over the build's tree)", "",
+                f"read {rd(c) / 1e6:.1f} MB + written {wr(c) / 1e6:.1f} MB = "
+                f"{(rd(c) + wr(c)) / n_rays:.1f} B per ray beyond L2"]
+        if c.get("TCC_HIT_sum", 0) + c.get("TCC_MISS_sum", 0) > 0:
+            h, m = c["TCC_HIT_sum"], c["TCC_MISS_sum"]
+            out.append(f"L2 hit ratio {h / (h + m):.4f} ({h:.3e} hits, {m:.3e} misses)")
+        if c.get("SQ_ACTIVE_INST_VALU", 0) > 0:
+            lanes = c["SQ_THREAD_CYCLES_VALU"] / (64 * c["SQ_ACTIVE_INST_VALU"])
+            out.append(f"VALU lanes active per VALU instruction {lanes:.3f}; SQ_INSTS_VALU {c['SQ_INSTS_VALU']:.4e}, "
+                       f"SQ_INSTS_VMEM_RD {c['SQ_INSTS_VMEM_RD']:.4e}, SQ_WAVES {c['SQ_WAVES']:.0f}, "
+                       f"SQ_WAVE_CYCLES {c['SQ_WAVE_CYCLES']:.4e}, SQ_BUSY_CYCLES {c['SQ_BUSY_CYCLES']:.4e}, "
+                       f"GRBM_GUI_ACTIVE {c['GRBM_GUI_ACTIVE']:.4e}")
+        for key in ("TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum"):
+            if key in c:
+                out.append(f"{key} {c[key]:.4e}")
+    cp = [e for e in order if "copyBuffer" in e["kernel"]]
+    if cp:
+        c = cp[-1]["c"]
+        out += ["", f"Copy check: a 268,435,456-byte device copy reads {rd(c) / 268435456:.4f}x and writes "
+                    f"{wr(c) / 268435456:.4f}x its size by these counters."]
+    return "\n".join(out)
+
+
 def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -104,7 +204,16 @@ def main() -> int:
     ap.add_argument("--no-batch", action="store_true", help="skip the 4096-mesh batched build and trace")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run three plain 256 MiB device copies (a known-size kernel for HBM counter scaling)")
+    ap.add_argument("--minimal", action="store_true",
+                    help="only the timed steps, the traversal counters and the roofline build (for profiler runs)")
+    ap.add_argument("--pmc-summary", nargs="+", metavar="DIR",
+                    help="summarise rocprofv3 --pmc output directories of a --minimal run and exit (no GPU)")
     args = ap.parse_args()
+    if args.minimal:
+        args.no_batch = args.no_sweep = args.no_cpu = True
+    if args.pmc_summary:
+        print(pmc_summary(args.pmc_summary, args.variant, args.radius))
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (every box has one GPU)",
               file=sys.stderr)
@@ -171,32 +280,34 @@ def main() -> int:
     _, visits, tests = bvh.trace_counters(rays)
     touched = (visits * 64 + tests * 48) / nrays
 
-    # ---- wide trees of the same BVH (4 and 8 children, 8-bit quantized child boxes): the
-    #      conversion (SOBB trees refitted to axis-aligned boxes first) and the closest-hit
-    #      trace of the same rays, whose hits must equal the two-child trace's ----
     from diplodocus_amd import _lib
 
-    two_child_hits = hits.clone()
     wide = {}
-    for width in (4, 8):
-        bvh.widen(width)
-        widen_ms = float(np.median([bvh.widen(width)["widen_ms"] for _ in range(3)]))
-        bvh.trace(rays, _lib.DPL_TRACE_WIDE, hits)
-        w0, w1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        w0.record(stream)
-        for _ in range(5):
+    if not args.minimal:
+        # ---- wide trees of the same BVH (4 and 8 children, 8-bit quantized child boxes): the
+        #      conversion (SOBB trees refitted to axis-aligned boxes first) and the closest-hit
+        #      trace of the same rays, whose hits must equal the two-child trace's ----
+
+        two_child_hits = hits.clone()
+        for width in (4, 8):
+            bvh.widen(width)
+            widen_ms = float(np.median([bvh.widen(width)["widen_ms"] for _ in range(3)]))
             bvh.trace(rays, _lib.DPL_TRACE_WIDE, hits)
-        w1.record(stream)
-        torch.cuda.synchronize()
-        wide_ms = w0.elapsed_time(w1) / 5
-        equal = bool(torch.equal(hits.view(torch.int32), two_child_hits.view(torch.int32)))
-        _, wv, wt = bvh.trace_counters(rays, _lib.DPL_TRACE_WIDE)
-        wide[f"width{width}"] = {"node_bytes": bvh.wide_info()["node_bytes"], "widen_ms": round(widen_ms, 4),
-                                 "trace_ms": round(wide_ms, 4), "trace_mrays_s": round(nrays / (wide_ms * 1e-3) / 1e6, 3),
-                                 "nodes_per_ray": round(wv / nrays, 2), "tris_per_ray": round(wt / nrays, 2),
-                                 "hits_equal_two_child": equal}
-        log(f"wide{width}: widen {widen_ms:.3f} ms, trace {wide_ms:.3f} ms = {nrays / wide_ms / 1e3:.1f} Mrays/s, "
-            f"{wv / nrays:.1f} nodes and {wt / nrays:.1f} tris per ray, hits equal: {equal}")
+            w0, w1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            w0.record(stream)
+            for _ in range(5):
+                bvh.trace(rays, _lib.DPL_TRACE_WIDE, hits)
+            w1.record(stream)
+            torch.cuda.synchronize()
+            wide_ms = w0.elapsed_time(w1) / 5
+            equal = bool(torch.equal(hits.view(torch.int32), two_child_hits.view(torch.int32)))
+            _, wv, wt = bvh.trace_counters(rays, _lib.DPL_TRACE_WIDE)
+            wide[f"width{width}"] = {"node_bytes": bvh.wide_info()["node_bytes"], "widen_ms": round(widen_ms, 4),
+                                     "trace_ms": round(wide_ms, 4), "trace_mrays_s": round(nrays / (wide_ms * 1e-3) / 1e6, 3),
+                                     "nodes_per_ray": round(wv / nrays, 2), "tris_per_ray": round(wt / nrays, 2),
+                                     "hits_equal_two_child": equal}
+            log(f"wide{width}: widen {widen_ms:.3f} ms, trace {wide_ms:.3f} ms = {nrays / wide_ms / 1e3:.1f} Mrays/s, "
+                f"{wv / nrays:.1f} nodes and {wt / nrays:.1f} tris per ray, hits equal: {equal}")
 
     # ---- roofline of the PLOC iteration kernel: one profiled build (per-launch events) ----
     prof = accel.GpuBvh(ctx).build(tris, accel.make_params(morton_type=mt, radius=args.radius, max_leaf=4,
@@ -224,90 +335,95 @@ def main() -> int:
             f"{achieved:.1f} GB/s algorithmic")
     prof.close()
 
-    # ---- primary rays: 1024x1024 camera of the icosphere room (PLOC, Morton30, r=16) ----
-    room = scenes.icosphere_room()
-    cam_rays = torch.from_numpy(scenes.camera_rays(room.camera, 1024, 1024)).to(dev)
-    cam_hits = torch.empty((cam_rays.shape[0], 4), dtype=torch.float32, device=dev)
-    rb = accel.GpuBvh(ctx).build(torch.from_numpy(room.tris).to(dev), accel.make_params(0, 16, 4))
-    for _ in range(3):
-        rb.trace(cam_rays, 0, cam_hits)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    reps = 20
-    e0.record(stream)
-    for _ in range(reps):
-        rb.trace(cam_rays, 0, cam_hits)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    prim_ms = e0.elapsed_time(e1) / reps
-    prim_mrays = cam_rays.shape[0] / (prim_ms * 1e-3) / 1e6
-    _, pv, pt = rb.trace_counters(cam_rays)
-    log(f"primary rays: {prim_ms:.3f} ms per 1M rays = {prim_mrays:.1f} Mrays/s, {pv / cam_rays.shape[0]:.1f} "
-        f"nodes and {pt / cam_rays.shape[0]:.1f} tris per ray")
-    rb.widen(4)
-    for _ in range(3):
-        rb.trace(cam_rays, _lib.DPL_TRACE_WIDE, cam_hits)
-    e0.record(stream)
-    for _ in range(reps):
-        rb.trace(cam_rays, _lib.DPL_TRACE_WIDE, cam_hits)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    prim_wide_mrays = cam_rays.shape[0] / (e0.elapsed_time(e1) / reps * 1e-3) / 1e6
-    wide["primary_width4_mrays_s"] = round(prim_wide_mrays, 3)
-    log(f"primary rays, wide4: {prim_wide_mrays:.1f} Mrays/s")
-
-    # ---- path tracing on the same room: 1024x1024 primary + 1 diffuse bounce, and the
-    #      differentiable case (512x512, 4 spp, depth 4): forward and adjoint path replay ----
-    from diplodocus_amd import render
-
-    mat_id = torch.from_numpy(room.mat_id).to(dev)
-    albedo = torch.from_numpy(room.albedo).to(dev)
-    emission = torch.from_numpy(room.emission).to(dev)
-
-    def timed(fn, reps):
-        fn()
-        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        for _ in range(reps):
-            fn()
-        b.record(stream)
-        torch.cuda.synchronize()
-        return a.elapsed_time(b) / reps
-
-    rp_b = render.make_render_params(1024, 1024, 1, 2, 1, room.camera, len(room.albedo))
-    img_b = torch.empty((1024, 1024, 3), dtype=torch.float32, device=dev)
-    bounce_ms = timed(lambda: render.render_forward(rb, rp_b, mat_id, albedo, emission, img_b), 10)
-    rp = render.make_render_params(512, 512, 4, 4, 1, room.camera, len(room.albedo))
-    gen = torch.Generator().manual_seed(5)
-    target_albedo = (albedo.cpu() + 0.05 * torch.randn(albedo.shape, generator=gen)).clamp(0, 1).to(dev)
-    target = render.render_forward(rb, rp, mat_id, target_albedo, emission)
-    img = torch.empty((512, 512, 3), dtype=torch.float32, device=dev)
-    fwd_ms = timed(lambda: render.render_forward(rb, rp, mat_id, albedo, emission, img), 10)
-    d_img = (2.0 * (img - target)).contiguous()
-    adj_ms = timed(lambda: render.render_adjoint(rb, rp, mat_id, albedo, emission, d_img), 10)
-    diff = {"workload": "icosphere_room 69,668 tris, 16 materials, 512x512 x 4 spp, depth 4, L2 loss vs albedo "
-                        "+ N(0, 0.05) target; gradients for 16x3 albedo and emission",
-            "forward_ms": round(fwd_ms, 4), "adjoint_ms": round(adj_ms, 4),
-            "fwd_plus_adjoint_ms": round(fwd_ms + adj_ms, 4),
-            "adjoint_over_forward": round(adj_ms / fwd_ms, 3)}
-    log(f"path tracing: 1024x1024 + 1 bounce {bounce_ms:.3f} ms; 512x512x4 depth 4 forward {fwd_ms:.3f} ms, "
-        f"adjoint {adj_ms:.3f} ms ({adj_ms / fwd_ms:.2f}x)")
-    rb.close()
-
-    # ---- Whitted ray tracing of the 36-triangle Cornell room (256x256, depth 3, 8 light
-    #      samples per hit), the reference's CPU renderer's own case ----
-    cs, cmats = scenes.cornell_whitted()
-    cb = accel.GpuBvh(ctx).build(torch.from_numpy(cs.tris).to(dev), accel.make_params(0, 16, 4))
-    c_tris, c_mat = torch.from_numpy(cs.tris).to(dev), torch.from_numpy(cs.mat_id).to(dev)
-    c_mats = torch.from_numpy(cmats).to(dev)
-    c_lights = torch.from_numpy(render.emissive_triangles(cs.mat_id, cmats)).to(dev)
+    prim_mrays = bounce_ms = None
+    diff = None
     whitted = {}
-    for side in (256, 1024):
-        wp = render.make_whitted_params(side, side, 3, 8, cs.camera)
-        wimg = torch.empty((side, side, 3), dtype=torch.float32, device=dev)
-        ms = timed(lambda: render.render_whitted(cb, wp, c_tris, c_mat, c_mats, c_lights, image=wimg), 10)
-        whitted[f"{side}x{side}_ms"] = round(ms, 4)
-    cb.close()
-    log(f"whitted cornell: 256x256 {whitted['256x256_ms']:.3f} ms, 1024x1024 {whitted['1024x1024_ms']:.3f} ms")
+    cs = cmats = None
+    if not args.minimal:
+        # ---- primary rays: 1024x1024 camera of the icosphere room (PLOC, Morton30, r=16) ----
+        room = scenes.icosphere_room()
+        cam_rays = torch.from_numpy(scenes.camera_rays(room.camera, 1024, 1024)).to(dev)
+        cam_hits = torch.empty((cam_rays.shape[0], 4), dtype=torch.float32, device=dev)
+        rb = accel.GpuBvh(ctx).build(torch.from_numpy(room.tris).to(dev), accel.make_params(0, 16, 4))
+        for _ in range(3):
+            rb.trace(cam_rays, 0, cam_hits)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = 20
+        e0.record(stream)
+        for _ in range(reps):
+            rb.trace(cam_rays, 0, cam_hits)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        prim_ms = e0.elapsed_time(e1) / reps
+        prim_mrays = cam_rays.shape[0] / (prim_ms * 1e-3) / 1e6
+        _, pv, pt = rb.trace_counters(cam_rays)
+        log(f"primary rays: {prim_ms:.3f} ms per 1M rays = {prim_mrays:.1f} Mrays/s, {pv / cam_rays.shape[0]:.1f} "
+            f"nodes and {pt / cam_rays.shape[0]:.1f} tris per ray")
+        rb.widen(4)
+        for _ in range(3):
+            rb.trace(cam_rays, _lib.DPL_TRACE_WIDE, cam_hits)
+        e0.record(stream)
+        for _ in range(reps):
+            rb.trace(cam_rays, _lib.DPL_TRACE_WIDE, cam_hits)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        prim_wide_mrays = cam_rays.shape[0] / (e0.elapsed_time(e1) / reps * 1e-3) / 1e6
+        wide["primary_width4_mrays_s"] = round(prim_wide_mrays, 3)
+        log(f"primary rays, wide4: {prim_wide_mrays:.1f} Mrays/s")
+
+        # ---- path tracing on the same room: 1024x1024 primary + 1 diffuse bounce, and the
+        #      differentiable case (512x512, 4 spp, depth 4): forward and adjoint path replay ----
+        from diplodocus_amd import render
+
+        mat_id = torch.from_numpy(room.mat_id).to(dev)
+        albedo = torch.from_numpy(room.albedo).to(dev)
+        emission = torch.from_numpy(room.emission).to(dev)
+
+        def timed(fn, reps):
+            fn()
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(reps):
+                fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            return a.elapsed_time(b) / reps
+
+        rp_b = render.make_render_params(1024, 1024, 1, 2, 1, room.camera, len(room.albedo))
+        img_b = torch.empty((1024, 1024, 3), dtype=torch.float32, device=dev)
+        bounce_ms = timed(lambda: render.render_forward(rb, rp_b, mat_id, albedo, emission, img_b), 10)
+        rp = render.make_render_params(512, 512, 4, 4, 1, room.camera, len(room.albedo))
+        gen = torch.Generator().manual_seed(5)
+        target_albedo = (albedo.cpu() + 0.05 * torch.randn(albedo.shape, generator=gen)).clamp(0, 1).to(dev)
+        target = render.render_forward(rb, rp, mat_id, target_albedo, emission)
+        img = torch.empty((512, 512, 3), dtype=torch.float32, device=dev)
+        fwd_ms = timed(lambda: render.render_forward(rb, rp, mat_id, albedo, emission, img), 10)
+        d_img = (2.0 * (img - target)).contiguous()
+        adj_ms = timed(lambda: render.render_adjoint(rb, rp, mat_id, albedo, emission, d_img), 10)
+        diff = {"workload": "icosphere_room 69,668 tris, 16 materials, 512x512 x 4 spp, depth 4, L2 loss vs albedo "
+                            "+ N(0, 0.05) target; gradients for 16x3 albedo and emission",
+                "forward_ms": round(fwd_ms, 4), "adjoint_ms": round(adj_ms, 4),
+                "fwd_plus_adjoint_ms": round(fwd_ms + adj_ms, 4),
+                "adjoint_over_forward": round(adj_ms / fwd_ms, 3)}
+        log(f"path tracing: 1024x1024 + 1 bounce {bounce_ms:.3f} ms; 512x512x4 depth 4 forward {fwd_ms:.3f} ms, "
+            f"adjoint {adj_ms:.3f} ms ({adj_ms / fwd_ms:.2f}x)")
+        rb.close()
+
+        # ---- Whitted ray tracing of the 36-triangle Cornell room (256x256, depth 3, 8 light
+        #      samples per hit), the reference's CPU renderer's own case ----
+        cs, cmats = scenes.cornell_whitted()
+        cb = accel.GpuBvh(ctx).build(torch.from_numpy(cs.tris).to(dev), accel.make_params(0, 16, 4))
+        c_tris, c_mat = torch.from_numpy(cs.tris).to(dev), torch.from_numpy(cs.mat_id).to(dev)
+        c_mats = torch.from_numpy(cmats).to(dev)
+        c_lights = torch.from_numpy(render.emissive_triangles(cs.mat_id, cmats)).to(dev)
+        whitted = {}
+        for side in (256, 1024):
+            wp = render.make_whitted_params(side, side, 3, 8, cs.camera)
+            wimg = torch.empty((side, side, 3), dtype=torch.float32, device=dev)
+            ms = timed(lambda: render.render_whitted(cb, wp, c_tris, c_mat, c_mats, c_lights, image=wimg), 10)
+            whitted[f"{side}x{side}_ms"] = round(ms, 4)
+        cb.close()
+        log(f"whitted cornell: 256x256 {whitted['256x256_ms']:.3f} ms, 1024x1024 {whitted['1024x1024_ms']:.3f} ms")
 
     # ---- batched builds: 4096 meshes (log-uniform 64..65536 tris) in one persistent
     #      PLOC+EMC launch under autotuned radius / block / digit width, then a
@@ -455,10 +571,10 @@ def main() -> int:
                                "4,194,304 rays from a bounding sphere toward the centre",
                    "variant": args.variant, "radius": args.radius, "max_leaf": 4},
         "trace_mrays_s": round(mrays, 3),
-        "trace_wide4_mrays_s": wide["width4"]["trace_mrays_s"],
-        "primary_mrays_s": round(prim_mrays, 3),
+        "trace_wide4_mrays_s": wide["width4"]["trace_mrays_s"] if wide else None,
+        "primary_mrays_s": round(prim_mrays, 3) if prim_mrays else None,
         "primary_workload": "icosphere_room 69,668 tris, 1024x1024 camera rays, PLOC Morton30 r=16",
-        "primary_bounce_ms": round(bounce_ms, 4),
+        "primary_bounce_ms": round(bounce_ms, 4) if bounce_ms else None,
         "differentiable": diff,
         "whitted_cornell": dict(whitted, workload="cornell room 36 tris, Whitted depth 3, 8 area-light samples "
                                                   "per hit (the reference CPU renderer's case)"),
