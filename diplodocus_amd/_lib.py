@@ -117,6 +117,7 @@ def _load() -> C.CDLL:
         "dpl_bvh_get_layout": (I32, [P, C.POINTER(BvhLayout)]),
         "dpl_bvh_download": (I32, [P, P, P, P, P, P, P, P]),
         "dpl_trace": (I32, [P, P, P, I64, U32, P, I32]),
+        "dpl_trace_noalloc": (I32, [P, P, P, I64, U32, P]),
         "dpl_trace_counters": (I32, [P, P, P, I64, U32, P, I32, P, P]),
         "dpl_trace_batched": (I32, [P, P, P, P, I64, U32, P, I32]),
         "dpl_compute_codes": (I32, [P, P, I64, I32, P, P]),
@@ -154,5 +155,5 @@ EXPORTED = [
     "dpl_bvh_get_layout", "dpl_bvh_download", "dpl_compute_codes", "dpl_radix_sort_pairs", "dpl_render_forward",
     "dpl_render_adjoint", "dpl_bvh_get_meshes", "dpl_trace_graph_create", "dpl_trace_graph_launch",
     "dpl_trace_graph_destroy", "dpl_autotune", "dpl_render_whitted", "dpl_render_whitted_stats", "dpl_bvh_widen",
-    "dpl_bvh_get_wide", "dpl_bvh_download_wide", "dpl_bvh_sah_aabb",
+    "dpl_bvh_get_wide", "dpl_bvh_download_wide", "dpl_bvh_sah_aabb", "dpl_trace_noalloc",
 ]

@@ -10,6 +10,7 @@
 #include <fstream>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <stdexcept>
 #include <string>
@@ -148,6 +149,34 @@ struct DevBuf {
 
 using namespace dpl;
 
+// Scratch of dpl_trace_noalloc: the ray counter (first 256 bytes) and the traversal spill
+// stacks, one copy per device for the life of the process. Host code uses only the device
+// address that hipGetSymbolAddress gives for the current device (kept per device below), never
+// the array's name.
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+constexpr size_t kNoAllocCounterBytes = 256;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+std::mutex g_scratch_mu;
+std::vector<uint8_t*> g_scratch_addr;  // per device, null until looked up
+
+// Called with `device` current, from dpl_context_create.
+uint8_t* ScratchAddress(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (size_t(device) >= g_scratch_addr.size()) g_scratch_addr.resize(size_t(device) + 1, nullptr);
+    if (!g_scratch_addr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            throw Error(DPL_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        }
+        g_scratch_addr[device] = static_cast<uint8_t*>(p);
+    }
+    return g_scratch_addr[device];
+}
+}  // namespace
+
 struct dpl_context {
     int device = 0;
     int cu_count = 0;
@@ -161,6 +190,7 @@ struct dpl_context {
     // batched builds and traces
     DevBuf b_off, b_foff, b_order, b_results, b_queue, ray_mesh;
     std::string device_name;
+    uint8_t* noalloc_scratch = nullptr;  // this device's address of the no-allocation scratch
     int sobb_kdop = -1;  // kdop_size the uploaded direction set was made for
     void* pinned = nullptr;          // host staging for small results (device-visible)
     int32_t* pinned_dev = nullptr;   // device address of `pinned`
@@ -573,6 +603,7 @@ dpl_status dpl_context_create(int32_t device, dpl_context** out) {
         DPL_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocMapped));
         DPL_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->pinned_dev), ctx->pinned, 0));
         ctx->trace_counter.ensure(256);
+        ctx->noalloc_scratch = ScratchAddress(device);
     });
     if (st != DPL_OK) return st;
     *out = ctx.release();
@@ -958,6 +989,29 @@ static dpl_status TraceImpl(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray*
 dpl_status dpl_trace(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
                      dpl_hit* hits, dpl_mem mem) {
     return TraceImpl(ctx, bvh, rays, n, flags, hits, mem, nullptr);
+}
+
+dpl_status dpl_trace_noalloc(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
+                             dpl_hit* hits) {
+    if (!ctx || !bvh) return ctx ? Fail(ctx, DPL_ERR_INVALID_ARG, "null argument") : DPL_ERR_INVALID_ARG;
+    return Guard(ctx, [&] {
+        Require(n >= 0, "n must be non-negative");
+        Require(n == 0 || (rays && hits), "rays and hits must be non-null device pointers");
+        Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK)) == 0u,
+                "dpl_trace_noalloc takes DPL_TRACE_ANY and DPL_TRACE_CULL_BACK only");
+        Require(bvh->ctx == ctx, "bvh belongs to another context");
+        Require(!bvh->batched, "dpl_trace_noalloc traces single-scene BVHs");
+        Require(ctx->noalloc_scratch != nullptr, "no scratch address for this device");
+        if (n == 0) return;
+        const size_t spill_bytes = kNoAllocScratchBytes - kNoAllocCounterBytes;
+        const int grid = TraceGridFor(ctx->cu_count, n, bvh->tree_height, spill_bytes);
+        Require(grid >= 1, "the tree is too tall for the no-allocation scratch");
+        uint32_t* counter = reinterpret_cast<uint32_t*>(ctx->noalloc_scratch);
+        int32_t* spill = reinterpret_cast<int32_t*>(ctx->noalloc_scratch + kNoAllocCounterBytes);
+        TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
+                       int32_t(bvh->prim_count), bvh->tree_height, bvh->ndirs, bvh->sobb_dirs.as<float>()};
+        DPL_HIP(LaunchTrace(tree, rays, n, flags, hits, ctx->cu_count, counter, spill, nullptr, ctx->stream, grid));
+    });
 }
 
 dpl_status dpl_trace_counters(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,

@@ -341,8 +341,12 @@ int SobbTriples13();
 void SobbTriple13(int t, int* a, int* b, int* c, float* inv_det);
 
 // counters (nullable, device): accumulates {inner nodes visited, triangle tests}.
+// max_grid > 0 caps the persistent grid (the spill region then needs max_grid workgroups'
+// stacks); TraceGridFor gives the largest grid whose stacks fit spill_bytes.
 hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,
-                       int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters, hipStream_t s);
+                       int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters, hipStream_t s,
+                       int max_grid = 0);
+int TraceGridFor(int cu_count, int64_t n, int height, size_t spill_bytes);
 // Batched trace: ray i against mesh ray_mesh[i] of a batched BVH (per-mesh trees in mt; a
 // mesh id out of range gives a miss); max_height sizes the spill stacks.
 hipError_t LaunchTraceBatched(const FNode* nodes, const float4* tris, const BatchTree* mt, int32_t mesh_count,

@@ -298,13 +298,23 @@ size_t TraceSpillBytes(int cu_count, int height) {
     return size_t(cu_count) * kTraceWgPerCu * kTraceBlock * size_t(TraceSpillStride(height)) * 4;
 }
 
+int TraceGridFor(int cu_count, int64_t n, int height, size_t spill_bytes) {
+    const size_t per_wg = size_t(kTraceBlock) * size_t(TraceSpillStride(height)) * 4;
+    const int grid = TraceGrid(cu_count, n);
+    if (per_wg == 0) return grid;
+    return int(std::min<size_t>(size_t(grid), spill_bytes / per_wg));
+}
+
 hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,
-                       int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters, hipStream_t s) {
+                       int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters, hipStream_t s,
+                       int max_grid) {
     if (n <= 0) return hipSuccess;
     if (n > int64_t(0x7FFFFFFF)) return hipErrorInvalidValue;  // the 32-bit ray counter never wraps
-    hipLaunchKernelGGL(KResetCounter, dim3(1), dim3(1), 0, s, counter, counters);
     const float4* r4 = reinterpret_cast<const float4*>(rays);
-    const int grid = TraceGrid(cu_count, n);
+    int grid = TraceGrid(cu_count, n);
+    if (max_grid > 0 && grid > max_grid) grid = max_grid;
+    if (grid < 1) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(KResetCounter, dim3(1), dim3(1), 0, s, counter, counters);
     const int32_t stride = TraceSpillStride(tree.height);
 #define DPL_TRACE(COUNT_, SOBB_)                                                                                   \
     hipLaunchKernelGGL((KTrace<kTraceBlock, COUNT_, SOBB_, false>), dim3(grid), dim3(kTraceBlock), 0, s, tree, r4, n, \

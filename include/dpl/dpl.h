@@ -191,6 +191,19 @@ dpl_status dpl_trace(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, 
 dpl_status dpl_trace_counters(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
                               dpl_hit* hits, dpl_mem mem, int64_t* node_visits, int64_t* triangle_tests);
 
+/* dpl_trace without any allocation, for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). No
+ * counterpart in the reference. rays and hits are device pointers; a single-scene BVH,
+ * closest hit or DPL_TRACE_ANY, optional DPL_TRACE_CULL_BACK (no DPL_TRACE_WIDE). Its ray
+ * counter and traversal stacks live in a fixed 64 MiB scratch array per device (held by each
+ * GPU the library runs on until the process ends), so the persistent grid is capped to the
+ * workgroups whose spill stacks fit it (every tree fits: a 45-level tree still gets about 1700
+ * workgroups); results equal dpl_trace's bit for bit. Every call on one GPU shares that GPU's
+ * array: calls that overlap on one GPU (other streams, other contexts, other threads) race and
+ * must be ordered by the caller. Does not synchronise. */
+dpl_status dpl_trace_noalloc(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, int64_t n, uint32_t flags,
+                             dpl_hit* hits);
+
 /* Batched BVH: ray i is traced against mesh ray_mesh[i] (device or host as `mem`); hit prim
  * ids are local to that mesh; a mesh id out of range gives a miss. */
 dpl_status dpl_trace_batched(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray* rays, const int32_t* ray_mesh,
