@@ -20,7 +20,10 @@ constexpr int kStackLdsSobb = 8;
 constexpr bool kSobbSpillInterleaved = true;
 constexpr int kTraceBlock = 256;
 constexpr int kTraceWgPerCu = 8;
-constexpr int kTraceRefillDefault = 8;  // 2M-triangle random rays: 8 -> 26.3 ms, 16 -> 26.7, 32 -> 28.5, wave-at-a-time 28.1
+constexpr int kTraceRefillDefault = 8;
+// leaves a lane keeps pending while its wave is among inner nodes: 2M-triangle random rays
+// 26.9 ms with 1, 22.4 with 2, 21.5 with 4, 21.6 with 8
+constexpr int kTracePending = 4;  // 2M-triangle random rays: 8 -> 26.3 ms, 16 -> 26.7, 32 -> 28.5, wave-at-a-time 28.1
 }  // namespace
 
 // BATCH: ray i is traced against mesh ray_mesh[i], whose tree starts at the per-mesh offsets
@@ -111,11 +114,13 @@ __global__ __launch_bounds__(BLOCK) void KTrace(TraceTree tree, const float4* __
 // speculative traversal and replacement of terminated rays: Aila & Laine, HPG 2009). The
 // wave fetches rays for all its idle lanes with one atomic once at least `refill` of them are
 // idle. Every lane runs the node loop while any lane is still looking for its first leaf; a
-// lane that meets a leaf keeps it pending and goes on with the next stack entry, and the
-// pending leaves are tested together. Closest hits are the smallest t in (tmin, tmax), ties
-// to the lower primitive id, and culling by the best t so far is conservative, so the result
-// does not depend on the visiting order: it equals the ordered traversal's bit for bit.
-template <int BLOCK, bool SOBB, bool BATCH, bool ANY>
+// lane that meets a leaf keeps it pending (up to Q of them) and goes on with the next stack
+// entry, and then the pending leaves are tested one triangle per step, so that a lane with
+// several small leaves is not held to the slowest lane of each leaf. Closest hits are the
+// smallest t in (tmin, tmax), ties to the lower primitive id, and culling by the best t so far
+// is conservative, so the result does not depend on the visiting order: it equals the ordered
+// traversal's bit for bit.
+template <int BLOCK, bool SOBB, bool BATCH, bool ANY, int Q>
 __global__ __launch_bounds__(BLOCK) void KTraceRefill(TraceTree tree, const float4* __restrict__ rays, int64_t n,
                                                       bool cull, dpl_hit* __restrict__ hits, uint32_t* counter,
                                                       int32_t* spill, int32_t spill_stride,
@@ -142,7 +147,11 @@ __global__ __launch_bounds__(BLOCK) void KTraceRefill(TraceTree tree, const floa
         TraceTree t = tree;
         float best_t = 0.0f;
         HitResult r{0.0f, 0.0f, 0.0f, -1, -1};
-        int32_t ref = kEnd, pend = 0;  // pend < 0: a leaf reference waiting for its test
+        int32_t ref = kEnd;
+        int32_t pq[Q];  // leaves waiting for their test (pq[0] newest), npend of them
+        int npend = 0;
+#pragma unroll
+        for (int q = 0; q < Q; q++) pq[q] = 0;
         int64_t budget = 0;
         bool more = true;  // wave-uniform: the ray counter may still hand out rays
         for (;;) {
@@ -178,7 +187,7 @@ __global__ __launch_bounds__(BLOCK) void KTraceRefill(TraceTree tree, const floa
                         }
                         best_t = w.tmax;
                         r = HitResult{0.0f, 0.0f, 0.0f, -1, -1};
-                        pend = 0;
+                        npend = 0;
                         st.sp = 0;
                         budget = 2 * int64_t(t.node_count) + 2;
                         ray = i;
@@ -190,18 +199,21 @@ __global__ __launch_bounds__(BLOCK) void KTraceRefill(TraceTree tree, const floa
                 continue;
             }
             if (ray >= 0) {
-                // inner nodes, while any lane has no pending leaf yet
+                // inner nodes, while any lane has no pending leaf yet (a lane keeps up to Q)
                 for (;;) {
-                    if (ref < 0 && pend >= 0) {
-                        pend = ref;
+                    if (ref < 0 && npend < Q) {
+#pragma unroll
+                        for (int q = Q - 1; q > 0; q--) pq[q] = pq[q - 1];
+                        pq[0] = ref;
+                        npend++;
                         ref = st.sp > 0 ? st.Pop() : kEnd;
                     }
                     const bool node = ref >= 0 && ref < t.node_count;
-                    if (!__any(node && pend >= 0)) break;
+                    if (!__any(node && npend == 0)) break;
                     if (node) {
                         if (--budget < 0) {  // only a malformed tree gets here
                             ref = kEnd;
-                            pend = 0;
+                            npend = 0;
                             st.sp = 0;
                             continue;
                         }
@@ -233,15 +245,33 @@ __global__ __launch_bounds__(BLOCK) void KTraceRefill(TraceTree tree, const floa
                     }
                 }
                 // the pending leaves
-                if (pend < 0) {
-                    const bool done = TestLeaf<ANY>(t.tris, t.prim_count, pend, w, cull, best_t, r, none);
-                    pend = 0;
-                    if (ANY && done) {
-                        ref = kEnd;
-                        st.sp = 0;
+                bool done = false;
+                // one triangle per step over all of the lane's pending leaves, so that a lane
+                // with several small leaves is not held to the slowest lane per leaf
+                int32_t k = 0, kend = 0;
+                for (;;) {
+                    while (k == kend && npend > 0) {  // (an empty leaf takes the next one)
+                        const int32_t packed = ~pq[0];
+#pragma unroll
+                        for (int q = 0; q < Q - 1; q++) pq[q] = pq[q + 1];
+                        npend--;
+                        k = packed >> 4;
+                        kend = k + (packed & 15);
+                        if (kend > t.prim_count) kend = k;
+                    }
+                    const bool go = k < kend && !done;
+                    if (!__any(go)) break;
+                    if (go) {
+                        done = TestTriangle<ANY>(t.tris, k, w, cull, best_t, r);
+                        k++;
                     }
                 }
-                if (ref >= t.node_count && pend >= 0) {
+                npend = 0;
+                if (ANY && done) {
+                    ref = kEnd;
+                    st.sp = 0;
+                }
+                if (ref >= t.node_count) {
                     dpl_hit o;
                     o.t = r.t;
                     o.b1 = r.b1;
@@ -329,9 +359,10 @@ hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, ui
         else DPL_TRACE(false, false);
     } else {
         const bool any = flags & DPL_TRACE_ANY, cull = flags & DPL_TRACE_CULL_BACK;
-#define DPL_TRACE_R(SOBB_, ANY_)                                                                                    \
-    hipLaunchKernelGGL((KTraceRefill<kTraceBlock, SOBB_, false, ANY_>), dim3(grid), dim3(kTraceBlock), 0, s, tree, r4, \
-                       n, cull, hits, counter, spill, stride, nullptr, nullptr, 0, refill)
+#define DPL_TRACE_R(SOBB_, ANY_)                                                                                     \
+    hipLaunchKernelGGL((KTraceRefill<kTraceBlock, SOBB_, false, ANY_, kTracePending>), dim3(grid),               \
+                       dim3(kTraceBlock), 0, s, tree, r4, n, cull, hits, counter, spill, stride, nullptr, nullptr, 0,  \
+                       refill)
         if (sobb) {
             if (any) DPL_TRACE_R(true, true);
             else DPL_TRACE_R(true, false);
@@ -363,10 +394,10 @@ hipError_t LaunchTraceBatched(const FNode* nodes, const float4* tris, const Batc
     } else {
         const bool cull = flags & DPL_TRACE_CULL_BACK;
         if (flags & DPL_TRACE_ANY)
-            hipLaunchKernelGGL((KTraceRefill<kTraceBlock, false, true, true>), dim3(grid), dim3(kTraceBlock), 0, s,
+            hipLaunchKernelGGL((KTraceRefill<kTraceBlock, false, true, true, kTracePending>), dim3(grid), dim3(kTraceBlock), 0, s,
                                tree, r4, n, cull, hits, counter, spill, stride, ray_mesh, mt, mesh_count, refill);
         else
-            hipLaunchKernelGGL((KTraceRefill<kTraceBlock, false, true, false>), dim3(grid), dim3(kTraceBlock), 0, s,
+            hipLaunchKernelGGL((KTraceRefill<kTraceBlock, false, true, false, kTracePending>), dim3(grid), dim3(kTraceBlock), 0, s,
                                tree, r4, n, cull, hits, counter, spill, stride, ray_mesh, mt, mesh_count, refill);
     }
     return hipGetLastError();

@@ -220,6 +220,30 @@ def test_trace_conservative_for_in_plane_and_grazing_rays(oracle, ctx, bv):
     g.close()
 
 
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 9])
+def test_trace_tiny_scenes(oracle, ctx, n):
+    """Scenes whose whole tree is one leaf (the root's right slot an empty leaf) or a few
+    nodes: every trace form equals brute force."""
+    from diplodocus_amd import _lib
+
+    rng = np.random.default_rng(n)
+    tris = (rng.uniform(0, 1, (n, 1, 3)) + rng.normal(0, 0.3, (n, 3, 3))).reshape(n, 9).astype(np.float32)
+    rays = scenes.random_rays_towards(tris, 4096, seed=n, jitter=0.4)
+    say(f"tiny scene n={n}")
+    for leaf in (1, 4, 15):
+        g = gpu_bvh(ctx, tris, leaf=leaf)
+        for flags in (0, 1, 2):
+            brute = oracle.trace_brute(tris, rays, flags)
+            h = g.trace(rays, flags)
+            if flags & 1:
+                assert ((h["prim"] >= 0) == (brute["prim"] >= 0)).all(), (leaf, flags)
+            else:
+                assert hits_equal(h, brute).all(), (leaf, flags)
+        g.widen(4)
+        assert hits_equal(g.trace(rays, _lib.DPL_TRACE_WIDE), oracle.trace_brute(tris, rays)).all(), leaf
+        g.close()
+
+
 def test_trace_error_paths(ctx):
     from diplodocus_amd import accel
 
