@@ -347,6 +347,9 @@ hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, ui
                        int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters, hipStream_t s,
                        int max_grid = 0);
 int TraceGridFor(int cu_count, int64_t n, int height, size_t spill_bytes);
+// Idle lanes per wave before a persistent trace wave fetches new rays (DPL_TRACE_REFILL, 1-64;
+// 0 selects the kernels that fetch 64 rays per wave once all 64 are done).
+int TraceRefillLanes();
 // Batched trace: ray i against mesh ray_mesh[i] of a batched BVH (per-mesh trees in mt; a
 // mesh id out of range gives a miss); max_height sizes the spill stacks.
 hipError_t LaunchTraceBatched(const FNode* nodes, const float4* tris, const BatchTree* mt, int32_t mesh_count,

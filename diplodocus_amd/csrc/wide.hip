@@ -23,6 +23,7 @@ namespace {
 constexpr int kWideBlock = 256;
 constexpr int kWideStackLds = 16;
 constexpr int kWideWgPerCu = 8;
+constexpr int kWidePending = 4;  // leaves a lane keeps pending (as the two-child trace)
 
 __device__ __forceinline__ bool EmptyLeaf(int32_t r) { return r < 0 && ((~r) & 15) == 0; }
 __device__ __forceinline__ float ScaleOf(uint32_t b) { return __uint_as_float(b << 23); }
@@ -347,6 +348,53 @@ __device__ __forceinline__ void SortChildren(float* t, int* k, int32_t* r) {
 #undef DPL_CAS
 }
 
+// One wide node of an ordered traversal: slab tests of all W children without a branch (a
+// wave's lanes sit at nodes with different child counts), the hit children sorted by
+// (tnear, slot), all but the nearest pushed farthest first, ref set to the nearest. False when
+// no child is hit (the caller pops the next entry).
+template <int W, class Stack>
+__device__ __forceinline__ bool WideNodeStep(const WideTree& tree, const WtRay& w, float best_t, int32_t& ref,
+                                             Stack& st) {
+    constexpr int NB = WideNodeBytes(W) / 16;  // float4 per node
+    const float4* np = tree.nodes + int64_t(ref) * NB;
+    uint32_t wd[NB * 4];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+        const uint4 v = reinterpret_cast<const uint4*>(np)[j];
+        wd[4 * j] = v.x, wd[4 * j + 1] = v.y, wd[4 * j + 2] = v.z, wd[4 * j + 3] = v.w;
+    }
+    const float ox = __uint_as_float(wd[0]), oy = __uint_as_float(wd[1]), oz = __uint_as_float(wd[2]);
+    const uint32_t meta = wd[3];
+    const float sx = ScaleOf(meta & 255u), sy = ScaleOf((meta >> 8) & 255u), sz = ScaleOf((meta >> 16) & 255u);
+    const int c = int(meta >> 24);
+    float key[W];
+    int slot[W];
+    int32_t val[W];
+    int nh = 0;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        constexpr int Q = 4 + W;  // first plane word
+        const int sh = 8 * (k % 4), wj = k / 4;
+        auto q = [&](int pl) { return float((wd[Q + pl * (W / 4) + wj] >> sh) & 255u); };
+        const float x0 = fmaf(q(0), sx, ox), x1 = fmaf(q(1), sx, ox);
+        const float y0 = fmaf(q(2), sy, oy), y1 = fmaf(q(3), sy, oy);
+        const float z0 = fmaf(q(4), sz, oz), z1 = fmaf(q(5), sz, oz);
+        float tn;
+        const bool hit = Box(w, best_t, x0, x1, y0, y1, z0, z1, tn) & (k < c);
+        key[k] = hit ? tn : __builtin_huge_valf();
+        slot[k] = hit ? k : k + W;
+        val[k] = int32_t(wd[4 + k]);
+        nh += hit ? 1 : 0;
+    }
+    if (nh == 0) return false;
+    SortChildren<W>(key, slot, val);
+#pragma unroll
+    for (int j = W - 1; j >= 1; j--)
+        if (j < nh) st.Push(val[j]);
+    ref = val[0];
+    return true;
+}
+
 // Ordered traversal of a wide tree: the hit children sorted by (tnear, slot), all but the
 // nearest pushed farthest first; leaves are pushed like nodes. Every child's slab test runs
 // without a branch (a wave's lanes sit at nodes with different child counts). ANY stops at
@@ -354,7 +402,6 @@ __device__ __forceinline__ void SortChildren(float* t, int* k, int32_t* r) {
 template <bool ANY, int W, class Stack>
 __device__ __forceinline__ HitResult TraverseWide(const WideTree& tree, const WtRay& w, bool cull, Stack& st,
                                                   TraceCounters& cnt) {
-    constexpr int NB = WideNodeBytes(W) / 16;  // float4 per node
     HitResult r{0.0f, 0.0f, 0.0f, -1, -1};
     float best_t = w.tmax;
     st.sp = 0;
@@ -365,45 +412,9 @@ __device__ __forceinline__ HitResult TraverseWide(const WideTree& tree, const Wt
         if (--budget < 0 || ref >= tree.node_count) break;
         if (ref >= 0) {
             cnt.nodes++;
-            const float4* np = tree.nodes + int64_t(ref) * NB;
-            uint32_t wd[NB * 4];
-#pragma unroll
-            for (int j = 0; j < NB; j++) {
-                const uint4 v = reinterpret_cast<const uint4*>(np)[j];
-                wd[4 * j] = v.x, wd[4 * j + 1] = v.y, wd[4 * j + 2] = v.z, wd[4 * j + 3] = v.w;
-            }
-            const float ox = __uint_as_float(wd[0]), oy = __uint_as_float(wd[1]), oz = __uint_as_float(wd[2]);
-            const uint32_t meta = wd[3];
-            const float sx = ScaleOf(meta & 255u), sy = ScaleOf((meta >> 8) & 255u), sz = ScaleOf((meta >> 16) & 255u);
-            const int c = int(meta >> 24);
-            float key[W];
-            int slot[W];
-            int32_t val[W];
-            int nh = 0;
-#pragma unroll
-            for (int k = 0; k < W; k++) {
-                constexpr int Q = 4 + W;  // first plane word
-                const int sh = 8 * (k % 4), wj = k / 4;
-                auto q = [&](int pl) { return float((wd[Q + pl * (W / 4) + wj] >> sh) & 255u); };
-                const float x0 = fmaf(q(0), sx, ox), x1 = fmaf(q(1), sx, ox);
-                const float y0 = fmaf(q(2), sy, oy), y1 = fmaf(q(3), sy, oy);
-                const float z0 = fmaf(q(4), sz, oz), z1 = fmaf(q(5), sz, oz);
-                float tn;
-                const bool hit = Box(w, best_t, x0, x1, y0, y1, z0, z1, tn) & (k < c);
-                key[k] = hit ? tn : __builtin_huge_valf();
-                slot[k] = hit ? k : k + W;
-                val[k] = int32_t(wd[4 + k]);
-                nh += hit ? 1 : 0;
-            }
-            if (nh == 0) {
+            if (!WideNodeStep<W>(tree, w, best_t, ref, st)) {
                 if (st.sp == 0) break;
                 ref = st.Pop();
-            } else {
-                SortChildren<W>(key, slot, val);
-#pragma unroll
-                for (int j = W - 1; j >= 1; j--)
-                    if (j < nh) st.Push(val[j]);
-                ref = val[0];
             }
         } else {
             if (TestLeaf<ANY>(tree.tris, tree.prim_count, ref, w, cull, best_t, r, cnt) || st.sp == 0) break;
@@ -479,6 +490,136 @@ __global__ __launch_bounds__(kWideBlock) void KTraceWide(WideTree tree, const fl
     }
 }
 
+// The wide trace with per-lane ray replacement and up to Q pending leaves per lane, tested one
+// triangle per step (as the two-child refill kernel, trace.hip): same closest hits.
+template <int W, bool BATCH, bool ANY, int Q>
+__global__ __launch_bounds__(kWideBlock) void KTraceWideRefill(WideTree tree, const float4* __restrict__ rays,
+                                                               int64_t n, bool cull, dpl_hit* __restrict__ hits,
+                                                               uint32_t* counter, int32_t* spill,
+                                                               const int32_t* __restrict__ ray_mesh,
+                                                               const BatchTree* __restrict__ mt, int32_t mesh_count,
+                                                               int refill) {
+    constexpr int32_t kEnd = 0x7FFFFFFF;
+    __shared__ int32_t s_stack[kWideStackLds][kWideBlock];
+    __shared__ uint32_t s_base[kWideBlock / kWave];
+    const int lane = LaneId(), wid = threadIdx.x / kWave;
+    const uint64_t below = (1ull << lane) - 1ull;
+    LaneStackInterleaved<kWideStackLds, kWideBlock> st{&s_stack[0][threadIdx.x],
+                                                       spill + int64_t(blockIdx.x) * kWideBlock + threadIdx.x,
+                                                       int64_t(gridDim.x) * kWideBlock, 0};
+    int64_t ray = -1;
+    WtRay w{};
+    WideTree t = tree;
+    float best_t = 0.0f;
+    HitResult r{0.0f, 0.0f, 0.0f, -1, -1};
+    int32_t ref = kEnd;
+    int32_t pq[Q];
+    int npend = 0;
+#pragma unroll
+    for (int q = 0; q < Q; q++) pq[q] = 0;
+    int64_t budget = 0;
+    bool more = true;
+    for (;;) {
+        const uint64_t idle = __ballot(ray < 0);
+        const int nidle = __popcll(idle);
+        if (more && nidle >= refill) {
+            if (lane == 0) s_base[wid] = atomicAdd(counter, uint32_t(nidle));
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            const int64_t base = s_base[wid];
+            __builtin_amdgcn_wave_barrier();
+            if (base + nidle >= n) more = false;
+            if (ray < 0) {
+                const int64_t i = base + __popcll(idle & below);
+                if (i < n) {
+                    const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+                    w = PrepareRay(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w);
+                    t = tree;
+                    ref = t.node_count > 0 ? 0 : kEnd;
+                    if (BATCH) {
+                        const int32_t m = ray_mesh[i];
+                        if (m >= 0 && m < mesh_count) {
+                            const BatchTree bt = mt[m];
+                            t.nodes = tree.nodes + bt.node_off * (WideNodeBytes(W) / 16);
+                            t.tris = tree.tris + 3 * bt.tri_off;
+                            t.node_count = bt.node_count;
+                            t.prim_count = bt.prim_count;
+                            ref = t.node_count > 0 ? 0 : kEnd;
+                        } else {
+                            ref = kEnd;  // a mesh id out of range: a miss
+                        }
+                    }
+                    best_t = w.tmax;
+                    r = HitResult{0.0f, 0.0f, 0.0f, -1, -1};
+                    npend = 0;
+                    st.sp = 0;
+                    budget = 2 * int64_t(t.node_count) + 2;
+                    ray = i;
+                }
+            }
+        }
+        if (__ballot(ray >= 0) == 0ull) {
+            if (!more) break;
+            continue;
+        }
+        if (ray >= 0) {
+            for (;;) {
+                if (ref < 0 && npend < Q) {
+#pragma unroll
+                    for (int q = Q - 1; q > 0; q--) pq[q] = pq[q - 1];
+                    pq[0] = ref;
+                    npend++;
+                    ref = st.sp > 0 ? st.Pop() : kEnd;
+                }
+                const bool node = ref >= 0 && ref < t.node_count;
+                if (!__any(node && npend == 0)) break;
+                if (node) {
+                    if (--budget < 0) {  // only a malformed tree gets here
+                        ref = kEnd;
+                        npend = 0;
+                        st.sp = 0;
+                        continue;
+                    }
+                    if (!WideNodeStep<W>(t, w, best_t, ref, st)) ref = st.sp > 0 ? st.Pop() : kEnd;
+                }
+            }
+            bool done = false;
+            int32_t k = 0, kend = 0;
+            for (;;) {
+                while (k == kend && npend > 0) {  // (an empty leaf takes the next one)
+                    const int32_t packed = ~pq[0];
+#pragma unroll
+                    for (int q = 0; q < Q - 1; q++) pq[q] = pq[q + 1];
+                    npend--;
+                    k = packed >> 4;
+                    kend = k + (packed & 15);
+                    if (kend > t.prim_count) kend = k;
+                }
+                const bool go = k < kend && !done;
+                if (!__any(go)) break;
+                if (go) {
+                    done = TestTriangle<ANY>(t.tris, k, w, cull, best_t, r);
+                    k++;
+                }
+            }
+            npend = 0;
+            if (ANY && done) {
+                ref = kEnd;
+                st.sp = 0;
+            }
+            if (ref >= t.node_count) {
+                dpl_hit o;
+                o.t = r.t;
+                o.b1 = r.b1;
+                o.b2 = r.b2;
+                o.prim = r.prim;
+                hits[ray] = o;
+                ray = -1;
+            }
+        }
+    }
+}
+
 __global__ void KResetWideCounter(uint32_t* c, unsigned long long* counters) {
     c[0] = 0u;
     if (counters) counters[0] = counters[1] = 0ull;
@@ -509,16 +650,30 @@ hipError_t LaunchTraceWide(const WideTree& tree, const dpl_ray* rays, int64_t n,
     hipLaunchKernelGGL(KResetWideCounter, dim3(1), dim3(1), 0, s, counter, counters);
     const float4* r4 = reinterpret_cast<const float4*>(rays);
     const int grid = WideGrid(cu_count, n);
+    const int refill = TraceRefillLanes();
 #define DPL_TRACE_W(W_, COUNT_)                                                                                 \
     hipLaunchKernelGGL((KTraceWide<W_, COUNT_, false>), dim3(grid), dim3(kWideBlock), 0, s, tree, r4, n, flags,     \
                        hits, counter, spill, counters, nullptr, nullptr, 0)
-    if (tree.width == 4) {
-        if (counters) DPL_TRACE_W(4, true);
-        else DPL_TRACE_W(4, false);
+#define DPL_TRACE_WR(W_, ANY_)                                                                                  \
+    hipLaunchKernelGGL((KTraceWideRefill<W_, false, ANY_, kWidePending>), dim3(grid), dim3(kWideBlock), 0, s, tree, \
+                       r4, n, (flags & DPL_TRACE_CULL_BACK) != 0, hits, counter, spill, nullptr, nullptr, 0, refill)
+    const bool any = flags & DPL_TRACE_ANY;
+    if (counters || refill == 0) {  // the counters count the ordered traversal
+        if (tree.width == 4) {
+            if (counters) DPL_TRACE_W(4, true);
+            else DPL_TRACE_W(4, false);
+        } else {
+            if (counters) DPL_TRACE_W(8, true);
+            else DPL_TRACE_W(8, false);
+        }
+    } else if (tree.width == 4) {
+        if (any) DPL_TRACE_WR(4, true);
+        else DPL_TRACE_WR(4, false);
     } else {
-        if (counters) DPL_TRACE_W(8, true);
-        else DPL_TRACE_W(8, false);
+        if (any) DPL_TRACE_WR(8, true);
+        else DPL_TRACE_WR(8, false);
     }
+#undef DPL_TRACE_WR
 #undef DPL_TRACE_W
     return hipGetLastError();
 }
@@ -532,12 +687,28 @@ hipError_t LaunchTraceWideBatched(const WideTree& tree, const BatchTree* mt, int
     hipLaunchKernelGGL(KResetWideCounter, dim3(1), dim3(1), 0, s, counter, nullptr);
     const float4* r4 = reinterpret_cast<const float4*>(rays);
     const int grid = WideGrid(cu_count, n);
-    if (tree.width == 4)
-        hipLaunchKernelGGL((KTraceWide<4, false, true>), dim3(grid), dim3(kWideBlock), 0, s, tree, r4, n, flags, hits,
-                           counter, spill, nullptr, ray_mesh, mt, mesh_count);
-    else
-        hipLaunchKernelGGL((KTraceWide<8, false, true>), dim3(grid), dim3(kWideBlock), 0, s, tree, r4, n, flags, hits,
-                           counter, spill, nullptr, ray_mesh, mt, mesh_count);
+    const int refill = TraceRefillLanes();
+    const bool any = flags & DPL_TRACE_ANY, cull = flags & DPL_TRACE_CULL_BACK;
+    if (refill == 0) {
+        if (tree.width == 4)
+            hipLaunchKernelGGL((KTraceWide<4, false, true>), dim3(grid), dim3(kWideBlock), 0, s, tree, r4, n, flags,
+                               hits, counter, spill, nullptr, ray_mesh, mt, mesh_count);
+        else
+            hipLaunchKernelGGL((KTraceWide<8, false, true>), dim3(grid), dim3(kWideBlock), 0, s, tree, r4, n, flags,
+                               hits, counter, spill, nullptr, ray_mesh, mt, mesh_count);
+        return hipGetLastError();
+    }
+#define DPL_TRACE_WB(W_, ANY_)                                                                                  \
+    hipLaunchKernelGGL((KTraceWideRefill<W_, true, ANY_, kWidePending>), dim3(grid), dim3(kWideBlock), 0, s, tree,  \
+                       r4, n, cull, hits, counter, spill, ray_mesh, mt, mesh_count, refill)
+    if (tree.width == 4) {
+        if (any) DPL_TRACE_WB(4, true);
+        else DPL_TRACE_WB(4, false);
+    } else {
+        if (any) DPL_TRACE_WB(8, true);
+        else DPL_TRACE_WB(8, false);
+    }
+#undef DPL_TRACE_WB
     return hipGetLastError();
 }
 
