@@ -1,7 +1,8 @@
 // Ray-batch trace kernel for gfx950: persistent grid, rays fetched 64 at a time per wave
-// from a global atomic counter, an LDS stack ring per lane (16 entries for AABB trees, 8 for
-// SOBB trees) with a global spill region for deeper stacks (see traverse.h), AABB or SOBB
-// child boxes, optional traversal counters.
+// from a global atomic counter, an LDS stack ring per lane (16 entries; 8 for SOBB trees in
+// the wave-at-a-time kernel) with a global spill region for deeper stacks (see traverse.h),
+// AABB or SOBB child boxes, optional traversal counters. The default kernel replaces a
+// lane's finished ray at once and keeps up to four leaves pending per lane.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
@@ -13,10 +14,15 @@ namespace dpl {
 
 namespace {
 constexpr int kStackLds = 16;
-// SOBB kernels keep 1/(n.d) of all 13 directions per lane in LDS (52 B, n.o is recomputed);
-// a shorter LDS stack ring keeps more workgroups resident per CU, which pays more than the
-// extra spills.
+// SOBB kernels keep 1/(n.d) of all 13 directions per lane in LDS (52 B, n.o is recomputed).
+// For the wave-at-a-time kernel a shorter LDS stack ring keeps more workgroups resident per
+// CU, which pays more than the extra spills. The per-lane refill kernel keeps 16 entries (2M
+// mixed scene, 4.2M random rays, SOBB tree: 4 entries 25.5 ms, 8 21.6 ms, 16 16.0 ms, 32
+// 19.0 ms; 8 entries spill ~2.6 KB per ray through L2 to the fabric). Keeping n.o per
+// direction in LDS as well gives the same time with 8 entries (fewer VALU instructions, fewer
+// resident waves) and 17.9 ms with 16. AABB trees: 8 entries 30.1 ms, 16 20.5, 32 20.2.
 constexpr int kStackLdsSobb = 8;
+constexpr int kStackLdsSobbRefill = 16;
 constexpr bool kSobbSpillInterleaved = true;
 constexpr int kTraceBlock = 256;
 constexpr int kTraceWgPerCu = 8;
@@ -127,7 +133,7 @@ __global__ __launch_bounds__(BLOCK) void KTraceRefill(TraceTree tree, const floa
                                                       const int32_t* __restrict__ ray_mesh,
                                                       const BatchTree* __restrict__ mt, int32_t mesh_count,
                                                       int refill) {
-    constexpr int S = SOBB ? kStackLdsSobb : kStackLds;
+    constexpr int S = SOBB ? kStackLdsSobbRefill : kStackLds;
     constexpr int32_t kEnd = 0x7FFFFFFF;  // no next node: the ray is done once no leaf is pending
     __shared__ int32_t s_stack[S][BLOCK];
     __shared__ uint32_t s_base[BLOCK / kWave];
@@ -140,7 +146,6 @@ __global__ __launch_bounds__(BLOCK) void KTraceRefill(TraceTree tree, const floa
     const int lane = LaneId(), wid = threadIdx.x / kWave;
     const uint64_t below = (1ull << lane) - 1ull;
     SobbRay sr{s_dirs, &s_idn[0][threadIdx.x], 0.0f, 0.0f, 0.0f, 0.0f};
-    TraceCounters none{0u, 0u};
     auto run = [&](auto& st) {
         int64_t ray = -1;  // this lane's ray, -1 while idle
         WtRay w{};
