@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libdpl.so")
 
 DPL_OK, DPL_ERR_INVALID_ARG, DPL_ERR_OOM, DPL_ERR_DEVICE, DPL_ERR_UNSUPPORTED, DPL_ERR_INTERNAL = range(6)
 DPL_MEM_HOST, DPL_MEM_DEVICE = 0, 1
-DPL_TRACE_ANY, DPL_TRACE_CULL_BACK, DPL_TRACE_WIDE = 1, 2, 4
+DPL_TRACE_ANY, DPL_TRACE_CULL_BACK, DPL_TRACE_WIDE, DPL_TRACE_COHERENT = 1, 2, 4, 8
 
 STATUS_NAMES = {0: "OK", 1: "INVALID_ARG", 2: "OOM", 3: "DEVICE", 4: "UNSUPPORTED", 5: "INTERNAL"}
 

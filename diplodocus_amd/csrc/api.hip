@@ -15,6 +15,7 @@
 #include <stdexcept>
 #include <string>
 #include <unordered_map>
+#include <unordered_set>
 #include <vector>
 
 #include "dpl_internal.h"
@@ -186,7 +187,7 @@ struct dpl_context {
     // scratch
     DevBuf tri, keys, vals, keys_alt, vals_alt, nodes, parent, clusters0, clusters1, ccount, status0, status1,
         small, eval, state, kept_id, leafslot, keptflag, scan_tmp, sort_tmp, rays, hits,
-        trace_counter, spill, kdop, sobb_set, wide_tmp;
+        trace_counter, spill, kdop, sobb_set, wide_tmp, ray_order, ray_order_tmp;
     // batched builds and traces
     DevBuf b_off, b_foff, b_order, b_results, b_queue, ray_mesh;
     std::string device_name;
@@ -203,6 +204,10 @@ struct dpl_context {
     // generation it was captured with.
     uint64_t gen_seq = 0;
     std::unordered_map<const dpl_bvh*, uint64_t> bvh_gen;
+    // Live trace graphs. Destroying the context detaches them and the live BVHs (their ctx
+    // becomes null), so that handles destroyed after their context free only their own
+    // device memory.
+    std::unordered_set<dpl_trace_graph*> graphs;
     hipEvent_t IterEvent(size_t k) {
         while (iter_ev.size() <= k) {
             hipEvent_t e;
@@ -240,6 +245,12 @@ struct dpl_bvh {
     int32_t wide_width = 0;
     double widen_ms = 0.0;
     float root_lo[3] = {}, root_hi[3] = {};  // scene box (single-scene BVHs)
+    dpl_bvh() = default;
+    dpl_bvh(const dpl_bvh&) = delete;
+    dpl_bvh& operator=(const dpl_bvh&) = delete;
+    ~dpl_bvh() {
+        if (ctx) ctx->bvh_gen.erase(this);  // also when a failed build drops the new handle
+    }
 };
 
 namespace {
@@ -610,10 +621,15 @@ dpl_status dpl_context_create(int32_t device, dpl_context** out) {
     return DPL_OK;
 }
 
+static void DetachGraphs(dpl_context* ctx);
+
 void dpl_context_destroy(dpl_context* ctx) {
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->bvh_gen) const_cast<dpl_bvh*>(kv.first)->ctx = nullptr;
+    ctx->bvh_gen.clear();
+    DetachGraphs(ctx);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->iter_ev) (void)hipEventDestroy(e);
@@ -655,7 +671,7 @@ dpl_status dpl_bvh_build(dpl_context* ctx, const float* tri, int64_t n, dpl_mem 
 
 dpl_status dpl_bvh_rebuild(dpl_bvh* bvh, const float* tri, int64_t n, dpl_mem mem, const dpl_build_params* params,
                            dpl_build_stats* stats) {
-    if (!bvh || !params) return DPL_ERR_INVALID_ARG;
+    if (!bvh || !bvh->ctx || !params) return DPL_ERR_INVALID_ARG;
     return Guard(bvh->ctx, [&] {
         dpl_build_params p = Normalize(params);
         BuildImpl(bvh->ctx, bvh, tri, n, mem, p, stats);
@@ -855,7 +871,7 @@ dpl_status dpl_bvh_get_stats(const dpl_bvh* bvh, dpl_build_stats* stats) {
 }
 
 dpl_status dpl_bvh_sah_aabb(const dpl_bvh* bvh, double* sah) {
-    if (!bvh || !sah) return DPL_ERR_INVALID_ARG;
+    if (!bvh || !bvh->ctx || !sah) return DPL_ERR_INVALID_ARG;
     dpl_context* ctx = bvh->ctx;
     return Guard(ctx, [&] {
         // AABB trees (and batched BVHs, which are AABB only): the build's own SAH; a
@@ -904,7 +920,7 @@ dpl_status dpl_bvh_get_layout(const dpl_bvh* bvh, dpl_bvh_layout* l) {
 
 dpl_status dpl_bvh_download(const dpl_bvh* bvh, void* nodes, void* tri_ordered, int32_t* prim_order, uint64_t* codes,
                             uint32_t* sorted_idx, void* ploc_nodes, int32_t* iter_counts) {
-    if (!bvh) return DPL_ERR_INVALID_ARG;
+    if (!bvh || !bvh->ctx) return DPL_ERR_INVALID_ARG;
     dpl_context* ctx = bvh->ctx;
     return Guard(ctx, [&] {
         hipStream_t s = ctx->stream;
@@ -939,7 +955,8 @@ static dpl_status TraceImpl(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray*
     return Guard(ctx, [&] {
         Require(n >= 0, "n must be non-negative");
         Require(n == 0 || (rays && hits), "rays and hits must be non-null");
-        Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK | DPL_TRACE_WIDE)) == 0u, "unknown trace flags");
+        Require((flags & ~(DPL_TRACE_ANY | DPL_TRACE_CULL_BACK | DPL_TRACE_WIDE | DPL_TRACE_COHERENT)) == 0u,
+                "unknown trace flags");
         Require(mem == DPL_MEM_HOST || mem == DPL_MEM_DEVICE, "mem must be DPL_MEM_HOST or DPL_MEM_DEVICE");
         Require(bvh->ctx == ctx, "bvh belongs to another context");
         Require(!bvh->batched, "a batched BVH is traced with dpl_trace_batched");
@@ -958,6 +975,16 @@ static dpl_status TraceImpl(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray*
         uint8_t* tc = ctx->trace_counter.as<uint8_t>();
         unsigned long long* counters = nullptr;
         if (counts_out) counters = reinterpret_cast<unsigned long long*>(tc + 64);  // zeroed by the launch
+        // a coherent tracing order on request (the per-ray results do not depend on it; the
+        // counter kernels trace in the given order)
+        const int32_t* order = nullptr;
+        if ((flags & DPL_TRACE_COHERENT) && !counts_out && TraceRefillLanes() > 0) {
+            uint32_t* buf = ctx->ray_order.ensure<uint32_t>(size_t(n) * 16);
+            const size_t tb = RayOrderTempBytes(n);
+            void* tmp = ctx->ray_order_tmp.ensure<uint8_t>(tb);
+            DPL_HIP(LaunchRayOrder(dr, n, bvh->root_lo, bvh->root_hi, buf, tmp, tb, &order, s));
+        }
+        flags &= ~DPL_TRACE_COHERENT;
         hipError_t e;
         if (wide) {
             const WideTree wt{bvh->wnodes.as<float4>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
@@ -965,13 +992,13 @@ static dpl_status TraceImpl(dpl_context* ctx, const dpl_bvh* bvh, const dpl_ray*
             int32_t* spill = ctx->spill.ensure<int32_t>(
                 std::max<size_t>(16, TraceWideSpillBytes(ctx->cu_count, bvh->wide_width, bvh->tree_height)));
             e = LaunchTraceWide(wt, dr, n, flags & ~DPL_TRACE_WIDE, dh, ctx->cu_count, reinterpret_cast<uint32_t*>(tc),
-                                spill, counters, s);
+                                spill, counters, s, order);
         } else {
             TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
                            int32_t(bvh->prim_count), bvh->tree_height, bvh->ndirs, bvh->sobb_dirs.as<float>()};
             int32_t* spill = ctx->spill.ensure<int32_t>(TraceSpillBytes(ctx->cu_count, bvh->tree_height));
             e = LaunchTrace(tree, dr, n, flags, dh, ctx->cu_count, reinterpret_cast<uint32_t*>(tc), spill, counters,
-                            s);
+                            s, 0, order);
         }
         if (counts_out) {
             DPL_HIP(hipMemcpyAsync(counts_out, counters, 16, hipMemcpyDeviceToHost, s));
@@ -1135,12 +1162,13 @@ dpl_status dpl_trace_graph_create(dpl_context* ctx, const dpl_bvh* bvh, const dp
         dpl_trace_graph_destroy(g.release());
         return st;
     }
+    ctx->graphs.insert(g.get());
     *out = g.release();
     return DPL_OK;
 }
 
 dpl_status dpl_trace_graph_launch(dpl_trace_graph* g) {
-    if (!g || !g->exec) return DPL_ERR_INVALID_ARG;
+    if (!g || !g->exec || !g->ctx) return DPL_ERR_INVALID_ARG;
     return Guard(g->ctx, [&] {
         // the graph holds the BVH's device addresses, node counts and a spill buffer sized
         // for its height at capture: replay only the tree it was captured from
@@ -1151,11 +1179,17 @@ dpl_status dpl_trace_graph_launch(dpl_trace_graph* g) {
     });
 }
 
+static void DetachGraphs(dpl_context* ctx) {
+    for (dpl_trace_graph* g : ctx->graphs) g->ctx = nullptr;
+    ctx->graphs.clear();
+}
+
 void dpl_trace_graph_destroy(dpl_trace_graph* g) {
     if (!g) return;
     if (g->ctx) {
         (void)hipSetDevice(g->ctx->device);
         (void)hipStreamSynchronize(g->ctx->stream);
+        g->ctx->graphs.erase(g);
     }
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
@@ -1174,7 +1208,7 @@ static void WideInfo(const dpl_bvh* bvh, dpl_wide_info* info) {
 }
 
 dpl_status dpl_bvh_widen(dpl_bvh* bvh, int32_t width, dpl_wide_info* info) {
-    if (!bvh) return DPL_ERR_INVALID_ARG;
+    if (!bvh || !bvh->ctx) return DPL_ERR_INVALID_ARG;
     dpl_context* ctx = bvh->ctx;
     return Guard(ctx, [&] {
         Require(width == 4 || width == 8, "width must be 4 or 8");
@@ -1219,7 +1253,7 @@ dpl_status dpl_bvh_get_wide(const dpl_bvh* bvh, dpl_wide_info* info) {
 }
 
 dpl_status dpl_bvh_download_wide(const dpl_bvh* bvh, void* nodes) {
-    if (!bvh || !nodes) return DPL_ERR_INVALID_ARG;
+    if (!bvh || !bvh->ctx || !nodes) return DPL_ERR_INVALID_ARG;
     dpl_context* ctx = bvh->ctx;
     return Guard(ctx, [&] {
         Require(bvh->wide_width != 0, "the BVH has no wide tree (dpl_bvh_widen)");

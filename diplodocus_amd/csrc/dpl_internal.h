@@ -343,9 +343,18 @@ void SobbTriple13(int t, int* a, int* b, int* c, float* inv_det);
 // counters (nullable, device): accumulates {inner nodes visited, triangle tests}.
 // max_grid > 0 caps the persistent grid (the spill region then needs max_grid workgroups'
 // stacks); TraceGridFor gives the largest grid whose stacks fit spill_bytes.
+// order (optional, n entries): the refill kernels take ray order[q] as the q-th ray to trace
+// (hits still land at the ray's own index); the counter kernels ignore it.
 hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,
                        int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters, hipStream_t s,
-                       int max_grid = 0);
+                       int max_grid = 0, const int32_t* order = nullptr);
+// A coherent tracing order for n rays: Morton codes (10 bits per axis over the box lo..hi) of
+// the point a quarter of the way along each ray's span inside the box, sorted with the ray
+// indices (rays missing the box last). buf holds 4 n words; temp RayOrderTempBytes(n). *order
+// is set to the sorted indices inside buf.
+size_t RayOrderTempBytes(int64_t n);
+hipError_t LaunchRayOrder(const dpl_ray* rays, int64_t n, const float lo[3], const float hi[3], uint32_t* buf,
+                          void* temp, size_t temp_bytes, const int32_t** order, hipStream_t s);
 int TraceGridFor(int cu_count, int64_t n, int height, size_t spill_bytes);
 // Idle lanes per wave before a persistent trace wave fetches new rays (DPL_TRACE_REFILL, 1-64;
 // 0 selects the kernels that fetch 64 rays per wave once all 64 are done).
@@ -382,7 +391,7 @@ hipError_t LaunchWiden(const FNode* aabb_nodes, int32_t node_count, int width, v
                        int32_t mesh_count, hipStream_t s);
 hipError_t LaunchTraceWide(const WideTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,
                            int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters,
-                           hipStream_t s);
+                           hipStream_t s, const int32_t* order = nullptr);
 size_t TraceWideSpillBytes(int cu_count, int width, int height);
 // Batched BVH: ray i against mesh ray_mesh[i]; tree.nodes / tree.tris are the bases, tree.height
 // the largest mesh height.

@@ -132,7 +132,7 @@ __global__ __launch_bounds__(BLOCK) void KTraceRefill(TraceTree tree, const floa
                                                       int32_t* spill, int32_t spill_stride,
                                                       const int32_t* __restrict__ ray_mesh,
                                                       const BatchTree* __restrict__ mt, int32_t mesh_count,
-                                                      int refill) {
+                                                      int refill, const int32_t* __restrict__ order) {
     constexpr int S = SOBB ? kStackLdsSobbRefill : kStackLds;
     constexpr int32_t kEnd = 0x7FFFFFFF;  // no next node: the ray is done once no leaf is pending
     __shared__ int32_t s_stack[S][BLOCK];
@@ -153,7 +153,7 @@ __global__ __launch_bounds__(BLOCK) void KTraceRefill(TraceTree tree, const floa
         float best_t = 0.0f;
         HitResult r{0.0f, 0.0f, 0.0f, -1, -1};
         int32_t ref = kEnd;
-        int32_t pq[Q];  // leaves waiting for their test (pq[0] newest), npend of them
+        int32_t pq[Q];  // leaves waiting for their test (pq[0] newest, pq[npend - 1] oldest)
         int npend = 0;
 #pragma unroll
         for (int q = 0; q < Q; q++) pq[q] = 0;
@@ -170,8 +170,9 @@ __global__ __launch_bounds__(BLOCK) void KTraceRefill(TraceTree tree, const floa
                 __builtin_amdgcn_wave_barrier();
                 if (base + nidle >= n) more = false;
                 if (ray < 0) {
-                    const int64_t i = base + __popcll(idle & below);
-                    if (i < n) {
+                    const int64_t q = base + __popcll(idle & below);
+                    if (q < n) {
+                        const int64_t i = order ? int64_t(order[q]) : q;  // the ray's index in rays / hits
                         const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
                         w = PrepareRay(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w);
                         if (SOBB) PrepareSobb<BLOCK>(sr, s_dirs, tree.ndirs, r0.x, r0.y, r0.z, r1.x, r1.y, r1.z);
@@ -256,9 +257,13 @@ __global__ __launch_bounds__(BLOCK) void KTraceRefill(TraceTree tree, const floa
                 int32_t k = 0, kend = 0;
                 for (;;) {
                     while (k == kend && npend > 0) {  // (an empty leaf takes the next one)
-                        const int32_t packed = ~pq[0];
+                        // the oldest pending leaf first: leaves are tested in the order the traversal
+                        // reached them, whatever the other lanes held the wave to (an any-hit query then
+                        // reports the first hit of its own ordered traversal)
+                        int32_t oldest = pq[0];
 #pragma unroll
-                        for (int q = 0; q < Q - 1; q++) pq[q] = pq[q + 1];
+                        for (int q = 1; q < Q; q++) oldest = npend - 1 == q ? pq[q] : oldest;
+                        const int32_t packed = ~oldest;
                         npend--;
                         k = packed >> 4;
                         kend = k + (packed & 15);
@@ -340,9 +345,71 @@ int TraceGridFor(int cu_count, int64_t n, int height, size_t spill_bytes) {
     return int(std::min<size_t>(size_t(grid), spill_bytes / per_wg));
 }
 
+// ---- coherent ray order ----
+// Rays whose span inside the scene box passes the same region traverse the same nodes: tracing
+// them side by side keeps a wave's lanes on common paths (fewer idle lanes per VALU
+// instruction) and the nodes they read in L2. The point a quarter of the way into the span
+// stands for the region (rays aimed through the scene meet surfaces early in it).
+__device__ __forceinline__ uint32_t Spread10(uint32_t v) {
+    v &= 1023u;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+// lo / hi: the scene box; q: 1023 / extent per axis (0 for a flat axis)
+__global__ void KRayKeys(const float4* __restrict__ rays, int64_t n, float3 lo, float3 hi, float3 q,
+                         uint32_t* __restrict__ keys, uint32_t* __restrict__ ids) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
+    const float ix = 1.0f / r1.x, iy = 1.0f / r1.y, iz = 1.0f / r1.z;
+    const float ax = (lo.x - r0.x) * ix, bx = (hi.x - r0.x) * ix;
+    const float ay = (lo.y - r0.y) * iy, by = (hi.y - r0.y) * iy;
+    const float az = (lo.z - r0.z) * iz, bz = (hi.z - r0.z) * iz;
+    const float tn = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fmaxf(fminf(az, bz), r0.w));
+    const float tf = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fminf(fmaxf(az, bz), r1.w));
+    uint32_t key = 0xFFFFFFFFu;  // misses the box (or a NaN on both sides): last
+    if (tn <= tf) {
+        const float t = tn + 0.25f * (tf - tn);
+        auto cell = [](float v) { return uint32_t(fminf(fmaxf(v, 0.0f), 1023.0f)); };  // NaN -> 0
+        const uint32_t cx = cell((r0.x + r1.x * t - lo.x) * q.x);
+        const uint32_t cy = cell((r0.y + r1.y * t - lo.y) * q.y);
+        const uint32_t cz = cell((r0.z + r1.z * t - lo.z) * q.z);
+        key = Spread10(cx) | (Spread10(cy) << 1) | (Spread10(cz) << 2);
+    }
+    keys[i] = key;
+    ids[i] = uint32_t(i);
+}
+
+size_t RayOrderTempBytes(int64_t n) { return RadixSortTempBytes(n, 32, 8, false); }
+
+
+hipError_t LaunchRayOrder(const dpl_ray* rays, int64_t n, const float lo[3], const float hi[3], uint32_t* buf,
+                          void* temp, size_t temp_bytes, const int32_t** order, hipStream_t s) {
+    *order = nullptr;
+    if (n <= 0) return hipSuccess;
+    if (n > int64_t(0x7FFFFFFF)) return hipErrorInvalidValue;
+    uint32_t *keys = buf, *ids = buf + n, *keys_alt = buf + 2 * n, *ids_alt = buf + 3 * n;
+    float q[3];
+    for (int a = 0; a < 3; a++) q[a] = hi[a] > lo[a] ? 1023.0f / (hi[a] - lo[a]) : 0.0f;
+    const int grid = int((n + 255) / 256);
+    hipLaunchKernelGGL(KRayKeys, dim3(grid), dim3(256), 0, s, reinterpret_cast<const float4*>(rays), n,
+                       make_float3(lo[0], lo[1], lo[2]), make_float3(hi[0], hi[1], hi[2]), make_float3(q[0], q[1], q[2]),
+                       keys, ids);
+    bool alt = false;
+    const hipError_t e = LaunchRadixSort(reinterpret_cast<uint64_t*>(keys), ids, reinterpret_cast<uint64_t*>(keys_alt),
+                                         ids_alt, n, 32, 8, false, temp, temp_bytes, &alt, s);
+    if (e != hipSuccess) return e;
+    *order = reinterpret_cast<const int32_t*>(alt ? ids_alt : ids);
+    return hipGetLastError();
+}
+
 hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,
                        int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters, hipStream_t s,
-                       int max_grid) {
+                       int max_grid, const int32_t* order) {
     if (n <= 0) return hipSuccess;
     if (n > int64_t(0x7FFFFFFF)) return hipErrorInvalidValue;  // the 32-bit ray counter never wraps
     const float4* r4 = reinterpret_cast<const float4*>(rays);
@@ -367,7 +434,7 @@ hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, ui
 #define DPL_TRACE_R(SOBB_, ANY_)                                                                                     \
     hipLaunchKernelGGL((KTraceRefill<kTraceBlock, SOBB_, false, ANY_, kTracePending>), dim3(grid),               \
                        dim3(kTraceBlock), 0, s, tree, r4, n, cull, hits, counter, spill, stride, nullptr, nullptr, 0,  \
-                       refill)
+                       refill, order)
         if (sobb) {
             if (any) DPL_TRACE_R(true, true);
             else DPL_TRACE_R(true, false);
@@ -400,10 +467,12 @@ hipError_t LaunchTraceBatched(const FNode* nodes, const float4* tris, const Batc
         const bool cull = flags & DPL_TRACE_CULL_BACK;
         if (flags & DPL_TRACE_ANY)
             hipLaunchKernelGGL((KTraceRefill<kTraceBlock, false, true, true, kTracePending>), dim3(grid), dim3(kTraceBlock), 0, s,
-                               tree, r4, n, cull, hits, counter, spill, stride, ray_mesh, mt, mesh_count, refill);
+                               tree, r4, n, cull, hits, counter, spill, stride, ray_mesh, mt, mesh_count, refill,
+                               nullptr);
         else
             hipLaunchKernelGGL((KTraceRefill<kTraceBlock, false, true, false, kTracePending>), dim3(grid), dim3(kTraceBlock), 0, s,
-                               tree, r4, n, cull, hits, counter, spill, stride, ray_mesh, mt, mesh_count, refill);
+                               tree, r4, n, cull, hits, counter, spill, stride, ray_mesh, mt, mesh_count, refill,
+                               nullptr);
     }
     return hipGetLastError();
 }

@@ -498,7 +498,7 @@ __global__ __launch_bounds__(kWideBlock) void KTraceWideRefill(WideTree tree, co
                                                                uint32_t* counter, int32_t* spill,
                                                                const int32_t* __restrict__ ray_mesh,
                                                                const BatchTree* __restrict__ mt, int32_t mesh_count,
-                                                               int refill) {
+                                                               int refill, const int32_t* __restrict__ order) {
     constexpr int32_t kEnd = 0x7FFFFFFF;
     __shared__ int32_t s_stack[kWideStackLds][kWideBlock];
     __shared__ uint32_t s_base[kWideBlock / kWave];
@@ -530,8 +530,9 @@ __global__ __launch_bounds__(kWideBlock) void KTraceWideRefill(WideTree tree, co
             __builtin_amdgcn_wave_barrier();
             if (base + nidle >= n) more = false;
             if (ray < 0) {
-                const int64_t i = base + __popcll(idle & below);
-                if (i < n) {
+                const int64_t q = base + __popcll(idle & below);
+                if (q < n) {
+                    const int64_t i = order ? int64_t(order[q]) : q;  // the ray's index in rays / hits
                     const float4 r0 = rays[2 * i], r1 = rays[2 * i + 1];
                     w = PrepareRay(r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w);
                     t = tree;
@@ -587,9 +588,13 @@ __global__ __launch_bounds__(kWideBlock) void KTraceWideRefill(WideTree tree, co
             int32_t k = 0, kend = 0;
             for (;;) {
                 while (k == kend && npend > 0) {  // (an empty leaf takes the next one)
-                    const int32_t packed = ~pq[0];
+                    // the oldest pending leaf first: leaves are tested in the order the traversal
+                    // reached them, whatever the other lanes held the wave to (an any-hit query then
+                    // reports the first hit of its own ordered traversal)
+                    int32_t oldest = pq[0];
 #pragma unroll
-                    for (int q = 0; q < Q - 1; q++) pq[q] = pq[q + 1];
+                    for (int q = 1; q < Q; q++) oldest = npend - 1 == q ? pq[q] : oldest;
+                    const int32_t packed = ~oldest;
                     npend--;
                     k = packed >> 4;
                     kend = k + (packed & 15);
@@ -643,7 +648,7 @@ size_t TraceWideSpillBytes(int cu_count, int width, int height) {
 
 hipError_t LaunchTraceWide(const WideTree& tree, const dpl_ray* rays, int64_t n, uint32_t flags, dpl_hit* hits,
                            int cu_count, uint32_t* counter, int32_t* spill, unsigned long long* counters,
-                           hipStream_t s) {
+                           hipStream_t s, const int32_t* order) {
     if (n <= 0) return hipSuccess;
     if (n > int64_t(0x7FFFFFFF)) return hipErrorInvalidValue;  // the 32-bit ray counter never wraps
     if (tree.width != 4 && tree.width != 8) return hipErrorInvalidValue;
@@ -656,7 +661,8 @@ hipError_t LaunchTraceWide(const WideTree& tree, const dpl_ray* rays, int64_t n,
                        hits, counter, spill, counters, nullptr, nullptr, 0)
 #define DPL_TRACE_WR(W_, ANY_)                                                                                  \
     hipLaunchKernelGGL((KTraceWideRefill<W_, false, ANY_, kWidePending>), dim3(grid), dim3(kWideBlock), 0, s, tree, \
-                       r4, n, (flags & DPL_TRACE_CULL_BACK) != 0, hits, counter, spill, nullptr, nullptr, 0, refill)
+                       r4, n, (flags & DPL_TRACE_CULL_BACK) != 0, hits, counter, spill, nullptr, nullptr, 0, refill, \
+                       order)
     const bool any = flags & DPL_TRACE_ANY;
     if (counters || refill == 0) {  // the counters count the ordered traversal
         if (tree.width == 4) {
@@ -700,7 +706,7 @@ hipError_t LaunchTraceWideBatched(const WideTree& tree, const BatchTree* mt, int
     }
 #define DPL_TRACE_WB(W_, ANY_)                                                                                  \
     hipLaunchKernelGGL((KTraceWideRefill<W_, true, ANY_, kWidePending>), dim3(grid), dim3(kWideBlock), 0, s, tree,  \
-                       r4, n, cull, hits, counter, spill, ray_mesh, mt, mesh_count, refill)
+                       r4, n, cull, hits, counter, spill, ray_mesh, mt, mesh_count, refill, nullptr)
     if (tree.width == 4) {
         if (any) DPL_TRACE_WB(4, true);
         else DPL_TRACE_WB(4, false);

@@ -115,10 +115,18 @@ typedef struct dpl_hit {
 #define DPL_TRACE_ANY 1u       /* any-hit (shadow) query: IntersectAny */
 #define DPL_TRACE_CULL_BACK 2u /* backface culling */
 #define DPL_TRACE_WIDE 4u      /* traverse the wide tree made by dpl_bvh_widen */
+/* dpl_trace: sort the rays into a coherent order before tracing them (by where each ray's
+ * span crosses the scene box; hits still land at each ray's own index and equal the unsorted
+ * trace's bit for bit). Pays for batches of incoherent rays (about 10% on 4M random rays
+ * through a 2M-triangle scene, sort included); camera rays in scanline order lose by it. */
+#define DPL_TRACE_COHERENT 8u
 
 /* Context: one device, one stream, scratch pools. The reference selects the device with
  * cudaSetDevice(0) (src/gpu/cuda_utils.h:41). */
 dpl_status dpl_context_create(int32_t device, dpl_context** out);
+/* BVHs and trace graphs should be destroyed first. Those still alive are detached: afterwards
+ * their destroy call frees their device memory, and every other call on them returns
+ * DPL_ERR_INVALID_ARG. */
 void dpl_context_destroy(dpl_context* ctx);
 const char* dpl_last_error(const dpl_context* ctx);
 /* stream: a hipStream_t; NULL selects the device's null stream. A new context uses a

@@ -43,8 +43,8 @@ def test_sobb_trace_matches_brute_force(oracle, ctx, kdop, flags):
     else:
         assert hits_equal(h, brute).all()
     d = g.download()
-    if not flags & 1:
-        assert hits_equal(h, oracle.trace_final(d["nodes"], d["tri_ordered"], rays, flags, sobb_kdop=kdop)).all()
+    # any-hit included: the first hit of the ordered traversal, whatever the wave around it
+    assert hits_equal(h, oracle.trace_final(d["nodes"], d["tri_ordered"], rays, flags, sobb_kdop=kdop)).all()
     # SOBB trees visit fewer nodes than AABB trees on this scene of thin triangles
     _, va, _ = gpu_bvh(ctx, s.tris, mt=1).trace_counters(rays)
     _, vs, _ = g.trace_counters(rays)
@@ -249,5 +249,61 @@ def test_trace_error_paths(ctx):
 
     g = gpu_bvh(ctx, soup(10, 1))
     with pytest.raises(accel.DplError):
-        g.trace(np.zeros((4, 8), np.float32), flags=8)
+        g.trace(np.zeros((4, 8), np.float32), flags=16)
     assert len(g.trace(np.zeros((0, 8), np.float32))) == 0
+
+
+def test_coherent_order_gives_the_same_hits(ctx):
+    """DPL_TRACE_COHERENT sorts the rays before tracing: the hits at every ray's index equal
+    the unsorted trace's bit for bit (closest, any-hit, culled, wide), including rays that miss
+    the scene box, start inside it, are cut short by tmax or carry NaN, and small batches."""
+    import torch
+    from diplodocus_amd import _lib
+
+    tris = scenes.mixed_2m(n_small=60_000, n_thin=8_000).tris
+    g = gpu_bvh(ctx, tris, mt=1, bv=1)
+    n = (1 << 20) + 4097
+    rays = scenes.random_rays_towards(tris, n, seed=5)
+    rng = np.random.default_rng(5)
+    k = rng.choice(n, 3000, replace=False)
+    rays[k[:1000], 4:7] *= -1.0  # aimed away from the scene: they miss the box
+    v = tris.reshape(-1, 3)
+    rays[k[1000:2000], 0:3] = v[rng.integers(0, len(v), 1000)] * 0.5 + v.mean(0) * 0.5  # inside the box
+    rays[k[2000:2500], 7] = 0.5  # short spans
+    rays[k[2500:2600], 4] = np.nan
+    dev = torch.from_numpy(rays).cuda()
+    for flags in (0, _lib.DPL_TRACE_ANY, _lib.DPL_TRACE_CULL_BACK):
+        a = g.trace(dev, flags | _lib.DPL_TRACE_COHERENT)
+        b = g.trace(dev, flags)
+        assert torch.equal(a.view(torch.int32), b.view(torch.int32)), flags
+    small = dev[:777]
+    a = g.trace(small, _lib.DPL_TRACE_COHERENT)
+    assert torch.equal(a.view(torch.int32), g.trace(small).view(torch.int32))
+    host = g.trace(rays[:5000], _lib.DPL_TRACE_COHERENT)  # host buffers
+    assert hits_equal(host, g.trace(rays[:5000])).all()
+    g.widen(4)
+    a = g.trace(dev, _lib.DPL_TRACE_WIDE | _lib.DPL_TRACE_COHERENT)
+    b = g.trace(dev, _lib.DPL_TRACE_WIDE)
+    assert torch.equal(a.view(torch.int32), b.view(torch.int32))
+    g.close()
+
+
+def test_handles_outliving_their_context():
+    """A BVH and a trace graph still alive when their context is destroyed are detached:
+    calls on them return an error and their destroy frees only their own memory."""
+    import torch
+    from diplodocus_amd import accel
+
+    c = accel.Context(0)
+    g = gpu_bvh(c, soup(500, 3))
+    rays = torch.from_numpy(scenes.random_rays_towards(soup(500, 3), 256, seed=2)).cuda()
+    hits = torch.empty((256, 4), dtype=torch.float32, device="cuda")
+    gr = accel.TraceGraph(g, rays, hits)
+    gr.launch()
+    c.close()
+    with pytest.raises(accel.DplError):
+        g.build(soup(10, 1), accel.make_params())
+    with pytest.raises(accel.DplError):
+        gr.launch()
+    gr.close()
+    g.close()
