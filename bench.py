@@ -33,20 +33,66 @@ def cpu_model() -> str:
     return platform.processor()
 
 
-def measured_traffic(variant: str, radius: int):
-    """HBM bytes per merging PLOC-iteration launch for this workload, as measured by the two
-    rocprofv3 --pmc passes (FETCH_SIZE, WRITE_SIZE) summarised in the profile below, or None
-    when no measurement of this variant and radius is recorded there."""
-    path = os.path.join(ROOT, "profiles", "kploc_iteration_mixed2m_pmc.md")
+PMC_PROFILE = os.path.join(ROOT, "profiles", "build_trace_mixed2m_pmc.md")
+
+
+def profile_value(key: str, variant: str, radius: int):
+    """The figure `key: <variant> r<radius> = V` recorded in the counter profile (rocprofv3
+    --pmc passes summarised by --pmc-summary), or None when it holds none for this variant and
+    radius."""
     try:
-        for line in open(path):
-            if line.startswith("traffic_bytes_per_launch:"):
-                key, _, val = line.split(":", 1)[1].partition("=")
-                if key.strip() == f"{variant} r{radius}":
-                    return int(val.strip())
+        for line in open(PMC_PROFILE):
+            if line.startswith(key + ":"):
+                k, _, val = line.split(":", 1)[1].partition("=")
+                if k.strip() == f"{variant} r{radius}":
+                    return float(val.strip())
     except OSError:
         pass
     return None
+
+
+def measured_traffic(variant: str, radius: int):
+    """HBM bytes per merging PLOC-iteration launch for this workload, from the counter profile
+    (None when it has no measurement of this variant and radius)."""
+    v = profile_value("traffic_bytes_per_launch", variant, radius)
+    return None if v is None else int(v)
+
+
+def measured_build_traffic(variant: str, radius: int, build_ms: float):
+    """HBM bytes of one whole build (every kernel between the build's first and last event),
+    from the counter profile, with the rate they give over this run's build time; None when no
+    measurement is recorded."""
+    b = profile_value("build_traffic_bytes", variant, radius)
+    if b is None:
+        return None
+    gbs = b / (build_ms * 1e-3) / 1e9
+    return {"bytes": int(b), "gbs": round(gbs, 2), "frac": round(gbs / HBM_PEAK_GBS, 5),
+            "source": os.path.relpath(PMC_PROFILE, ROOT)}
+
+
+def trace_roofline(variant: str, radius: int, nrays: int, trace_ms: float, touched_per_ray: float):
+    """The trace's bytes and instruction issue. Bytes: the tree bytes a ray touches (64 per node
+    visit, 48 per triangle test, from the traversal counters) and the bytes per ray the L2
+    fetched from the fabric (counter profile), each as a rate over this run's trace time
+    against HBM's 8 TB/s (the scene's ~220 MB of nodes and triangles also fit the 256 MB
+    Infinity Cache, so fabric bytes are not all HBM bytes). Issue: VALU wave-instructions per
+    CU cycle and the fraction of lanes active in them, from the same profile."""
+    per_ray = profile_value("trace_bytes_per_ray", variant, radius)
+    issue = profile_value("trace_valu_per_cu_cycle", variant, radius)
+    lanes = profile_value("trace_valu_lane_util", variant, radius)
+    hit = profile_value("trace_l2_hit", variant, radius)
+    touched_gbs = touched_per_ray * nrays / (trace_ms * 1e-3) / 1e9
+    out = {"kernel": "KTraceRefill<256, SOBB, false, false, 4> (closest hit)", "touched_bytes_per_ray": round(touched_per_ray, 1),
+           "touched_gbs": round(touched_gbs, 2), "touched_frac": round(touched_gbs / HBM_PEAK_GBS, 5),
+           "counter_bytes_per_ray": None if per_ray is None else round(per_ray, 1),
+           "counter_gbs": None, "counter_frac": None, "l2_hit_ratio": hit,
+           "valu_per_cu_cycle": issue, "valu_lane_util": lanes,
+           "source": os.path.relpath(PMC_PROFILE, ROOT)}
+    if per_ray is not None:
+        g = per_ray * nrays / (trace_ms * 1e-3) / 1e9
+        out["counter_gbs"] = round(g, 2)
+        out["counter_frac"] = round(g / HBM_PEAK_GBS, 5)
+    return out
 
 
 def build_algorithmic_bytes(n: int, morton_type: int, digit_bits: int, iter_counts, kept_nodes: int) -> dict:
@@ -67,25 +113,6 @@ def build_algorithmic_bytes(n: int, morton_type: int, digit_bits: int, iter_coun
           "ploc": ploc, "layout": 64 * kept_nodes + (36 + 48) * n}
     st["total"] = sum(st.values())
     return st
-
-
-def measured_build_traffic(variant: str, radius: int, build_ms: float):
-    """HBM bytes of one whole build (every kernel between the build's first and last event), as
-    measured by rocprofv3 --pmc passes and summarised in the profile below, with the rate they
-    give over this run's build time; None when no measurement is recorded there."""
-    path = os.path.join(ROOT, "profiles", "build_mixed2m_pmc.md")
-    try:
-        for line in open(path):
-            if line.startswith("build_traffic_bytes:"):
-                key, _, val = line.split(":", 1)[1].partition("=")
-                if key.strip() == f"{variant} r{radius}":
-                    b = int(val.strip())
-                    gbs = b / (build_ms * 1e-3) / 1e9
-                    return {"bytes": b, "gbs": round(gbs, 2), "frac": round(gbs / HBM_PEAK_GBS, 5),
-                            "source": "profiles/build_mixed2m_pmc.md"}
-    except OSError:
-        pass
-    return None
 
 
 def _kname(full: str) -> str:
@@ -136,23 +163,47 @@ def pmc_summary(dirs, variant: str, radius: int, n_tris: int = 2_000_002, n_rays
         w, w64 = c.get("TCC_EA0_WRREQ_sum", 0), c.get("TCC_EA0_WRREQ_64B_sum", 0)
         return 32 * (w - w64) + 64 * w64
 
+    # FETCH_SIZE / WRITE_SIZE (KiB per dispatch) scaled by what they read for the plain copies
+    cp = [e for e in order if "copyBuffer" in e["kernel"]]
+    f_scale = w_scale = None
+    if cp and cp[-1]["c"].get("FETCH_SIZE", 0) > 0:
+        f_scale = 268435456 / (cp[-1]["c"]["FETCH_SIZE"] * 1024)
+    if cp and cp[-1]["c"].get("WRITE_SIZE", 0) > 0:
+        w_scale = 268435456 / (cp[-1]["c"]["WRITE_SIZE"] * 1024)
+
+    def fw(c):
+        f = c.get("FETCH_SIZE", 0) * 1024 * f_scale if f_scale else 0.0
+        w = c.get("WRITE_SIZE", 0) * 1024 * w_scale if w_scale else 0.0
+        return f, w
+
     start = next(i for i, e in enumerate(order) if e["kernel"] == "KBuildInit")
     end = next(i for i in range(start, len(order)) if order[i]["kernel"] == "KFinish")
     build = order[start:end + 1]
     per = {}
     for e in build:
-        t = per.setdefault(e["kernel"], [0, 0.0, 0.0])
+        t = per.setdefault(e["kernel"], [0, 0.0, 0.0, 0.0, 0.0])
         t[0] += 1
         t[1] += rd(e["c"])
         t[2] += wr(e["c"])
+        f, w = fw(e["c"])
+        t[3] += f
+        t[4] += w
     tot_r = sum(v[1] for v in per.values())
     tot_w = sum(v[2] for v in per.values())
     out = [f"Dispatches of the first {n_tris:,}-triangle build ({variant} r{radius}): {len(build)}.", "",
-           "| kernel | dispatches | read MB | written MB | read+written per dispatch (MB) |", "|---|---|---|---|---|"]
-    for k, (cnt, r, w) in sorted(per.items(), key=lambda kv: -(kv[1][1] + kv[1][2])):
-        out.append(f"| `{k}` | {cnt} | {r / 1e6:.2f} | {w / 1e6:.2f} | {(r + w) / cnt / 1e6:.3f} |")
+           "| kernel | dispatches | read MB | written MB | read+written per dispatch (MB) | FETCH_SIZE MB | "
+           "WRITE_SIZE MB |", "|---|---|---|---|---|---|---|"]
+    for k, (cnt, r, w, f, fwr) in sorted(per.items(), key=lambda kv: -(kv[1][1] + kv[1][2])):
+        out.append(f"| `{k}` | {cnt} | {r / 1e6:.2f} | {w / 1e6:.2f} | {(r + w) / cnt / 1e6:.3f} | {f / 1e6:.2f} | "
+                   f"{fwr / 1e6:.2f} |")
     out += ["", f"Whole build: {tot_r / 1e6:.1f} MB read + {tot_w / 1e6:.1f} MB written = "
-                f"{(tot_r + tot_w) / 1e6:.1f} MB.", ""]
+                f"{(tot_r + tot_w) / 1e6:.1f} MB by the request counters"]
+    if f_scale and w_scale:
+        tf, tw = sum(v[3] for v in per.values()), sum(v[4] for v in per.values())
+        out[-1] += (f"; {tf / 1e6:.1f} MB fetched + {tw / 1e6:.1f} MB written = {(tf + tw) / 1e6:.1f} MB by "
+                    f"FETCH_SIZE x {f_scale:.4f} and WRITE_SIZE x {w_scale:.4f}")
+    out[-1] += "."
+    out.append("")
     merging = [e for e in build if e["kernel"].startswith("KPlocIteration")]
     # pass-through launches (clusters already at the tail size) move almost nothing
     merging = [e for e in merging if rd(e["c"]) + wr(e["c"]) > 256 * 1024]
@@ -162,29 +213,41 @@ def pmc_summary(dirs, variant: str, radius: int, n_tris: int = 2_000_002, n_rays
         out.append("")
         out.append(f"traffic_bytes_per_launch: {variant} r{radius} = {int(round(pl))}")
     out.append(f"build_traffic_bytes: {variant} r{radius} = {int(round(tot_r + tot_w))}")
-    tr = next((e for e in order[end:] if e["kernel"].startswith("KTrace<")), None)
+    tr = next((e for e in order[end:] if e["kernel"].startswith("KTrace")), None)
     if tr is not None:
         c = tr["c"]
+        tag = f"{variant} r{radius}"
         out += ["", f"## `{tr['kernel']}` ({n_rays:,} rays over the build's tree)", "",
                 f"read {rd(c) / 1e6:.1f} MB + written {wr(c) / 1e6:.1f} MB = "
-                f"{(rd(c) + wr(c)) / n_rays:.1f} B per ray beyond L2"]
+                f"{(rd(c) + wr(c)) / n_rays:.1f} B per ray beyond L2", "",
+                f"trace_bytes_per_ray: {tag} = {(rd(c) + wr(c)) / n_rays:.1f}"]
         if c.get("TCC_HIT_sum", 0) + c.get("TCC_MISS_sum", 0) > 0:
             h, m = c["TCC_HIT_sum"], c["TCC_MISS_sum"]
-            out.append(f"L2 hit ratio {h / (h + m):.4f} ({h:.3e} hits, {m:.3e} misses)")
+            out += [f"trace_l2_hit: {tag} = {h / (h + m):.4f}", "",
+                    f"L2 hit ratio {h / (h + m):.4f} ({h:.3e} hits, {m:.3e} misses)"]
         if c.get("SQ_ACTIVE_INST_VALU", 0) > 0:
             lanes = c["SQ_THREAD_CYCLES_VALU"] / (64 * c["SQ_ACTIVE_INST_VALU"])
-            out.append(f"VALU lanes active per VALU instruction {lanes:.3f}; SQ_INSTS_VALU {c['SQ_INSTS_VALU']:.4e}, "
-                       f"SQ_INSTS_VMEM_RD {c['SQ_INSTS_VMEM_RD']:.4e}, SQ_WAVES {c['SQ_WAVES']:.0f}, "
-                       f"SQ_WAVE_CYCLES {c['SQ_WAVE_CYCLES']:.4e}, SQ_BUSY_CYCLES {c['SQ_BUSY_CYCLES']:.4e}, "
-                       f"GRBM_GUI_ACTIVE {c['GRBM_GUI_ACTIVE']:.4e}")
+            # GRBM_GUI_ACTIVE is summed over the 8 XCDs (32 CUs each): per-XCD busy cycles of the
+            # dispatch = GRBM_GUI_ACTIVE / 8
+            cycles = c["GRBM_GUI_ACTIVE"] / 8
+            issue = c["SQ_INSTS_VALU"] / (256 * cycles)
+            out += [f"trace_valu_lane_util: {tag} = {lanes:.4f}", f"trace_valu_per_cu_cycle: {tag} = {issue:.4f}", "",
+                    f"VALU lanes active per VALU instruction {lanes:.3f}; VALU wave-instructions per CU cycle "
+                    f"{issue:.3f} ({c['SQ_INSTS_VALU']:.4e} over 256 CUs x {cycles:.4e} cycles); "
+                    f"SQ_INSTS_VMEM_RD {c['SQ_INSTS_VMEM_RD']:.4e}, SQ_WAVES {c['SQ_WAVES']:.0f}, "
+                    f"SQ_WAVE_CYCLES {c['SQ_WAVE_CYCLES']:.4e}, SQ_BUSY_CYCLES {c['SQ_BUSY_CYCLES']:.4e}, "
+                    f"GRBM_GUI_ACTIVE {c['GRBM_GUI_ACTIVE']:.4e}"]
         for key in ("TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_EA0_RDREQ_64B_sum", "TCC_EA0_RDREQ_128B_sum"):
             if key in c:
                 out.append(f"{key} {c[key]:.4e}")
-    cp = [e for e in order if "copyBuffer" in e["kernel"]]
     if cp:
         c = cp[-1]["c"]
         out += ["", f"Copy check: a 268,435,456-byte device copy reads {rd(c) / 268435456:.4f}x and writes "
-                    f"{wr(c) / 268435456:.4f}x its size by these counters."]
+                    f"{wr(c) / 268435456:.4f}x its size by the request counters"]
+        if f_scale and w_scale:
+            out[-1] += (f"; FETCH_SIZE reads {c['FETCH_SIZE']:.1f} KiB and WRITE_SIZE {c['WRITE_SIZE']:.1f} KiB for "
+                        f"its 262,144 KiB (scales {f_scale:.4f}, {w_scale:.4f})")
+        out[-1] += "."
     return "\n".join(out)
 
 
@@ -272,6 +335,22 @@ def main() -> int:
         f"{trace_ms.mean():.3f} ms, hit rate {(h >= 0).mean():.3f}, iterations {last['ploc_iterations']}, "
         f"stages {last['stage_ms']}")
 
+    # ---- the same trace with DPL_TRACE_COHERENT (rays sorted by where they cross the scene
+    #      box before tracing, sort included; hits must equal the step's) ----
+    from diplodocus_amd import _lib
+
+    step_hits = hits.clone()
+    bvh.trace(rays, _lib.DPL_TRACE_COHERENT, hits)
+    c0, c1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    c0.record(stream)
+    for _ in range(5):
+        bvh.trace(rays, _lib.DPL_TRACE_COHERENT, hits)
+    c1.record(stream)
+    torch.cuda.synchronize()
+    coherent_ms = c0.elapsed_time(c1) / 5
+    coherent_equal = bool(torch.equal(hits.view(torch.int32), step_hits.view(torch.int32)))
+    log(f"trace with DPL_TRACE_COHERENT: {coherent_ms:.3f} ms (sort included), hits equal: {coherent_equal}")
+
     # SAH with axis-aligned child boxes (a SOBB tree's boxes refitted): the same cost formula
     # as the CPU SAH BVH's, for the ratio that gates tree quality
     sah_aabb = bvh.sah_aabb()
@@ -280,15 +359,13 @@ def main() -> int:
     _, visits, tests = bvh.trace_counters(rays)
     touched = (visits * 64 + tests * 48) / nrays
 
-    from diplodocus_amd import _lib
-
     wide = {}
     if not args.minimal:
         # ---- wide trees of the same BVH (4 and 8 children, 8-bit quantized child boxes): the
         #      conversion (SOBB trees refitted to axis-aligned boxes first) and the closest-hit
         #      trace of the same rays, whose hits must equal the two-child trace's ----
 
-        two_child_hits = hits.clone()
+        two_child_hits = step_hits
         for width in (4, 8):
             bvh.widen(width)
             widen_ms = float(np.median([bvh.widen(width)["widen_ms"] for _ in range(3)]))
@@ -593,6 +670,12 @@ def main() -> int:
         "trace_nodes_per_ray": round(visits / nrays, 2),
         "trace_tris_per_ray": round(tests / nrays, 2),
         "trace_touched_bytes_per_ray": round(touched, 1),
+        "trace_coherent": {"flag": "DPL_TRACE_COHERENT", "trace_ms": round(coherent_ms, 4),
+                           "trace_mrays_s": round(nrays / (coherent_ms * 1e-3) / 1e6, 3),
+                           "hits_equal_step": coherent_equal,
+                           "note": "the same rays sorted by where they cross the scene box before tracing, "
+                                   "sort included; the step's trace_ms keeps the given order"},
+        "trace_roofline": trace_roofline(args.variant, args.radius, nrays, float(trace_ms.mean()), touched),
         "wide": dict(wide, workload="the step's BVH and rays; wide trees converted from it (SOBB trees refitted "
                                     "to axis-aligned boxes), 4 or 8 children, 8-bit quantized child boxes"),
         "wall_s": round(wall, 3),
