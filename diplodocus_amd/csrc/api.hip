@@ -156,7 +156,7 @@ using namespace dpl;
 // the array's name.
 namespace {
 constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
-constexpr size_t kNoAllocCounterBytes = 256;
+constexpr size_t kNoAllocCounterBytes = kTraceCounterBytes;
 static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
 std::mutex g_scratch_mu;
 std::vector<uint8_t*> g_scratch_addr;  // per device, null until looked up
@@ -613,7 +613,7 @@ dpl_status dpl_context_create(int32_t device, dpl_context** out) {
         for (auto& e : ctx->ev) DPL_HIP(hipEventCreate(&e));
         DPL_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocMapped));
         DPL_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&ctx->pinned_dev), ctx->pinned, 0));
-        ctx->trace_counter.ensure(256);
+        ctx->trace_counter.ensure(kTraceCounterBytes);
         ctx->noalloc_scratch = ScratchAddress(device);
     });
     if (st != DPL_OK) return st;
@@ -1125,7 +1125,7 @@ dpl_status dpl_trace_graph_create(dpl_context* ctx, const dpl_bvh* bvh, const dp
         Require(live != ctx->bvh_gen.end(), "the BVH has no built tree");
         g->bvh = bvh;
         g->generation = live->second;
-        uint32_t* counter = g->counter.ensure<uint32_t>(256);
+        uint32_t* counter = g->counter.ensure<uint32_t>(kTraceCounterBytes);
         const size_t spill_bytes = wide ? TraceWideSpillBytes(ctx->cu_count, bvh->wide_width, bvh->tree_height)
                                         : TraceSpillBytes(ctx->cu_count, bvh->tree_height);
         int32_t* spill = g->spill.ensure<int32_t>(std::max<size_t>(16, spill_bytes));
@@ -1374,7 +1374,7 @@ dpl_status WhittedImpl(dpl_context* ctx, const dpl_bvh* bvh, const dpl_whitted_p
         TraceTree tree{bvh->fnodes.as<FNode>(), bvh->tri_ordered.as<float4>(), bvh->node_count,
                        int32_t(bvh->prim_count), bvh->tree_height, bvh->ndirs, bvh->sobb_dirs.as<float>()};
         int32_t* spill = ctx->spill.ensure<int32_t>(std::max<size_t>(16, TraceSpillBytes(ctx->cu_count, bvh->tree_height)));
-        uint8_t* tc = ctx->trace_counter.ensure<uint8_t>(256);
+        uint8_t* tc = ctx->trace_counter.ensure<uint8_t>(kTraceCounterBytes);
         // the six totals sit past the pixel counter; zeroed by the launch
         unsigned long long* totals = stats ? reinterpret_cast<unsigned long long*>(tc + 64) : nullptr;
         hipEvent_t ev[2] = {nullptr, nullptr};

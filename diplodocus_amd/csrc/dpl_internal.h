@@ -6,6 +6,7 @@
 // add/sub/mul/div/sqrt in a fixed order, min/max in the std::min/std::max select form,
 // and is compiled with -ffp-contract=off (no FMA contraction) and IEEE denormals.
 #pragma once
+#include <cstdlib>
 
 #include <hip/hip_runtime.h>
 
@@ -353,6 +354,21 @@ hipError_t LaunchTrace(const TraceTree& tree, const dpl_ray* rays, int64_t n, ui
 // indices (rays missing the box last). buf holds 4 n words; temp RayOrderTempBytes(n). *order
 // is set to the sorted indices inside buf.
 size_t RayOrderTempBytes(int64_t n);
+// Ray shares of the refill traces: the rays are split into `shares` contiguous ranges (8 by
+// default, one per XCD; DPL_TRACE_SHARES, 1-32), range s first taken by the workgroups b with
+// b % shares == s, each with its own counter on its own 128-byte line (atomics from several
+// XCDs to different words of one line serialise as badly as to one word).
+constexpr int kTraceShares = 32;       // most shares
+constexpr int kTraceShareStride = 32;  // words between share counters
+constexpr size_t kTraceCounterBytes = size_t(kTraceShares) * kTraceShareStride * 4;
+inline int TraceSharesEnv() {
+    static const int v = [] {
+        const char* e = getenv("DPL_TRACE_SHARES");
+        const int x = e ? atoi(e) : 8;
+        return x < 1 ? 1 : (x > kTraceShares ? kTraceShares : x);
+    }();
+    return v;
+}
 hipError_t LaunchRayOrder(const dpl_ray* rays, int64_t n, const float lo[3], const float hi[3], uint32_t* buf,
                           void* temp, size_t temp_bytes, const int32_t** order, hipStream_t s);
 int TraceGridFor(int cu_count, int64_t n, int height, size_t spill_bytes);
