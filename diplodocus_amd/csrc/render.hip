@@ -96,6 +96,11 @@ constexpr int kRenderStack = 16;
 
 }  // namespace
 
+// A sample per lane: spp a power of two that divides the wave, and a 31-bit item count.
+__host__ __device__ inline bool SampleLanes(int spp, int64_t pixels) {
+    return spp > 1 && spp <= kWave && (spp & (spp - 1)) == 0 && pixels * spp < (int64_t(1) << 31);
+}
+
 template <bool ADJ, bool SOBB, int BLOCK, int MAXD>
 __global__ __launch_bounds__(BLOCK) void KRender(RenderArgs a, uint32_t* counter) {
     __shared__ int32_t s_stack[kRenderStack][BLOCK];
@@ -117,6 +122,13 @@ __global__ __launch_bounds__(BLOCK) void KRender(RenderArgs a, uint32_t* counter
         __syncthreads();
     }
     const int64_t pixels = int64_t(a.W) * a.H;
+    // One sample per lane when spp divides the wave (a pixel's samples then sit in adjacent
+    // lanes of one wave, spp lanes per pixel): spp times the parallelism of a pixel per lane.
+    // The forward sums a pixel's samples in sample order through shuffles, the same additions
+    // as one lane's loop; the adjoint's per-sample terms are independent.
+    const bool spl = SampleLanes(a.spp, pixels);
+    const int sh = spl ? __builtin_ctz(unsigned(a.spp)) : 0;
+    const int64_t items = pixels << sh;
     const V3 pos{a.basis[0], a.basis[1], a.basis[2]}, p00{a.basis[3], a.basis[4], a.basis[5]};
     const V3 qw{a.basis[6], a.basis[7], a.basis[8]}, qh{a.basis[9], a.basis[10], a.basis[11]};
     for (;;) {
@@ -124,17 +136,19 @@ __global__ __launch_bounds__(BLOCK) void KRender(RenderArgs a, uint32_t* counter
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const int64_t base = s_base[wid];
-        if (base >= pixels) break;
-        const int64_t p = base + lane;
-        if (p < pixels) {
+        if (base >= items) break;
+        const int64_t q = base + lane;
+        const int64_t p = q >> sh;
+        const int s0 = spl ? int(q & (a.spp - 1)) : 0, s1 = spl ? s0 + 1 : a.spp;
+        V3 acc{0.0f, 0.0f, 0.0f};
+        if (q < items) {
             const int x = int(p % a.W), y = int(p / a.W);
-            V3 acc{0.0f, 0.0f, 0.0f};
             V3 delta{0.0f, 0.0f, 0.0f};
             if (ADJ) {
                 const float inv = float(a.spp);
                 delta = {a.d_image[3 * p] / inv, a.d_image[3 * p + 1] / inv, a.d_image[3 * p + 2] / inv};
             }
-            for (int s = 0; s < a.spp; s++) {
+            for (int s = s0; s < s1; s++) {
                 Pcg rng;
                 PcgSeed(rng, a.seed, uint64_t(p) * uint64_t(a.spp) + uint64_t(s));
                 float fx = float(x), fy = float(y);
@@ -216,7 +230,18 @@ __global__ __launch_bounds__(BLOCK) void KRender(RenderArgs a, uint32_t* counter
                     acc = Add(acc, L);
                 }
             }
-            if (!ADJ) {
+        }
+        if (!ADJ) {
+            if (spl) {  // the pixel's first lane adds the others' samples in sample order
+                V3 tot = acc;
+                for (int k = 1; k < a.spp; k++) {
+                    const int src = lane + k < kWave ? lane + k : lane;
+                    const V3 o{__shfl(acc.x, src, kWave), __shfl(acc.y, src, kWave), __shfl(acc.z, src, kWave)};
+                    tot = Add(tot, o);
+                }
+                acc = tot;
+            }
+            if (q < items && (!spl || (q & (a.spp - 1)) == 0)) {
                 const float inv = float(a.spp);
                 a.image[3 * p] = acc.x / inv;
                 a.image[3 * p + 1] = acc.y / inv;
@@ -275,7 +300,8 @@ hipError_t LaunchRender(bool adjoint, const TraceTree& tree, const dpl_render_pa
         hipLaunchKernelGGL(KZero, dim3(1), dim3(256), 0, s, grad_emission, int64_t(3 * a.M));
     }
     constexpr int B = 256;  // same block and grid policy as the trace kernel (shared spill sizing)
-    const int grid = TraceGrid(cu_count, int64_t(a.W) * a.H);
+    const int64_t pixels = int64_t(a.W) * a.H;
+    const int grid = TraceGrid(cu_count, SampleLanes(a.spp, pixels) ? pixels * a.spp : pixels);
 #define DPL_RENDER(ADJ_, SOBB_, MD_) \
     hipLaunchKernelGGL((KRender<ADJ_, SOBB_, B, MD_>), dim3(grid), dim3(B), 0, s, a, counter)
     const bool sobb = tree.ndirs > 0;
