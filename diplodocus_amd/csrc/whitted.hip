@@ -7,7 +7,9 @@
 //
 // The recursion I = L_local + I(reflected) ks + I(refracted) kt is evaluated as a weighted
 // sum over the ray tree, depth first, reflection subtree first: a per-lane stack of
-// (ray, weight, depth) items. Persistent grid: waves take 64 pixels at a time from a counter.
+// (ray, weight, depth) items. Persistent grid: waves take 64 / G pixels at a time from a
+// counter, G lanes per pixel (1-8, more for small images), which trace the pixel's rays in
+// step and split each hit's light samples.
 // Every float operation follows the oracle's restatement in order (no FMA contraction), so
 // the image is bit-identical to it. With COUNT the kernel also keeps the reference's ray
 // counters (primary, secondary, shadow: cpu_raytracer.cc:103,131,147,223) and the traversal
@@ -24,6 +26,7 @@ namespace {
 
 constexpr float kEps = 1e-6f;  // the reference's kEpsilon (src/util/util.h:15)
 constexpr int kWhittedStack = 16;
+constexpr int kWhittedLanesMax = 8;  // most lanes per pixel (light samples traced side by side)
 
 // x^y for x >= 0, y >= 0 by exp2(y log2 x): exponent extraction, the atanh series of the
 // mantissa for log2, the Taylor series of e^(f ln 2) for the fraction (same as the oracle).
@@ -130,12 +133,15 @@ struct Tracer {
     uint32_t queries = 0;  // closest-hit and any-hit traces
     uint32_t shadow = 0;   // IsShadowed calls
 
-    __device__ HitResult Trace(V3 o, V3 d, float tmax, bool any, bool cull) {
-        queries++;
+    // count: this lane's trace is one of the reference's queries (a lane group traces its
+    // pixel's closest-hit rays on every lane; only the group's first lane counts them)
+    __device__ HitResult Trace(V3 o, V3 d, float tmax, bool any, bool cull, bool count = true) {
         const WtRay w = PrepareRay(o.x, o.y, o.z, kEps, d.x, d.y, d.z, tmax);
         if (SOBB) PrepareSobb<BLOCK>(sr, s_dirs, a.tree.ndirs, o.x, o.y, o.z, d.x, d.y, d.z);
-        return any ? Traverse<true, SOBB>(a.tree, w, sr, cull, st, cnt)
-                   : Traverse<false, SOBB>(a.tree, w, sr, cull, st, cnt);
+        TraceCounters none{0u, 0u};
+        TraceCounters& c = count ? cnt : none;
+        queries += count ? 1u : 0u;
+        return any ? Traverse<true, SOBB>(a.tree, w, sr, cull, st, c) : Traverse<false, SOBB>(a.tree, w, sr, cull, st, c);
     }
 };
 
@@ -149,9 +155,15 @@ __device__ __forceinline__ V3 OffsetRayOrigin(V3 pos, float eps, V3 gn, V3 dir) 
     return Add(pos, Mul(off, eps));
 }
 
-template <bool SOBB, int BLOCK>
+// Area-light next-event estimation of one hit (cpu_raytracer.cc:135-182) by a group of G
+// adjacent lanes that hold the same hit: lane g takes samples g, g + G, ... of each light, and
+// after every round of G samples the group adds the round's contributions to the colour in
+// sample order (shuffles from the group's lanes), so every lane ends with the colour of the
+// reference's sequential loop, bit for bit. Shadow rays are counted by the lane that traces
+// them. All lanes of a group run this together (their control flow is the same up to here).
+template <bool SOBB, int BLOCK, int G>
 __device__ V3 LocalAreaLights(const WhittedArgs& a, Tracer<SOBB, BLOCK>& T, int x, int y, const WHit& h,
-                              const dpl_material& m, V3 cam) {
+                              const dpl_material& m, V3 cam, int g, int lead) {
     V3 color{0.0f, 0.0f, 0.0f};
     const int cnt = a.samples;
     for (int al = 0; al < a.light_count; al++) {
@@ -162,34 +174,54 @@ __device__ V3 LocalAreaLights(const WhittedArgs& a, Tracer<SOBB, BLOCK>& T, int 
         const V3 v0 = Ld(ltri), v1 = Ld(ltri + 3), v2 = Ld(ltri + 6);
         const float area = 0.5f * Len(Cross(Sub(v1, v0), Sub(v2, v0)));
         const V3 lgn = Normalize(Cross(Sub(v1, v0), Sub(v2, v0)));
-        for (int s = 0; s < cnt; s++) {
-            const float r1 = AreaLightSample01(uint32_t(x), uint32_t(y), uint32_t(al), uint32_t(s), 0u);
-            const float r2 = AreaLightSample01(uint32_t(x), uint32_t(y), uint32_t(al), uint32_t(s), 1u);
-            const float u = r1 + r2 > 1.0f ? 1.0f - r1 : r1;
-            const float v = r1 + r2 > 1.0f ? 1.0f - r2 : r2;
-            const V3 lpos = Add(Add(v0, Mul(Sub(v1, v0), u)), Mul(Sub(v2, v0), v));
-            // IsShadowed (cpu_raytracer.cc:222-236), counted on entry
-            T.shadow++;
-            const V3 to_l = Sub(lpos, h.pos);
-            const float dist = Len(to_l);
-            if (!(dist < kEps)) {
-                const V3 sd = DivS(to_l, dist);
-                const V3 so = OffsetRayOrigin(h.pos, h.eps, h.gn, sd);
-                if (T.Trace(so, sd, dist * (1.0f - h.eps), true, false).prim >= 0) continue;
+        for (int r0 = 0; r0 < cnt; r0 += G) {
+            const int s = r0 + g;
+            V3 c{0.0f, 0.0f, 0.0f};
+            int lit = 0;
+            if (s < cnt) {
+                const float r1 = AreaLightSample01(uint32_t(x), uint32_t(y), uint32_t(al), uint32_t(s), 0u);
+                const float r2 = AreaLightSample01(uint32_t(x), uint32_t(y), uint32_t(al), uint32_t(s), 1u);
+                const float u = r1 + r2 > 1.0f ? 1.0f - r1 : r1;
+                const float v = r1 + r2 > 1.0f ? 1.0f - r2 : r2;
+                const V3 lpos = Add(Add(v0, Mul(Sub(v1, v0), u)), Mul(Sub(v2, v0), v));
+                // IsShadowed (cpu_raytracer.cc:222-236), counted on entry
+                T.shadow++;
+                const V3 to_l = Sub(lpos, h.pos);
+                const float dist = Len(to_l);
+                bool shadowed = false;
+                if (!(dist < kEps)) {
+                    const V3 sd = DivS(to_l, dist);
+                    const V3 so = OffsetRayOrigin(h.pos, h.eps, h.gn, sd);
+                    shadowed = T.Trace(so, sd, dist * (1.0f - h.eps), true, false).prim >= 0;
+                }
+                if (!shadowed) {
+                    const float d = Len(to_l);
+                    const V3 d_l = DivS(to_l, d);
+                    const float w = ((area * Max2(0.0f, Dot(lgn, Neg(d_l)))) * Max2(0.0f, Dot(h.normal, d_l))) /
+                                    ((float(cnt) * d * d) + kEps);
+                    const V3 lc = Mul(lcolor, w);
+                    // Phong (cpu_raytracer.cc:231-255)
+                    const V3 pl = Normalize(Sub(lpos, h.pos));
+                    const V3 pv = Normalize(Sub(cam, h.pos));
+                    const V3 pr = Normalize(Sub(Mul(Mul(h.normal, 2.0f), Dot(h.normal, pl)), pl));
+                    const V3 I_a = Mul(lc, V3{0.0f, 0.0f, 0.0f});
+                    const V3 I_d = Mul(Mul(lc, Ld(m.diffuse)), Max2(0.0f, Dot(h.normal, pl)));
+                    const V3 I_s = Mul(Mul(lc, Ld(m.specular)), PowPos(Max2(0.0f, Dot(pv, pr)), m.shininess));
+                    c = Add(Add(I_a, I_d), I_s);
+                    lit = 1;
+                }
             }
-            const float d = Len(to_l);
-            const V3 d_l = DivS(to_l, d);
-            const float w = ((area * Max2(0.0f, Dot(lgn, Neg(d_l)))) * Max2(0.0f, Dot(h.normal, d_l))) /
-                            ((float(cnt) * d * d) + kEps);
-            const V3 lc = Mul(lcolor, w);
-            // Phong (cpu_raytracer.cc:231-255)
-            const V3 pl = Normalize(Sub(lpos, h.pos));
-            const V3 pv = Normalize(Sub(cam, h.pos));
-            const V3 pr = Normalize(Sub(Mul(Mul(h.normal, 2.0f), Dot(h.normal, pl)), pl));
-            const V3 I_a = Mul(lc, V3{0.0f, 0.0f, 0.0f});
-            const V3 I_d = Mul(Mul(lc, Ld(m.diffuse)), Max2(0.0f, Dot(h.normal, pl)));
-            const V3 I_s = Mul(Mul(lc, Ld(m.specular)), PowPos(Max2(0.0f, Dot(pv, pr)), m.shininess));
-            color = Add(color, Add(Add(I_a, I_d), I_s));
+            // the round's samples in order (a shadowed or missing sample adds nothing)
+            if constexpr (G == 1) {
+                if (lit) color = Add(color, c);
+            } else {
+#pragma unroll
+                for (int k = 0; k < G; k++) {
+                    const int src = lead + k;
+                    const V3 o{__shfl(c.x, src, kWave), __shfl(c.y, src, kWave), __shfl(c.z, src, kWave)};
+                    if (__shfl(lit, src, kWave)) color = Add(color, o);
+                }
+            }
         }
     }
     return color;
@@ -197,7 +229,7 @@ __device__ V3 LocalAreaLights(const WhittedArgs& a, Tracer<SOBB, BLOCK>& T, int 
 
 }  // namespace
 
-template <bool SOBB, int BLOCK, bool COUNT>
+template <bool SOBB, int BLOCK, bool COUNT, int G>
 __global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* counter, unsigned long long* totals) {
     __shared__ int32_t s_stack[kWhittedStack][BLOCK];
     __shared__ uint32_t s_base[BLOCK / kWave];
@@ -217,15 +249,19 @@ __global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* count
     const V3 qw{a.basis[6], a.basis[7], a.basis[8]}, qh{a.basis[9], a.basis[10], a.basis[11]};
     const V3 bg{a.background[0], a.background[1], a.background[2]};
     uint32_t primary = 0, secondary = 0;
+    // G adjacent lanes per pixel: each traces the pixel's rays (the same ones, in step) and a
+    // share of every hit's light samples
+    const int g = lane % G, lead = lane - g;
+    const bool first = g == 0;
     for (;;) {
-        if (lane == 0) s_base[wid] = atomicAdd(counter, uint32_t(kWave));
+        if (lane == 0) s_base[wid] = atomicAdd(counter, uint32_t(kWave / G));
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const int64_t base = s_base[wid];
         if (base >= pixels) break;
-        const int64_t p = base + lane;
+        const int64_t p = base + lane / G;
         if (p < pixels) {
-            primary++;
+            primary += first ? 1u : 0u;
             const int x = int(p % a.W), y = int(p / a.W);
             Item stack[kDplWhittedMaxDepth + 2];
             int sp = 0;
@@ -234,7 +270,7 @@ __global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* count
             V3 acc{0.0f, 0.0f, 0.0f};
             while (sp > 0) {
                 const Item it = stack[--sp];
-                const HitResult hr = T.Trace(it.o, it.d, it.tmax, false, a.cull != 0);
+                const HitResult hr = T.Trace(it.o, it.d, it.tmax, false, a.cull != 0, first);
                 if (hr.prim < 0) {
                     acc = Add(acc, Mul(it.w, bg));
                     continue;
@@ -257,7 +293,7 @@ __global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* count
                     if (NonZero(vn) || NonZero(vn + 3) || NonZero(vn + 6))
                         h.normal = Normalize(Add(Add(Mul(Ld(vn), b0), Mul(Ld(vn + 3), hr.b1)), Mul(Ld(vn + 6), hr.b2)));
                 }
-                acc = Add(acc, Mul(it.w, LocalAreaLights<SOBB, BLOCK>(a, T, x, y, h, m, cam)));
+                acc = Add(acc, Mul(it.w, LocalAreaLights<SOBB, BLOCK, G>(a, T, x, y, h, m, cam, g, lead)));
                 if (it.depth < a.max_depth) {
                     if (NonZero(m.transmittance)) {  // cpu_raytracer.cc:264-287
                         const V3 v = Neg(it.d);
@@ -280,20 +316,22 @@ __global__ __launch_bounds__(BLOCK) void KWhitted(WhittedArgs a, uint32_t* count
                         }
                         stack[sp++] = Item{OffsetRayOrigin(h.pos, h.eps, h.gn, dir), dir, Mul(it.w, Ld(m.transmittance)),
                                            it.tmax, it.depth + 1};
-                        secondary++;
+                        secondary += first ? 1u : 0u;
                     }
                     if (NonZero(m.specular)) {  // cpu_raytracer.cc:257-262
                         const V3 dv = Neg(it.d);
                         const V3 dir = Add(Neg(dv), Mul(h.normal, 2.0f * Dot(dv, h.normal)));
                         stack[sp++] = Item{OffsetRayOrigin(h.pos, h.eps, h.gn, dir), dir, Mul(it.w, Ld(m.specular)),
                                            it.tmax, it.depth + 1};
-                        secondary++;
+                        secondary += first ? 1u : 0u;
                     }
                 }
             }
-            a.image[3 * p] = acc.x;
-            a.image[3 * p + 1] = acc.y;
-            a.image[3 * p + 2] = acc.z;
+            if (first) {
+                a.image[3 * p] = acc.x;
+                a.image[3 * p + 1] = acc.y;
+                a.image[3 * p + 2] = acc.z;
+            }
         }
         __builtin_amdgcn_wave_barrier();
     }
@@ -345,18 +383,30 @@ hipError_t LaunchWhitted(const TraceTree& tree, const dpl_whitted_params& wp, co
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     constexpr int B = 256;
-    const int grid = TraceGrid(cu_count, int64_t(a.W) * a.H);
-    if (tree.ndirs > 0) {
-        if (totals)
-            hipLaunchKernelGGL((KWhitted<true, B, true>), dim3(grid), dim3(B), 0, s, a, counter, totals);
-        else
-            hipLaunchKernelGGL((KWhitted<true, B, false>), dim3(grid), dim3(B), 0, s, a, counter, totals);
-    } else {
-        if (totals)
-            hipLaunchKernelGGL((KWhitted<false, B, true>), dim3(grid), dim3(B), 0, s, a, counter, totals);
-        else
-            hipLaunchKernelGGL((KWhitted<false, B, false>), dim3(grid), dim3(B), 0, s, a, counter, totals);
+    // lanes per pixel: enough pixel-lanes to fill the persistent grid (8 workgroups of 256
+    // lanes per CU), at most 8; a large image keeps one lane per pixel (the group's lanes
+    // repeat the pixel's closest-hit traces)
+    const int64_t pixels = int64_t(a.W) * a.H;
+    int G = 1;
+    while (G < kWhittedLanesMax && pixels * G < int64_t(cu_count) * 8 * B) G *= 2;
+    const int grid = TraceGrid(cu_count, pixels * G);
+#define DPL_WHITTED(G_)                                                                                           \
+    do {                                                                                                          \
+        if (tree.ndirs > 0) {                                                                                     \
+            if (totals) hipLaunchKernelGGL((KWhitted<true, B, true, G_>), dim3(grid), dim3(B), 0, s, a, counter, totals); \
+            else hipLaunchKernelGGL((KWhitted<true, B, false, G_>), dim3(grid), dim3(B), 0, s, a, counter, totals);       \
+        } else {                                                                                                  \
+            if (totals) hipLaunchKernelGGL((KWhitted<false, B, true, G_>), dim3(grid), dim3(B), 0, s, a, counter, totals); \
+            else hipLaunchKernelGGL((KWhitted<false, B, false, G_>), dim3(grid), dim3(B), 0, s, a, counter, totals);       \
+        }                                                                                                         \
+    } while (0)
+    switch (G) {
+        case 1: DPL_WHITTED(1); break;
+        case 2: DPL_WHITTED(2); break;
+        case 4: DPL_WHITTED(4); break;
+        default: DPL_WHITTED(8); break;
     }
+#undef DPL_WHITTED
     return hipGetLastError();
 }
 

@@ -219,3 +219,22 @@ def test_gpu_whitted_icosphere_vertex_normals(oracle, ctx, bv, cull):
     ref = oracle.whitted(oracle.Ploc(s.tris, 1, 16, 4), s.mat_id, mats, 160, 120, 4, 4, s.camera,
                          background=(0.1, 0.2, 0.3), cull=cull, vertex_normals=vn)
     assert (img.view(np.uint32) == ref.view(np.uint32)).all(), np.abs(img - ref).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("side", [384, 512, 768])
+def test_gpu_whitted_lanes_per_pixel(oracle, ctx, side):
+    """Larger images run with 4, 2 and 1 lanes per pixel on a 256-CU GPU (small ones with 8):
+    image and counters equal the oracle's for a sample count that is no multiple of the
+    lanes (3 samples per light)."""
+    from diplodocus_amd import accel, render
+
+    s, mats = scenes.cornell_whitted()
+    g = accel.GpuBvh(ctx).build(s.tris, accel.make_params(0, 16, 4))
+    lights = render.emissive_triangles(s.mat_id, mats)
+    wp = render.make_whitted_params(side, side, 3, 3, s.camera)
+    img, st = render.render_whitted(g, wp, _gpu(s.tris), _gpu(s.mat_id), _gpu(mats), _gpu(lights), stats=True)
+    img = img.cpu().numpy()
+    ref, rc = oracle.whitted(oracle.Ploc(s.tris, 0, 16, 4), s.mat_id, mats, side, side, 3, 3, s.camera, counts=True)
+    assert (img.view(np.uint32) == ref.view(np.uint32)).all(), np.abs(img - ref).max()
+    assert {k: st[k] for k in rc} == rc
