@@ -156,6 +156,9 @@ using namespace dpl;
 // the array's name.
 namespace {
 constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+// PLOC iterations with more tiles than this many rounds of the fused kernel's workgroups run
+// as two launches (neighbours, then merges) instead of one
+constexpr int kPlocSplitRounds = 1;
 constexpr size_t kNoAllocCounterBytes = kTraceCounterBytes;
 static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
 std::mutex g_scratch_mu;
@@ -428,6 +431,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
         std::swap(b.keys, b.keys_alt);
         std::swap(b.vals, b.vals_alt);
     }
+    b.pairs = b.vals_alt;  // free once the sort is done
     DPL_HIP(hipEventRecord(C.ev[3], s));
     DPL_HIP(LaunchInitClusters(b, s));
 
@@ -454,6 +458,10 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
         est = std::max(1, std::min(est, 96));
         const int64_t T = (c_known + p.block_size - 1) / p.block_size;  // tiles at one cluster per thread
         const int64_t per_tile = p.block_size - 2 * p.nn_search_radius;
+        // tiles of the two-launch iteration (two clusters per thread) at the known count: an
+        // upper bound for every iteration of this batch
+        const int64_t split_chunk = PlocChunk(p.block_size) - 2 * p.nn_search_radius;
+        const int64_t split_tiles = (c_known + split_chunk - 1) / split_chunk;
         for (int g = 0; g < est; g++) {
             // once the modelled count (c_known / rate^g) needs at most two tiles per CU (two
             // 1024-thread workgroups are resident per CU): four lanes per neighbour search, twice
@@ -462,9 +470,17 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
             const int64_t t4 = int64_t(std::ceil(double(c_known) / std::pow(rate, g) / double(per_tile)));
             const bool four = p.block_size == 256 && t4 <= 2 * int64_t(C.cu_count);
             const int grid = int(four ? std::min<int64_t>(2 * t4, 2 * int64_t(C.cu_count)) : std::min<int64_t>(T, cap));
+            // more tiles of two clusters per thread than kPlocSplitRounds rounds of the fused
+            // kernel's workgroups: two launches that never wait on each other
+            const int64_t t2 = int64_t(std::ceil(double(c_known) / std::pow(rate, g) / double(split_chunk)));
+            const bool split = t2 > int64_t(kPlocSplitRounds) * cap;
             if (time_iters) DPL_HIP(hipEventRecord(C.IterEvent(2 * size_t(iter)), s));
-            DPL_HIP(LaunchPlocIteration(b, iter, p.nn_search_radius, p.block_size, four ? 4 : 1, tail,
-                                        p.max_triangles_per_leaf, grid, cap, s));
+            if (split)
+                DPL_HIP(LaunchPlocIterationSplit(b, iter, p.nn_search_radius, p.block_size, tail,
+                                                 p.max_triangles_per_leaf, int(std::min<int64_t>(split_tiles, 1 << 16)), s));
+            else
+                DPL_HIP(LaunchPlocIteration(b, iter, p.nn_search_radius, p.block_size, four ? 4 : 1, tail,
+                                            p.max_triangles_per_leaf, grid, cap, s));
             if (time_iters) DPL_HIP(hipEventRecord(C.IterEvent(2 * size_t(iter) + 1), s));
             iter++;
         }

@@ -397,6 +397,195 @@ __global__ __launch_bounds__(BLOCK * LANES) void KPlocIteration(BuildBuffers b, 
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// The same iteration in two launches, for cluster counts of several rounds of tiles per
+// workgroup. In the fused kernel the workgroups of a CU move through their tiles in step
+// (window, search, look-back, merge), so the search's arithmetic and the other phases' memory
+// latency never overlap. Here no workgroup waits on another: KPlocNeighbours finds every
+// cluster's nearest neighbour and merge decision, KPlocMerge sums each tile's offsets from the
+// per-tile counts and merges and compacts. Same tiles, same search, same merges and order.
+//
+// Decision word per cluster i (in b.pairs): bits 0..29 the neighbour j, bit 30 the cluster's
+// slot survives (not the higher index of a mutual pair), bit 31 it creates the merge (the
+// lower index of a mutual pair). Tile counts (survivors << 32 | merges) go to the iteration's
+// look-back words, which the fused kernel does not use in this iteration.
+// ---------------------------------------------------------------------------------------
+constexpr uint32_t kPairSurvives = 1u << 30, kPairMerges = 1u << 31, kPairIndex = kPairSurvives - 1;
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void KPlocNeighbours(BuildBuffers b, int iter, int R, int tail) {
+    constexpr int CHUNK = BLOCK * kIpt;
+    constexpr int WMAX = CHUNK + 4 * kMaxRadius;
+    __shared__ float s_lx[WMAX], s_ly[WMAX], s_lz[WMAX], s_hx[WMAX], s_hy[WMAX], s_hz[WMAX];
+    __shared__ int32_t s_nn[CHUNK + 2 * kMaxRadius];
+    __shared__ uint32_t s_scan[BLOCK / kWave + 1];
+
+    const int c = b.ccount[iter];
+    if (c <= tail) return;  // KPlocMerge passes the clusters through
+    const PNode* __restrict__ in = b.clusters[iter & 1];
+    const int chunk = CHUNK - 2 * R;
+    const int T = (c + chunk - 1) / chunk;
+    uint64_t* counts = b.status[iter & 1];
+    const WindowLds w{s_lx, s_ly, s_lz, s_hx, s_hy, s_hz};
+    for (int tile = blockIdx.x; tile < T; tile += gridDim.x) {
+        const int a = tile * chunk;
+        const int e = a + chunk < c ? a + chunk : c;
+        const int w0 = a - 2 * R;
+        const int wlen = (e - a) + 4 * R;
+        for (int t = threadIdx.x; t < wlen; t += BLOCK) {
+            const int j = w0 + t;
+            if (j >= 0 && j < c) {
+                const float4* p = reinterpret_cast<const float4*>(in + j);
+                const float4 q0 = p[0], q1 = p[1];
+                s_lx[t] = q0.x, s_ly[t] = q0.y, s_lz[t] = q0.z, s_hx[t] = q0.w;
+                s_hy[t] = q1.x, s_hz[t] = q1.y;
+            }
+        }
+        __syncthreads();
+        const int n0 = a - R < 0 ? 0 : a - R;
+        const int n1 = e + R > c ? c : e + R;
+        for (int i = n0 + threadIdx.x; i < n1; i += BLOCK) s_nn[i - (a - R)] = NearestNeighbour(w, i, w0, c, R);
+        __syncthreads();
+        uint32_t ns = 0, nm = 0;
+#pragma unroll
+        for (int q = 0; q < kIpt; q++) {
+            const int i = a + threadIdx.x * kIpt + q;
+            if (i < e) {
+                const int j = s_nn[i - (a - R)];
+                const bool mutual = s_nn[j - (a - R)] == i;
+                const bool survives = !(mutual && j < i), merges = mutual && i < j;
+                b.pairs[i] = uint32_t(j) | (survives ? kPairSurvives : 0u) | (merges ? kPairMerges : 0u);
+                ns += survives;
+                nm += merges;
+            }
+        }
+        uint32_t total;
+        BlockExclusiveScan<BLOCK>(ns | (nm << 16), s_scan, total);
+        if (threadIdx.x == 0) counts[tile] = (uint64_t(total & 0xFFFFu) << 32) | (total >> 16);
+        __syncthreads();  // the next tile overwrites the window
+    }
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void KPlocMerge(BuildBuffers b, int iter, int R, int tail, int max_leaf) {
+    constexpr int CHUNK = BLOCK * kIpt;
+    __shared__ uint32_t s_scan[BLOCK / kWave + 1];
+    __shared__ uint64_t s_sum[BLOCK / kWave];
+
+    const int c = b.ccount[iter];
+    const PNode* __restrict__ in = b.clusters[iter & 1];
+    PNode* __restrict__ out = b.clusters[(iter + 1) & 1];
+    const int64_t n = b.n;
+    if (c <= tail) {  // done with multi-workgroup iterations: pass the clusters through
+        for (int k = blockIdx.x * BLOCK + threadIdx.x; k < c; k += gridDim.x * BLOCK) out[k] = in[k];
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            b.ccount[iter + 1] = c;
+            b.tile_counter[(iter + 1) & 1] = 0u;
+            b.results->progress = c;
+        }
+        return;
+    }
+    const int chunk = CHUNK - 2 * R;
+    const int T = (c + chunk - 1) / chunk;
+    // the next iteration (fused or not) has at most Tn tiles; each tile clears kc of its words
+    const int Tn = (c + (BLOCK - 2 * R) - 1) / (BLOCK - 2 * R);
+    const int kc = (Tn + T - 1) / T;
+    const uint64_t* counts = b.status[iter & 1];
+    uint64_t* status_next = b.status[(iter + 1) & 1];
+    const int32_t node_base = int32_t(2 * n - c);
+    for (int tile = blockIdx.x; tile < T; tile += gridDim.x) {
+        if (int(threadIdx.x) < kc && tile * kc + int(threadIdx.x) < Tn) status_next[tile * kc + threadIdx.x] = 0ull;
+        if (threadIdx.x == 0 && tile == 0) b.tile_counter[(iter + 1) & 1] = 0u;
+        // exclusive prefix of the tile: the counts of every earlier tile (survivors in the high
+        // word, merges in the low word; neither sum reaches 2^32)
+        // (sixteen loads per thread in flight at once, not one round trip per load)
+        uint64_t part = 0;
+        for (int q0 = 0; q0 < tile; q0 += 16 * BLOCK) {
+            uint64_t v[16];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const int q = q0 + u * BLOCK + int(threadIdx.x);
+                v[u] = q < tile ? counts[q] : 0ull;
+            }
+#pragma unroll
+            for (int u = 0; u < 16; u++) part += v[u];
+        }
+        part = WaveSumU64(part);
+        if (LaneId() == 0) s_sum[threadIdx.x / kWave] = part;
+        const int a = tile * chunk;
+        const int e = a + chunk < c ? a + chunk : c;
+        uint32_t word[kIpt];
+        uint32_t sflags = 0, mflags = 0;
+#pragma unroll
+        for (int q = 0; q < kIpt; q++) {
+            const int i = a + threadIdx.x * kIpt + q;
+            word[q] = i < e ? b.pairs[i] : 0u;
+            if (word[q] & kPairSurvives) sflags |= 1u << q;
+            if (word[q] & kPairMerges) mflags |= 1u << q;
+        }
+        uint32_t total;
+        const uint32_t before = BlockExclusiveScan<BLOCK>(uint32_t(__popc(sflags)) | (uint32_t(__popc(mflags)) << 16),
+                                                          s_scan, total);
+        uint64_t excl = 0;
+#pragma unroll
+        for (int k = 0; k < BLOCK / kWave; k++) excl += s_sum[k];
+        if (threadIdx.x == 0 && tile == T - 1) {
+            const int32_t next = int32_t((excl >> 32) + (total & 0xFFFFu));
+            b.ccount[iter + 1] = next;
+            b.results->progress = next;
+            b.results->multi_iters = iter + 1;
+            b.results->flags = b.bounds[12];
+        }
+        int32_t spos = int32_t(excl >> 32) + int32_t(before & 0xFFFFu);
+        int32_t mpos = node_base + int32_t(excl & 0xFFFFFFFFull) + int32_t(before >> 16);
+#pragma unroll
+        for (int q = 0; q < kIpt; q++) {
+            if (!(sflags & (1u << q))) continue;
+            const int i = a + threadIdx.x * kIpt + q;
+            PNode o = in[i];
+            if (mflags & (1u << q)) {
+                const PNode pj = in[word[q] & kPairIndex];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    o.lo[k] = Min2(o.lo[k], pj.lo[k]);
+                    o.hi[k] = Max2(o.hi[k], pj.hi[k]);
+                }
+                const int32_t id = mpos++;
+                o.right = pj.left;
+                b.nodes[id] = o;
+                b.parent[o.left] = id;
+                b.parent[o.right] = id;
+                StoreEval(b, id, InnerEval(o.lo, o.hi, LoadEval(b, o.left), LoadEval(b, o.right), max_leaf));
+                if (b.ndirs > 3) UnionKdop(b, id, o.left, o.right);
+                o.left = id;
+            }
+            o.right = 0;
+            out[spos++] = o;
+        }
+        __syncthreads();  // s_sum and s_scan are reused by the next tile
+    }
+}
+
+hipError_t LaunchPlocIterationSplit(const BuildBuffers& b, int iter, int radius, int block, int tail, int max_leaf,
+                                    int grid, hipStream_t s) {
+    if (grid < 1) grid = 1;
+    switch (block) {
+        case 128:
+            hipLaunchKernelGGL(KPlocNeighbours<128>, dim3(grid), dim3(128), 0, s, b, iter, radius, tail);
+            hipLaunchKernelGGL(KPlocMerge<128>, dim3(grid), dim3(128), 0, s, b, iter, radius, tail, max_leaf);
+            break;
+        case 512:
+            hipLaunchKernelGGL(KPlocNeighbours<512>, dim3(grid), dim3(512), 0, s, b, iter, radius, tail);
+            hipLaunchKernelGGL(KPlocMerge<512>, dim3(grid), dim3(512), 0, s, b, iter, radius, tail, max_leaf);
+            break;
+        default:
+            hipLaunchKernelGGL(KPlocNeighbours<256>, dim3(grid), dim3(256), 0, s, b, iter, radius, tail);
+            hipLaunchKernelGGL(KPlocMerge<256>, dim3(grid), dim3(256), 0, s, b, iter, radius, tail, max_leaf);
+            break;
+    }
+    return hipGetLastError();
+}
+
 int PlocChunk(int block) { return block * kIpt; }
 // tiles hold at least block - 2 kMaxRadius clusters
 int64_t PlocStatusWords(int64_t n, int block) {
@@ -488,7 +677,7 @@ __global__ __launch_bounds__(256) void KEmit(BuildBuffers b) {
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void KFitSobb(BuildBuffers b) {
     const int32_t K = b.counters[5];
     __shared__ int4 s_tri[kMaxTriples];
-    __shared__ float s_w[kMaxDirs * 256];
+    __shared__ float s_w[2 * kMaxDirs * 256];
     for (int t = threadIdx.x; t < b.ntriples; t += blockDim.x) s_tri[t] = b.triples[t];
     __syncthreads();
     const float pad = KdopPad(b.bounds);

@@ -305,26 +305,65 @@ constexpr TripleTable13 MakeTripleTable13() {
 }
 constexpr TripleTable13 kTriples13 = MakeTripleTable13();
 
-// The 26-DOP triple scan unrolled at compile time: each step is the triple's area estimate
-// (w_a w_b + w_b w_c + w_a w_c) / |det| from widths held in registers, the first minimum
-// kept (strictly smaller wins); the result is the triple's a | b << 4 | c << 8.
+// The 26-DOP triple scan unrolled at compile time. A triple's area estimate is
+// ((p_ab + p_bc) + p_ac) / |det| with p_xy = w_x w_y (x < y), the 78 pairwise width products
+// computed once and shared by the triples. The triples are taken in chunks of kChunk13 in
+// table order: each chunk's minimum area (fminf: the areas are finite or +inf, never NaN,
+// since every width is at least twice the k-DOP pad), and the first chunk whose minimum is
+// strictly smaller than every earlier chunk's. The first minimum of the whole table is the
+// first triple of that chunk whose area equals its minimum (ResolveChunk13), the same triple
+// as a strict-minimum scan over all triples.
+constexpr int kChunk13 = 8;
+constexpr int kChunks13 = (kTriples13.n + kChunk13 - 1) / kChunk13;
+
 template <int T>
-__device__ __forceinline__ void ScanStep13(const float (&w)[13], float& best, uint32_t& abc) {
+__device__ __forceinline__ float TripleArea13(const float (&p)[13][13]) {
     constexpr Triple13 tr = kTriples13.t[T];
-    constexpr uint32_t code = uint32_t(tr.a) | (uint32_t(tr.b) << 4) | (uint32_t(tr.c) << 8);
-    const float wab = w[tr.a] * w[tr.b];
-    const float sa = ((wab + w[tr.b] * w[tr.c]) + w[tr.a] * w[tr.c]) * tr.inv_det;
-    if (T == 0 || sa < best) {
-        best = sa;
-        abc = code;
+    return ((p[tr.a][tr.b] + p[tr.b][tr.c]) + p[tr.a][tr.c]) * tr.inv_det;
+}
+
+template <int C, int U>
+__device__ __forceinline__ void ChunkStep13(const float (&p)[13][13], float& m) {
+    constexpr int T = C * kChunk13 + U;
+    if constexpr (U > 0 && T < kTriples13.n) m = fminf(m, TripleArea13<T>(p));
+}
+
+template <int C, int... U>
+__device__ __forceinline__ void ScanChunk13(const float (&p)[13][13], float& best, int& bc,
+                                            std::integer_sequence<int, U...>) {
+    float m = TripleArea13<C * kChunk13>(p);
+    (ChunkStep13<C, U>(p, m), ...);
+    if (C == 0 || m < best) {
+        best = m;
+        bc = C;
     }
 }
 
-template <int... T>
-__device__ __forceinline__ uint32_t ScanTriples13(const float (&w)[13], std::integer_sequence<int, T...>) {
-    float best = 0.0f;
+template <int... C>
+__device__ __forceinline__ void ScanChunks13(const float (&p)[13][13], float& best, int& bc,
+                                             std::integer_sequence<int, C...>) {
+    (ScanChunk13<C>(p, best, bc, std::make_integer_sequence<int, kChunk13>{}), ...);
+}
+
+// The first triple of chunk bc whose area equals best (the chunk's minimum), from the slabs in
+// this lane's LDS column wl (lo_k at k, hi_k at 13 + k, stride apart; width = hi - lo, the
+// same subtraction as the scan's) and the triple table in LDS (the same triples in the same
+// order, with the same 1/|det| bits): the same operations give the same areas.
+__device__ __forceinline__ uint32_t ResolveChunk13(const float* wl, int stride, const int4* triples, int bc,
+                                                   float best) {
     uint32_t abc = 0;
-    (ScanStep13<T>(w, best, abc), ...);
+#pragma unroll
+    for (int u = kChunk13 - 1; u >= 0; u--) {
+        const int t = bc * kChunk13 + u;
+        if (t < kTriples13.n) {
+            const int4 T = triples[t];
+            const float wa = wl[(13 + T.x) * stride] - wl[T.x * stride];
+            const float wb = wl[(13 + T.y) * stride] - wl[T.y * stride];
+            const float wc = wl[(13 + T.z) * stride] - wl[T.z * stride];
+            const float sa = ((wa * wb + wb * wc) + wa * wc) * __int_as_float(T.w);
+            if (sa == best) abc = uint32_t(T.x) | (uint32_t(T.y) << 4) | (uint32_t(T.z) << 8);
+        }
+    }
     return abc;
 }
 
@@ -381,18 +420,27 @@ __device__ __forceinline__ uint32_t FitSobbT(const BuildBuffers& b, const int4* 
             hi[3 + k] = -s[NX + k];
             w[3 + k] = -s[NX + k] - s[k];
         }
-        const uint32_t abc = ScanTriples13(w, std::make_integer_sequence<int, kTriples13.n>{});
+        // the slabs go to this lane's LDS column (lo at 0..12, hi at 13..25, stride apart): the
+        // chunk resolution and the chosen slabs read them there
+#pragma unroll
+        for (int k = 0; k < 13; k++) {
+            wl[k * stride] = lo[k];
+            wl[(13 + k) * stride] = hi[k];
+        }
+        float p[13][13];
+#pragma unroll
+        for (int x = 0; x < 13; x++)
+#pragma unroll
+            for (int y = x + 1; y < 13; y++) p[x][y] = w[x] * w[y];
+        float best = 0.0f;
+        int bc = 0;
+        ScanChunks13(p, best, bc, std::make_integer_sequence<int, kChunks13>{});
+        const uint32_t abc = ResolveChunk13(wl, stride, triples, bc, best);
 #pragma unroll
         for (int q = 0; q < 3; q++) {
             const uint32_t k = (abc >> (4 * q)) & 15u;
-            float l = lo[0], h = hi[0];
-#pragma unroll
-            for (int d = 1; d < 13; d++) {
-                l = k == uint32_t(d) ? lo[d] : l;
-                h = k == uint32_t(d) ? hi[d] : h;
-            }
-            f[2 * q] = l;
-            f[2 * q + 1] = h;
+            f[2 * q] = wl[k * stride];
+            f[2 * q + 1] = wl[(13 + k) * stride];
         }
         return abc;
     }
@@ -427,7 +475,7 @@ __device__ __forceinline__ uint32_t FitSobbT(const BuildBuffers& b, const int4* 
     return uint32_t(T.x) | (uint32_t(T.y) << 4) | (uint32_t(T.z) << 8);
 }
 
-// wl / stride: kMaxDirs LDS floats of this thread (the 26-DOP scan reads its widths there).
+// wl / stride: 2 kMaxDirs LDS floats of this thread (widths; the 26-DOP scan keeps its slabs there).
 __device__ __forceinline__ uint32_t FitSobb(const BuildBuffers& b, const int4* triples, int32_t id, float pad,
                                             float* f, float* wl, int stride) {
     switch (b.ndirs) {

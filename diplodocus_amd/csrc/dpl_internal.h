@@ -212,6 +212,8 @@ struct BuildBuffers {
     uint32_t* vals;        // n
     uint64_t* keys_alt;    // n
     uint32_t* vals_alt;    // n
+    uint32_t* pairs;       // n: per-cluster merge decisions of the two-launch PLOC iteration
+                           // (the sort's spare value buffer once the sort is done)
     PNode* nodes;          // 2n-1
     int32_t* parent;       // 2n-1
     PNode* clusters[2];    // n each
@@ -249,6 +251,10 @@ hipError_t LaunchInitClusters(const BuildBuffers& b, hipStream_t s);
 // search over four lanes; for iterations with about one tile per CU)
 hipError_t LaunchPlocIteration(const BuildBuffers& b, int iter, int radius, int block, int lanes, int tail,
                                int max_leaf, int grid, int min_tiles, hipStream_t s);
+// The same iteration as KPlocNeighbours + KPlocMerge (no workgroup waits on another), for
+// iterations with several rounds of tiles per workgroup; grid: workgroups of each launch.
+hipError_t LaunchPlocIterationSplit(const BuildBuffers& b, int iter, int radius, int block, int tail, int max_leaf,
+                                    int grid, hipStream_t s);
 hipError_t LaunchPlocTail(const BuildBuffers& b, int iter, int radius, int max_leaf, hipStream_t s);
 int PlocChunk(int block);
 int64_t PlocStatusWords(int64_t n, int block);  // look-back states per parity
