@@ -413,12 +413,13 @@ __global__ __launch_bounds__(BLOCK * LANES) void KPlocIteration(BuildBuffers b, 
 constexpr uint32_t kPairSurvives = 1u << 30, kPairMerges = 1u << 31, kPairIndex = kPairSurvives - 1;
 
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void KPlocNeighbours(BuildBuffers b, int iter, int R, int tail) {
+__global__ __launch_bounds__(BLOCK * kIpt) void KPlocNeighbours(BuildBuffers b, int iter, int R, int tail) {
     constexpr int CHUNK = BLOCK * kIpt;
+    constexpr int NT = CHUNK;  // one thread per searched cluster
     constexpr int WMAX = CHUNK + 4 * kMaxRadius;
     __shared__ float s_lx[WMAX], s_ly[WMAX], s_lz[WMAX], s_hx[WMAX], s_hy[WMAX], s_hz[WMAX];
     __shared__ int32_t s_nn[CHUNK + 2 * kMaxRadius];
-    __shared__ uint32_t s_scan[BLOCK / kWave + 1];
+    __shared__ uint32_t s_scan[NT / kWave + 1];
 
     const int c = b.ccount[iter];
     if (c <= tail) return;  // KPlocMerge passes the clusters through
@@ -432,7 +433,7 @@ __global__ __launch_bounds__(BLOCK) void KPlocNeighbours(BuildBuffers b, int ite
         const int e = a + chunk < c ? a + chunk : c;
         const int w0 = a - 2 * R;
         const int wlen = (e - a) + 4 * R;
-        for (int t = threadIdx.x; t < wlen; t += BLOCK) {
+        for (int t = threadIdx.x; t < wlen; t += NT) {
             const int j = w0 + t;
             if (j >= 0 && j < c) {
                 const float4* p = reinterpret_cast<const float4*>(in + j);
@@ -444,40 +445,40 @@ __global__ __launch_bounds__(BLOCK) void KPlocNeighbours(BuildBuffers b, int ite
         __syncthreads();
         const int n0 = a - R < 0 ? 0 : a - R;
         const int n1 = e + R > c ? c : e + R;
-        for (int i = n0 + threadIdx.x; i < n1; i += BLOCK) s_nn[i - (a - R)] = NearestNeighbour(w, i, w0, c, R);
+        for (int i = n0 + threadIdx.x; i < n1; i += NT) s_nn[i - (a - R)] = NearestNeighbour(w, i, w0, c, R);
         __syncthreads();
         uint32_t ns = 0, nm = 0;
-#pragma unroll
-        for (int q = 0; q < kIpt; q++) {
-            const int i = a + threadIdx.x * kIpt + q;
-            if (i < e) {
-                const int j = s_nn[i - (a - R)];
-                const bool mutual = s_nn[j - (a - R)] == i;
-                const bool survives = !(mutual && j < i), merges = mutual && i < j;
-                b.pairs[i] = uint32_t(j) | (survives ? kPairSurvives : 0u) | (merges ? kPairMerges : 0u);
-                ns += survives;
-                nm += merges;
-            }
+        const int i = a + int(threadIdx.x);
+        if (i < e) {
+            const int j = s_nn[i - (a - R)];
+            const bool mutual = s_nn[j - (a - R)] == i;
+            const bool survives = !(mutual && j < i), merges = mutual && i < j;
+            b.pairs[i] = uint32_t(j) | (survives ? kPairSurvives : 0u) | (merges ? kPairMerges : 0u);
+            ns = survives;
+            nm = merges;
         }
         uint32_t total;
-        BlockExclusiveScan<BLOCK>(ns | (nm << 16), s_scan, total);
+        BlockExclusiveScan<NT>(ns | (nm << 16), s_scan, total);
         if (threadIdx.x == 0) counts[tile] = (uint64_t(total & 0xFFFFu) << 32) | (total >> 16);
         __syncthreads();  // the next tile overwrites the window
     }
 }
 
+// One thread per cluster of the tile (BLOCK * kIpt threads): the merges' dependent loads
+// (cluster records, collapse records, k-DOP slabs) are spread over twice the waves.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK) void KPlocMerge(BuildBuffers b, int iter, int R, int tail, int max_leaf) {
+__global__ __launch_bounds__(BLOCK * kIpt) void KPlocMerge(BuildBuffers b, int iter, int R, int tail, int max_leaf) {
     constexpr int CHUNK = BLOCK * kIpt;
-    __shared__ uint32_t s_scan[BLOCK / kWave + 1];
-    __shared__ uint64_t s_sum[BLOCK / kWave];
+    constexpr int NT = CHUNK;
+    __shared__ uint32_t s_scan[NT / kWave + 1];
+    __shared__ uint64_t s_sum[NT / kWave];
 
     const int c = b.ccount[iter];
     const PNode* __restrict__ in = b.clusters[iter & 1];
     PNode* __restrict__ out = b.clusters[(iter + 1) & 1];
     const int64_t n = b.n;
     if (c <= tail) {  // done with multi-workgroup iterations: pass the clusters through
-        for (int k = blockIdx.x * BLOCK + threadIdx.x; k < c; k += gridDim.x * BLOCK) out[k] = in[k];
+        for (int k = blockIdx.x * NT + threadIdx.x; k < c; k += gridDim.x * NT) out[k] = in[k];
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             b.ccount[iter + 1] = c;
             b.tile_counter[(iter + 1) & 1] = 0u;
@@ -497,38 +498,31 @@ __global__ __launch_bounds__(BLOCK) void KPlocMerge(BuildBuffers b, int iter, in
         if (int(threadIdx.x) < kc && tile * kc + int(threadIdx.x) < Tn) status_next[tile * kc + threadIdx.x] = 0ull;
         if (threadIdx.x == 0 && tile == 0) b.tile_counter[(iter + 1) & 1] = 0u;
         // exclusive prefix of the tile: the counts of every earlier tile (survivors in the high
-        // word, merges in the low word; neither sum reaches 2^32)
-        // (sixteen loads per thread in flight at once, not one round trip per load)
+        // word, merges in the low word; neither sum reaches 2^32), eight loads per thread in
+        // flight at once
         uint64_t part = 0;
-        for (int q0 = 0; q0 < tile; q0 += 16 * BLOCK) {
-            uint64_t v[16];
+        for (int q0 = 0; q0 < tile; q0 += 8 * NT) {
+            uint64_t v[8];
 #pragma unroll
-            for (int u = 0; u < 16; u++) {
-                const int q = q0 + u * BLOCK + int(threadIdx.x);
+            for (int u = 0; u < 8; u++) {
+                const int q = q0 + u * NT + int(threadIdx.x);
                 v[u] = q < tile ? counts[q] : 0ull;
             }
 #pragma unroll
-            for (int u = 0; u < 16; u++) part += v[u];
+            for (int u = 0; u < 8; u++) part += v[u];
         }
         part = WaveSumU64(part);
         if (LaneId() == 0) s_sum[threadIdx.x / kWave] = part;
         const int a = tile * chunk;
         const int e = a + chunk < c ? a + chunk : c;
-        uint32_t word[kIpt];
-        uint32_t sflags = 0, mflags = 0;
-#pragma unroll
-        for (int q = 0; q < kIpt; q++) {
-            const int i = a + threadIdx.x * kIpt + q;
-            word[q] = i < e ? b.pairs[i] : 0u;
-            if (word[q] & kPairSurvives) sflags |= 1u << q;
-            if (word[q] & kPairMerges) mflags |= 1u << q;
-        }
+        const int i = a + int(threadIdx.x);
+        const uint32_t word = i < e ? b.pairs[i] : 0u;
+        const bool survives = (word & kPairSurvives) != 0, merges = (word & kPairMerges) != 0;
         uint32_t total;
-        const uint32_t before = BlockExclusiveScan<BLOCK>(uint32_t(__popc(sflags)) | (uint32_t(__popc(mflags)) << 16),
-                                                          s_scan, total);
+        const uint32_t before = BlockExclusiveScan<NT>(uint32_t(survives) | (uint32_t(merges) << 16), s_scan, total);
         uint64_t excl = 0;
 #pragma unroll
-        for (int k = 0; k < BLOCK / kWave; k++) excl += s_sum[k];
+        for (int k = 0; k < NT / kWave; k++) excl += s_sum[k];
         if (threadIdx.x == 0 && tile == T - 1) {
             const int32_t next = int32_t((excl >> 32) + (total & 0xFFFFu));
             b.ccount[iter + 1] = next;
@@ -536,21 +530,16 @@ __global__ __launch_bounds__(BLOCK) void KPlocMerge(BuildBuffers b, int iter, in
             b.results->multi_iters = iter + 1;
             b.results->flags = b.bounds[12];
         }
-        int32_t spos = int32_t(excl >> 32) + int32_t(before & 0xFFFFu);
-        int32_t mpos = node_base + int32_t(excl & 0xFFFFFFFFull) + int32_t(before >> 16);
-#pragma unroll
-        for (int q = 0; q < kIpt; q++) {
-            if (!(sflags & (1u << q))) continue;
-            const int i = a + threadIdx.x * kIpt + q;
+        if (survives) {
             PNode o = in[i];
-            if (mflags & (1u << q)) {
-                const PNode pj = in[word[q] & kPairIndex];
+            if (merges) {
+                const PNode pj = in[word & kPairIndex];
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
                     o.lo[k] = Min2(o.lo[k], pj.lo[k]);
                     o.hi[k] = Max2(o.hi[k], pj.hi[k]);
                 }
-                const int32_t id = mpos++;
+                const int32_t id = node_base + int32_t(excl & 0xFFFFFFFFull) + int32_t(before >> 16);
                 o.right = pj.left;
                 b.nodes[id] = o;
                 b.parent[o.left] = id;
@@ -560,7 +549,7 @@ __global__ __launch_bounds__(BLOCK) void KPlocMerge(BuildBuffers b, int iter, in
                 o.left = id;
             }
             o.right = 0;
-            out[spos++] = o;
+            out[int32_t(excl >> 32) + int32_t(before & 0xFFFFu)] = o;
         }
         __syncthreads();  // s_sum and s_scan are reused by the next tile
     }
@@ -571,16 +560,16 @@ hipError_t LaunchPlocIterationSplit(const BuildBuffers& b, int iter, int radius,
     if (grid < 1) grid = 1;
     switch (block) {
         case 128:
-            hipLaunchKernelGGL(KPlocNeighbours<128>, dim3(grid), dim3(128), 0, s, b, iter, radius, tail);
-            hipLaunchKernelGGL(KPlocMerge<128>, dim3(grid), dim3(128), 0, s, b, iter, radius, tail, max_leaf);
+            hipLaunchKernelGGL(KPlocNeighbours<128>, dim3(grid), dim3(128 * kIpt), 0, s, b, iter, radius, tail);
+            hipLaunchKernelGGL(KPlocMerge<128>, dim3(grid), dim3(128 * kIpt), 0, s, b, iter, radius, tail, max_leaf);
             break;
         case 512:
-            hipLaunchKernelGGL(KPlocNeighbours<512>, dim3(grid), dim3(512), 0, s, b, iter, radius, tail);
-            hipLaunchKernelGGL(KPlocMerge<512>, dim3(grid), dim3(512), 0, s, b, iter, radius, tail, max_leaf);
+            hipLaunchKernelGGL(KPlocNeighbours<512>, dim3(grid), dim3(512 * kIpt), 0, s, b, iter, radius, tail);
+            hipLaunchKernelGGL(KPlocMerge<512>, dim3(grid), dim3(512 * kIpt), 0, s, b, iter, radius, tail, max_leaf);
             break;
         default:
-            hipLaunchKernelGGL(KPlocNeighbours<256>, dim3(grid), dim3(256), 0, s, b, iter, radius, tail);
-            hipLaunchKernelGGL(KPlocMerge<256>, dim3(grid), dim3(256), 0, s, b, iter, radius, tail, max_leaf);
+            hipLaunchKernelGGL(KPlocNeighbours<256>, dim3(grid), dim3(256 * kIpt), 0, s, b, iter, radius, tail);
+            hipLaunchKernelGGL(KPlocMerge<256>, dim3(grid), dim3(256 * kIpt), 0, s, b, iter, radius, tail, max_leaf);
             break;
     }
     return hipGetLastError();
