@@ -660,7 +660,9 @@ __global__ __launch_bounds__(256) void KEmit(BuildBuffers b) {
         EmitNode(b, int32_t(id), K);
 }
 
-// SOBB trees: the child slabs of every final inner node, one thread per node.
+// SOBB trees: the child slabs of every final inner node, one thread per child (lanes 2k and
+// 2k + 1 fit the left and right child of node slot k, and lane 2k writes both direction
+// triples): each thread waits on one child's records, not on two in turn.
 // At least four waves per SIMD: the unrolled 26-DOP triple scan would otherwise be
 // scheduled for instruction-level parallelism at the cost of every register.
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void KFitSobb(BuildBuffers b) {
@@ -670,9 +672,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void K
     for (int t = threadIdx.x; t < b.ntriples; t += blockDim.x) s_tri[t] = b.triples[t];
     __syncthreads();
     const float pad = KdopPad(b.bounds);
-    const int64_t m = K > 0 ? int64_t(K) : 1;
-    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < m; i += int64_t(gridDim.x) * blockDim.x)
-        FitNode(b, i, K, s_tri, pad, s_w + threadIdx.x, 256);
+    if (K == 0) {  // the whole scene is one leaf: the root's SOBB in both halves of node 0
+        if (blockIdx.x == 0 && threadIdx.x == 0) FitNode(b, 0, 0, s_tri, pad, s_w + threadIdx.x, 256);
+        return;
+    }
+    for (int64_t t = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; t < 2 * int64_t(K);
+         t += int64_t(gridDim.x) * blockDim.x) {
+        const int64_t slot = t >> 1;
+        const int side = int(t & 1);
+        const int2 ch = b.kept_child[slot];
+        float g[6];
+        const uint32_t d = FitSobb(b, s_tri, side ? ch.y : ch.x, pad, g, s_w + threadIdx.x, 256);
+        float2* o = reinterpret_cast<float2*>(b.fnodes + slot) + 3 * side;
+        o[0] = make_float2(g[0], g[1]);
+        o[1] = make_float2(g[2], g[3]);
+        o[2] = make_float2(g[4], g[5]);
+        // both lanes of the pair are in the loop together (2K is even, pairs start at even t)
+        const uint32_t other = uint32_t(__shfl_xor(int(d), 1, kWave));
+        if (side == 0) b.fnodes[slot].pad0 = int32_t(d | (other << 12));
+    }
 }
 
 // SAH of the final tree in FP64 (Ct = Ci = 1, divided by SA(root) on the host); the node
@@ -713,7 +731,8 @@ hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s) {
 // SOBB trees: fit every kept child's slab triple (after LaunchLayout); nothing for AABB trees.
 hipError_t LaunchFitSobb(const BuildBuffers& b, hipStream_t s) {
     if (b.ndirs == 0) return hipSuccess;
-    const int fgrid = int(std::min<int64_t>(4096, (b.n + 255) / 256));
+    // at most 2 (n - 1) children: about one per thread
+    const int fgrid = int(std::min<int64_t>(16384, (2 * b.n + 255) / 256));
     hipLaunchKernelGGL(KFitSobb, dim3(fgrid > 0 ? fgrid : 1), dim3(256), 0, s, b);
     return hipGetLastError();
 }
