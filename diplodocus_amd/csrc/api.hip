@@ -156,9 +156,11 @@ using namespace dpl;
 // the array's name.
 namespace {
 constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
-// PLOC iterations with more tiles than this many rounds of the fused kernel's workgroups run
-// as two launches (neighbours, then merges) instead of one
-constexpr int kPlocSplitRounds = 1;
+// PLOC iterations with more 512-cluster tiles than this many rounds of the fused kernel's
+// workgroups run as two launches (neighbours, then merges) instead of one: for 256-thread
+// blocks, above ~98k clusters (2M build: 0.1 / 0.15 / 0.2 / 0.4 / 1 round measured 2.28 /
+// 2.27 / 2.26 / 2.27 / 2.32 ms)
+constexpr double kPlocSplitRounds = 0.2;
 constexpr size_t kNoAllocCounterBytes = kTraceCounterBytes;
 static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
 std::mutex g_scratch_mu;
@@ -473,7 +475,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
             // more tiles of two clusters per thread than kPlocSplitRounds rounds of the fused
             // kernel's workgroups: two launches that never wait on each other
             const int64_t t2 = int64_t(std::ceil(double(c_known) / std::pow(rate, g) / double(split_chunk)));
-            const bool split = t2 > int64_t(kPlocSplitRounds) * cap;
+            const bool split = double(t2) > kPlocSplitRounds * double(cap);
             if (time_iters) DPL_HIP(hipEventRecord(C.IterEvent(2 * size_t(iter)), s));
             if (split)
                 DPL_HIP(LaunchPlocIterationSplit(b, iter, p.nn_search_radius, p.block_size, tail,
