@@ -230,7 +230,9 @@ template <int BLOCK, int LANES>
 __global__ __launch_bounds__(BLOCK * LANES) void KPlocIteration(BuildBuffers b, int iter, int R, int tail,
                                                                 int max_leaf, int min_tiles) {
     constexpr int NT = BLOCK * LANES;
-    constexpr int CHUNK = BLOCK * kIpt;
+    // the four-lane form runs on iterations of about one tile per CU: one cluster per thread
+    constexpr int IPT = LANES > 1 ? 1 : kIpt;
+    constexpr int CHUNK = BLOCK * IPT;
     constexpr int WMAX = CHUNK + 4 * kMaxRadius;
     __shared__ float s_lx[WMAX], s_ly[WMAX], s_lz[WMAX], s_hx[WMAX], s_hy[WMAX], s_hz[WMAX];
     __shared__ int32_t s_id[WMAX];
@@ -256,7 +258,7 @@ __global__ __launch_bounds__(BLOCK * LANES) void KPlocIteration(BuildBuffers b, 
         return;
     }
     int ipt = 1;
-    while (ipt < kIpt && c >= int64_t(BLOCK) * 2 * ipt * min_tiles) ipt *= 2;
+    while (ipt < IPT && c >= int64_t(BLOCK) * 2 * ipt * min_tiles) ipt *= 2;
     // BLOCK * ipt nearest neighbours per tile (its clusters and R on either side): every
     // thread computes exactly ipt of them
     const int chunk = BLOCK * ipt - 2 * R;
@@ -309,11 +311,11 @@ __global__ __launch_bounds__(BLOCK * LANES) void KPlocIteration(BuildBuffers b, 
         }
         __syncthreads();
 
-        // merge decisions for the tile's clusters (blocked: thread t owns kIpt in a row)
+        // merge decisions for the tile's clusters (blocked: thread t owns IPT in a row)
         uint32_t sflags = 0, mflags = 0;
-        int32_t nnj[kIpt];
+        int32_t nnj[IPT];
 #pragma unroll
-        for (int q = 0; q < kIpt; q++) {
+        for (int q = 0; q < IPT; q++) {
             const int i = a + threadIdx.x * ipt + q;
             nnj[q] = -1;
             if (q < ipt && i < e) {
@@ -359,7 +361,7 @@ __global__ __launch_bounds__(BLOCK * LANES) void KPlocIteration(BuildBuffers b, 
         int32_t spos = int32_t((excl >> 31) & kMask31) + int32_t(before & 0xFFFFu);
         int32_t mpos = node_base + int32_t(excl & kMask31) + int32_t(before >> 16);
 #pragma unroll
-        for (int q = 0; q < kIpt; q++) {
+        for (int q = 0; q < IPT; q++) {
             const int i = a + threadIdx.x * ipt + q;
             if (!(sflags & (1u << q))) continue;
             const int ii = i - w0;
