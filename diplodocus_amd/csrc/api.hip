@@ -417,15 +417,21 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
         DPL_HIP(hipEventRecord(C.ev[9], s));
     }
 
+    // per-stage and per-launch events only on request (keep_intermediates bit 1): an event
+    // recorded between two kernels leaves the GPU idle for about 5 us
+    const bool time_iters = (p.keep_intermediates & 2) != 0;
+    auto stage_event = [&](int k) {
+        if (time_iters) DPL_HIP(hipEventRecord(C.ev[k], s));
+    };
     DPL_HIP(hipEventRecord(C.ev[0], s));
     DPL_HIP(LaunchBuildInit(b, p.block_size, s));
     DPL_HIP(LaunchBounds(b, s));
-    DPL_HIP(hipEventRecord(C.ev[1], s));
+    stage_event(1);
     DPL_HIP(LaunchCodes(b, p.morton_type, s));
     if (p.keep_intermediates) {
         DPL_HIP(hipMemcpyAsync(bvh->codes.p, b.keys, size_t(n) * (key64 ? 8 : 4), hipMemcpyDeviceToDevice, s));
     }
-    DPL_HIP(hipEventRecord(C.ev[2], s));
+    stage_event(2);
     bool in_alt = false;
     DPL_HIP(LaunchRadixSort(b.keys, b.vals, b.keys_alt, b.vals_alt, n, key_bits, p.sort_digit_bits, key64, sort_tmp,
                             sort_bytes, &in_alt, s));
@@ -434,7 +440,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
         std::swap(b.vals, b.vals_alt);
     }
     b.pairs = b.vals_alt;  // free once the sort is done
-    DPL_HIP(hipEventRecord(C.ev[3], s));
+    stage_event(3);
     DPL_HIP(LaunchInitClusters(b, s));
 
     // ---- PLOC iterations: multi-workgroup kernels while clusters exceed the tail size.
@@ -450,7 +456,6 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     int iter = 0;
     int64_t c_known = n;
     double rate = 1.15;  // clusters shrink by about this factor per iteration
-    const bool time_iters = (p.keep_intermediates & 2) != 0;
     // workgroups per launch, also the tile count below which an iteration takes one cluster per
     // thread (2 above it): 4 per CU for 256 threads (PLOC iterations 3 % slower at 3 or 8 per CU,
     // 12 % at 2, 13 % at 16)
@@ -494,12 +499,12 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
         c_known = c_now;
     }
     DPL_HIP(LaunchPlocTail(b, iter, p.nn_search_radius, p.max_triangles_per_leaf, s));
-    DPL_HIP(hipEventRecord(C.ev[4], s));
-    DPL_HIP(hipEventRecord(C.ev[5], s));
+    stage_event(4);
+    stage_event(5);
     DPL_HIP(LaunchLayout(b, p.max_triangles_per_leaf, s));
-    DPL_HIP(hipEventRecord(C.ev[6], s));
+    stage_event(6);
     DPL_HIP(LaunchFitSobb(b, s));
-    DPL_HIP(hipEventRecord(C.ev[10], s));
+    stage_event(10);
     DPL_HIP(LaunchSah(b.fnodes, b.counters + 5, reinterpret_cast<double*>(b.bounds_part),
                       b.ndirs > 0 ? b.dirs : nullptr, s));
     DPL_HIP(LaunchFinish(b, s));
@@ -545,14 +550,16 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
 
     st.build_ms = ElapsedMs(C.ev[0], C.ev[7]);
     st.upload_ms = mem == DPL_MEM_HOST ? ElapsedMs(C.ev[8], C.ev[9]) : 0.0;
-    st.stage_ms[0] = ElapsedMs(C.ev[0], C.ev[1]);
-    st.stage_ms[1] = ElapsedMs(C.ev[1], C.ev[2]);
-    st.stage_ms[2] = ElapsedMs(C.ev[2], C.ev[3]);
-    st.stage_ms[3] = ElapsedMs(C.ev[3], C.ev[4]);
-    st.stage_ms[4] = ElapsedMs(C.ev[4], C.ev[5]);
-    st.stage_ms[5] = ElapsedMs(C.ev[5], C.ev[6]);
-    st.stage_ms[6] = ElapsedMs(C.ev[6], C.ev[10]);
-    st.stage_ms[7] = ElapsedMs(C.ev[10], C.ev[7]);
+    if (time_iters) {
+        st.stage_ms[0] = ElapsedMs(C.ev[0], C.ev[1]);
+        st.stage_ms[1] = ElapsedMs(C.ev[1], C.ev[2]);
+        st.stage_ms[2] = ElapsedMs(C.ev[2], C.ev[3]);
+        st.stage_ms[3] = ElapsedMs(C.ev[3], C.ev[4]);
+        st.stage_ms[4] = ElapsedMs(C.ev[4], C.ev[5]);
+        st.stage_ms[5] = ElapsedMs(C.ev[5], C.ev[6]);
+        st.stage_ms[6] = ElapsedMs(C.ev[6], C.ev[10]);
+        st.stage_ms[7] = ElapsedMs(C.ev[10], C.ev[7]);
+    }
     const int32_t fcount = K > 0 ? K : 1;
     st.inner_node_count = fcount;
     st.leaf_node_count = leaves;

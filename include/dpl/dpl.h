@@ -68,7 +68,8 @@ typedef struct dpl_build_params {
     int32_t block_size;             /* PLOC tile threads: 0, 128, 256 or 512 */
     int32_t sort_digit_bits;        /* radix digit width: 0, 4 or 8 */
     int32_t keep_intermediates;     /* bit 0: keep codes/order/merge list for download;
-                                       bit 1: time every PLOC iteration launch (adds events) */
+                                       bit 1: time every stage and PLOC iteration (adds events,
+                                       each about 5 us of idle GPU time) */
 } dpl_build_params;
 
 /* Fills *p from an acceleration-structure type and the reference's config values.
@@ -82,7 +83,8 @@ dpl_status dpl_build_params_from_type(int32_t accel_type, int32_t nn_search_radi
 typedef struct dpl_build_stats {
     double build_ms;         /* device time of the whole build (hipEvents), upload excluded */
     double upload_ms;        /* host-to-device copy of the triangles (0 for device input) */
-    double stage_ms[8];      /* bounds, codes, sort, ploc, collapse, layout, sobb, stats */
+    double stage_ms[8];      /* bounds, codes, sort, ploc, collapse, layout, sobb, stats
+                                (keep_intermediates bit 1), else 0 */
     int32_t node_count;      /* final nodes: inner + leaves */
     int32_t inner_node_count;
     int32_t leaf_node_count;
@@ -90,7 +92,8 @@ typedef struct dpl_build_stats {
     int64_t memory_consumption; /* bytes of the final BVH (nodes + ordered triangles + ids) */
     double sah_cost;         /* (Ct*sum_inner SA + Ci*sum_leaf SA*|leaf|) / SA(root), Ct=Ci=1 */
     int32_t tree_height;     /* longest root-to-leaf path of the final tree */
-    int32_t ploc_launches;   /* multi-workgroup PLOC iteration launches that merged */
+    int32_t ploc_launches;   /* multi-workgroup PLOC iterations that merged (one fused launch, or a
+                                neighbour launch and a merge launch, each) */
     double ploc_launch_ms;   /* their summed device time (keep_intermediates bit 1), else 0 */
     double ploc_bytes;       /* their algorithmic bytes: 32 c_k + 32 c_k+1 + (91 + 24 x) m_k, with
                                 x = SOBB directions - 3 (10 for the 26-DOP set), 0 for AABB (bit 1) */
