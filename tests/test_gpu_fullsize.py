@@ -65,8 +65,8 @@ def build_and_compare(oracle, ctx, tris, mt, bv, radius, block=0, digit=0, keep_
 @pytest.mark.parametrize("mt,bv,radius", [(1, 1, 16), (0, 0, 8), (0, 0, 32), (1, 0, 16), (0, 1, 16)])
 def test_mixed2m_full_build_bitexact(oracle, ctx, mixed, mt, bv, radius):
     """The headline build (PLOC+EMC+SOBB r16) and the other variants at the full 2,000,002
-    triangles: every PLOC iteration kernel instance the headline build launches (256-thread
-    tiles of two and of one cluster per thread, the four-lane iterations, the tail) runs here."""
+    triangles: every PLOC iteration kernel instance the headline build launches (the two-launch
+    iterations above ~98k clusters, the fused four-lane iterations below, the tail) runs here."""
     build_and_compare(oracle, ctx, mixed.tris, mt, bv, radius)
 
 
@@ -75,6 +75,14 @@ def test_large_soup_all_blocks_bitexact(oracle, ctx, block, digit):
     """1.2M triangles: blocks of 128 and 512 threads take two clusters per thread from 1,048,576
     clusters on a 256-CU device (4096 and 1024 workgroups), 256 from 524,288."""
     build_and_compare(oracle, ctx, soup(1_200_000, block + digit), 0, 0, 16, block=block, digit=digit)
+
+
+@pytest.mark.parametrize("radius,mt,bv", [(1, 1, 1), (32, 1, 1), (2, 0, 0)])
+def test_two_launch_iterations_radius_extremes(oracle, ctx, radius, mt, bv):
+    """300k triangles: the iterations above ~98k clusters run as a neighbour launch and a merge
+    launch; at the smallest and largest radii the tiles' halos (2 r on each side) and the range
+    of searched clusters reach their bounds."""
+    build_and_compare(oracle, ctx, soup(300_000, 7 + radius), mt, bv, radius)
 
 
 def test_mixed2m_full_trace_matches_oracle(oracle, ctx, mixed):
