@@ -156,11 +156,6 @@ using namespace dpl;
 // the array's name.
 namespace {
 constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
-// PLOC iterations with more 512-cluster tiles than this many rounds of the fused kernel's
-// workgroups run as two launches (neighbours, then merges) instead of one: for 256-thread
-// blocks, above ~98k clusters (2M build: 0.1 / 0.15 / 0.2 / 0.4 / 1 round measured 2.28 /
-// 2.27 / 2.26 / 2.27 / 2.32 ms)
-constexpr double kPlocSplitRounds = 0.2;
 constexpr size_t kNoAllocCounterBytes = kTraceCounterBytes;
 static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
 std::mutex g_scratch_mu;
@@ -320,6 +315,12 @@ float ElapsedMs(hipEvent_t a, hipEvent_t b) {
 }
 
 
+// PLOC iterations with more two-launch tiles (2 x block clusters) than this many rounds of
+// the fused kernel's workgroups run as two launches (neighbours, then merges) instead of one:
+// for 256-thread blocks, above ~98k clusters (2M build: 0.1 / 0.15 / 0.2 / 0.4 / 1 round
+// measured 2.28 / 2.27 / 2.26 / 2.27 / 2.32 ms)
+constexpr double kPlocSplitRounds = 0.2;
+
 void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, dpl_mem mem,
                const dpl_build_params& p, dpl_build_stats* out_stats) {
     Require(tri_in != nullptr, "tri_xyz is null");
@@ -465,8 +466,8 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
         est = std::max(1, std::min(est, 96));
         const int64_t T = (c_known + p.block_size - 1) / p.block_size;  // tiles at one cluster per thread
         const int64_t per_tile = p.block_size - 2 * p.nn_search_radius;
-        // tiles of the two-launch iteration (two clusters per thread) at the known count: an
-        // upper bound for every iteration of this batch
+        // tiles of the two-launch iteration (2 x block clusters) at the known count: an upper
+        // bound for every iteration of this batch
         const int64_t split_chunk = PlocChunk(p.block_size) - 2 * p.nn_search_radius;
         const int64_t split_tiles = (c_known + split_chunk - 1) / split_chunk;
         for (int g = 0; g < est; g++) {
@@ -477,8 +478,8 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
             const int64_t t4 = int64_t(std::ceil(double(c_known) / std::pow(rate, g) / double(per_tile)));
             const bool four = p.block_size == 256 && t4 <= 2 * int64_t(C.cu_count);
             const int grid = int(four ? std::min<int64_t>(2 * t4, 2 * int64_t(C.cu_count)) : std::min<int64_t>(T, cap));
-            // more tiles of two clusters per thread than kPlocSplitRounds rounds of the fused
-            // kernel's workgroups: two launches that never wait on each other
+            // more two-launch tiles than kPlocSplitRounds rounds of the fused kernel's
+            // workgroups: two launches that never wait on each other
             const int64_t t2 = int64_t(std::ceil(double(c_known) / std::pow(rate, g) / double(split_chunk)));
             const bool split = double(t2) > kPlocSplitRounds * double(cap);
             if (time_iters) DPL_HIP(hipEventRecord(C.IterEvent(2 * size_t(iter)), s));

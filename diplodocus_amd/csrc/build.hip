@@ -400,12 +400,13 @@ __global__ __launch_bounds__(BLOCK * LANES) void KPlocIteration(BuildBuffers b, 
 }
 
 // ---------------------------------------------------------------------------------------
-// The same iteration in two launches, for cluster counts of several rounds of tiles per
-// workgroup. In the fused kernel the workgroups of a CU move through their tiles in step
-// (window, search, look-back, merge), so the search's arithmetic and the other phases' memory
-// latency never overlap. Here no workgroup waits on another: KPlocNeighbours finds every
-// cluster's nearest neighbour and merge decision, KPlocMerge sums each tile's offsets from the
-// per-tile counts and merges and compacts. Same tiles, same search, same merges and order.
+// The same iteration in two launches, for the large cluster counts (the host's threshold).
+// In the fused kernel the workgroups of a CU move through their tiles in step (window,
+// search, look-back, merge), so the search's arithmetic and the other phases' memory latency
+// never overlap, and the merges' dependent loads are spread over few waves. Here no
+// workgroup waits on another: KPlocNeighbours finds every cluster's nearest neighbour and
+// merge decision, KPlocMerge sums each tile's offsets from the per-tile counts and merges and
+// compacts, both with one thread per cluster. Same search, same merges, same order.
 //
 // Decision word per cluster i (in b.pairs): bits 0..29 the neighbour j, bit 30 the cluster's
 // slot survives (not the higher index of a mutual pair), bit 31 it creates the merge (the
