@@ -201,7 +201,7 @@ __global__ __launch_bounds__(256) void KInitClusters(BuildBuffers b) {
     const int64_t n = b.n;
     const float pad = b.ndirs > 0 ? KdopPad(b.bounds) : 0.0f;
     for (int64_t k = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; k < n; k += int64_t(gridDim.x) * blockDim.x)
-        InitLeaf<true>(b, k, pad);
+        InitLeaf<true, true>(b, k, pad);
 }
 
 hipError_t LaunchInitClusters(const BuildBuffers& b, hipStream_t s) {
@@ -242,13 +242,19 @@ __global__ __launch_bounds__(BLOCK * LANES) void KPlocIteration(BuildBuffers b, 
     __shared__ uint64_t s_prefix;
 
     const int c = b.ccount[iter];
-    const PNode* __restrict__ in = b.clusters[iter & 1];
+    // the first iteration reads the leaves from the node array: cluster k is leaf k
+    const bool leaves = iter == 0;
+    const PNode* __restrict__ in = leaves ? b.nodes : b.clusters[iter & 1];
     PNode* __restrict__ out = b.clusters[(iter + 1) & 1];
     const int64_t n = b.n;
 
     if (c <= tail) {  // done with multi-workgroup iterations: pass the clusters through
         if (blockIdx.x == 0) {
-            for (int k = threadIdx.x; k < c; k += NT) out[k] = in[k];
+            for (int k = threadIdx.x; k < c; k += NT) {
+                PNode o = in[k];
+                if (leaves) o.left = k, o.right = 0;
+                out[k] = o;
+            }
             if (threadIdx.x == 0) {
                 b.ccount[iter + 1] = c;
                 b.tile_counter[(iter + 1) & 1] = 0u;
@@ -292,7 +298,7 @@ __global__ __launch_bounds__(BLOCK * LANES) void KPlocIteration(BuildBuffers b, 
                 const float4* p = reinterpret_cast<const float4*>(in + j);
                 float4 q0 = p[0], q1 = p[1];
                 s_lx[t] = q0.x, s_ly[t] = q0.y, s_lz[t] = q0.z, s_hx[t] = q0.w;
-                s_hy[t] = q1.x, s_hz[t] = q1.y, s_id[t] = __float_as_int(q1.z);
+                s_hy[t] = q1.x, s_hz[t] = q1.y, s_id[t] = leaves ? j : __float_as_int(q1.z);
             }
         }
         __syncthreads();
@@ -380,7 +386,9 @@ __global__ __launch_bounds__(BLOCK * LANES) void KPlocIteration(BuildBuffers b, 
                 b.nodes[id] = o;
                 b.parent[o.left] = id;
                 b.parent[o.right] = id;
-                StoreEval(b, id, InnerEval(o.lo, o.hi, LoadEval(b, o.left), LoadEval(b, o.right), max_leaf));
+                const float li[3] = {s_lx[ii], s_ly[ii], s_lz[ii]}, hi_[3] = {s_hx[ii], s_hy[ii], s_hz[ii]};
+                const float lj[3] = {s_lx[jj], s_ly[jj], s_lz[jj]}, hj[3] = {s_hx[jj], s_hy[jj], s_hz[jj]};
+                StoreEval(b, id, InnerEval(o.lo, o.hi, EvalOf(b, o.left, li, hi_), EvalOf(b, o.right, lj, hj), max_leaf));
                 if (b.ndirs > 3) UnionKdop(b, id, o.left, o.right);
                 o.left = id;
                 o.right = 0;
@@ -426,7 +434,8 @@ __global__ __launch_bounds__(BLOCK * kIpt) void KPlocNeighbours(BuildBuffers b, 
 
     const int c = b.ccount[iter];
     if (c <= tail) return;  // KPlocMerge passes the clusters through
-    const PNode* __restrict__ in = b.clusters[iter & 1];
+    // the first iteration reads the leaves from the node array (only boxes are read here)
+    const PNode* __restrict__ in = iter == 0 ? b.nodes : b.clusters[iter & 1];
     const int chunk = CHUNK - 2 * R;
     const int T = (c + chunk - 1) / chunk;
     uint64_t* counts = b.status[iter & 1];
@@ -470,18 +479,24 @@ __global__ __launch_bounds__(BLOCK * kIpt) void KPlocNeighbours(BuildBuffers b, 
 // One thread per cluster of the tile (BLOCK * kIpt threads): the merges' dependent loads
 // (cluster records, collapse records, k-DOP slabs) are spread over twice the waves.
 template <int BLOCK>
-__global__ __launch_bounds__(BLOCK * kIpt) void KPlocMerge(BuildBuffers b, int iter, int R, int tail, int max_leaf) {
+__global__ __launch_bounds__(BLOCK * kIpt) __attribute__((amdgpu_waves_per_eu(6))) void KPlocMerge(BuildBuffers b, int iter, int R, int tail, int max_leaf) {
     constexpr int CHUNK = BLOCK * kIpt;
     constexpr int NT = CHUNK;
     __shared__ uint32_t s_scan[NT / kWave + 1];
     __shared__ uint64_t s_sum[NT / kWave];
 
     const int c = b.ccount[iter];
-    const PNode* __restrict__ in = b.clusters[iter & 1];
+    // the first iteration reads the leaves from the node array: cluster k is leaf k
+    const bool leaves = iter == 0;
+    const PNode* __restrict__ in = leaves ? b.nodes : b.clusters[iter & 1];
     PNode* __restrict__ out = b.clusters[(iter + 1) & 1];
     const int64_t n = b.n;
     if (c <= tail) {  // done with multi-workgroup iterations: pass the clusters through
-        for (int k = blockIdx.x * NT + threadIdx.x; k < c; k += gridDim.x * NT) out[k] = in[k];
+        for (int k = blockIdx.x * NT + threadIdx.x; k < c; k += gridDim.x * NT) {
+            PNode o = in[k];
+            if (leaves) o.left = k, o.right = 0;
+            out[k] = o;
+        }
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             b.ccount[iter + 1] = c;
             b.tile_counter[(iter + 1) & 1] = 0u;
@@ -535,8 +550,12 @@ __global__ __launch_bounds__(BLOCK * kIpt) void KPlocMerge(BuildBuffers b, int i
         }
         if (survives) {
             PNode o = in[i];
+            if (leaves) o.left = i;
             if (merges) {
-                const PNode pj = in[word & kPairIndex];
+                const int32_t j = int32_t(word & kPairIndex);
+                PNode pj = in[j];
+                if (leaves) pj.left = j;
+                const NodeEval el = EvalOf(b, o.left, o.lo, o.hi), er = EvalOf(b, pj.left, pj.lo, pj.hi);
 #pragma unroll
                 for (int k = 0; k < 3; k++) {
                     o.lo[k] = Min2(o.lo[k], pj.lo[k]);
@@ -547,7 +566,7 @@ __global__ __launch_bounds__(BLOCK * kIpt) void KPlocMerge(BuildBuffers b, int i
                 b.nodes[id] = o;
                 b.parent[o.left] = id;
                 b.parent[o.right] = id;
-                StoreEval(b, id, InnerEval(o.lo, o.hi, LoadEval(b, o.left), LoadEval(b, o.right), max_leaf));
+                StoreEval(b, id, InnerEval(o.lo, o.hi, el, er, max_leaf));
                 if (b.ndirs > 3) UnionKdop(b, id, o.left, o.right);
                 o.left = id;
             }
@@ -625,8 +644,9 @@ __global__ __launch_bounds__(kTailThreads) void KPlocTail(BuildBuffers b, int it
     __shared__ uint32_t s_scan[kTailThreads / kWave + 1];
     const TailLds L{s_lx, s_ly, s_lz, s_hx, s_hy, s_hz, s_id, s_nn, s_cnt, s_ml, s_ht, s_cost, s_scan};
     const int c = b.ccount[iter];
-    const int it = iter + TailIterations<kTailThreads, kTail>(b, b.clusters[iter & 1], c, R, max_leaf, L,
-                                                               b.ccount + iter, &b.bounds[12]);
+    // with no multi-workgroup iteration before it, the tail reads the leaves from the node array
+    const int it = iter + TailIterations<kTailThreads, kTail>(b, iter == 0 ? b.nodes : b.clusters[iter & 1], c, R,
+                                                               max_leaf, L, b.ccount + iter, &b.bounds[12], iter == 0);
     if (threadIdx.x == 0) {
         b.iter_done[0] = it;
         b.results->iter_done = it;
@@ -757,7 +777,7 @@ __global__ __launch_bounds__(256) void KFinish(BuildBuffers b) {
     const int64_t root = 2 * b.n - 2;
     r->kept = b.counters[5];
     r->leaves = b.counters[5] + 1;
-    r->root_height = b.eval[root].height;
+    r->root_height = root < b.n ? 0 : b.eval[root].height;  // a one-triangle scene: the leaf
     r->sah = b.sah[0];
     r->flags = b.bounds[12];
     const PNode nd = b.nodes[root];

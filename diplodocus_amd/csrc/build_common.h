@@ -192,6 +192,12 @@ __device__ __forceinline__ NodeEval LoadEval(const BuildBuffers& b, int32_t id) 
     return NodeEval{r.count_col & 0x7FFFFFFF, r.cost, r.minleaf, r.height, uint8_t(uint32_t(r.count_col) >> 31)};
 }
 
+// The collapse record of node id whose box is (lo, hi). Single-scene builds store no leaf
+// records: a leaf's (id < n) follows from its box and position, as LeafEval gives it.
+__device__ __forceinline__ NodeEval EvalOf(const BuildBuffers& b, int32_t id, const float lo[3], const float hi[3]) {
+    return id < b.n ? LeafEval(lo, hi, id) : LoadEval(b, id);
+}
+
 __device__ __forceinline__ void StoreEval(const BuildBuffers& b, int32_t id, const NodeEval& e) {
     b.eval[id] = EvalRec{int32_t(uint32_t(e.count) | (uint32_t(e.collapsed) << 31)), e.cost, e.minleaf, e.height};
 }
@@ -689,7 +695,9 @@ __device__ __forceinline__ uint64_t CodeOf(const float v[9], const CodeFrame& f,
 // ---------------------------------------------------------------------------------------
 // GATHER16: read the triangle with GatherTri (single-scene builds, where the gathers miss the
 // caches); batched meshes are small and cache-resident, where the nine plain loads are cheaper.
-template <bool GATHER16 = false>
+// SINGLE (single-scene builds): only the leaf node and its k-DOP are written; the first
+// iteration reads the leaves from the node array, and a leaf's collapse record is EvalOf's.
+template <bool GATHER16 = false, bool SINGLE = false>
 __device__ __forceinline__ void InitLeaf(const BuildBuffers& b, int64_t k, float pad) {
     const uint32_t orig = b.vals[k];
     float v[9];
@@ -702,8 +710,9 @@ __device__ __forceinline__ void InitLeaf(const BuildBuffers& b, int64_t k, float
     nd.left = int32_t(orig);
     nd.right = -1;
     b.nodes[k] = nd;
-    StoreEval(b, int32_t(k), LeafEval(nd.lo, nd.hi, int32_t(k)));
     if (b.ndirs > 3) LeafKdop(b, v, pad, b.kdop + k * KdopWidth(b.ndirs));
+    if (SINGLE) return;
+    StoreEval(b, int32_t(k), LeafEval(nd.lo, nd.hi, int32_t(k)));
     nd.left = int32_t(k);
     nd.right = 0;
     b.clusters[0][k] = nd;
@@ -718,7 +727,8 @@ __device__ __forceinline__ void InitLeaf(const BuildBuffers& b, int64_t k, float
 __device__ __forceinline__ void ClassifyNode(const BuildBuffers& b, int32_t id, int max_leaf, int& kept,
                                              int& leaves) {
     const int64_t n = b.n;
-    const EvalRec e = b.eval[id];
+    // a leaf's record: one triangle, collapsed, its own position as minleaf
+    const EvalRec e = id < n ? EvalRec{int32_t(1u | (1u << 31)), 0.0f, id, 0} : b.eval[id];
     // ancestors hold more triangles: from max_leaf up, none can be a collapsed one
     const bool in = (e.count_col & 0x7FFFFFFF) < max_leaf && Inside(b, id, max_leaf);
     const bool col = e.count_col < 0;
@@ -756,13 +766,13 @@ __device__ __forceinline__ void EmitTriangle(const BuildBuffers& b, int32_t k) {
         do {
             const int32_t p = b.parent[x];
             const int32_t left = b.nodes[p].left;
-            if (x != left) off += b.eval[left].count_col & 0x7FFFFFFF;
+            if (x != left) off += left < b.n ? 1 : (b.eval[left].count_col & 0x7FFFFFFF);
             x = p;
         } while (b.state[x] == 0);
         first = b.eval[x].minleaf;
     }
     const int32_t pos = b.leafslot[first] + off;
-    const int32_t orig = b.nodes[k].left;
+    const int32_t orig = int32_t(b.vals[k]);  // leaf k's triangle (the sorted ids: 4 bytes, not its 32-byte node)
     b.prim_order[pos] = orig;
     float p[9];
     if (GATHER16)
@@ -782,7 +792,7 @@ __device__ __forceinline__ void EmitNode(const BuildBuffers& b, int32_t id, int3
     const uint8_t st = b.state[id];
     if (st == 1) {
         if (id == nn - 1) {  // the whole scene is one leaf: one node, empty right slot
-            const EvalRec e = b.eval[id];
+            const EvalRec e = id < n ? EvalRec{int32_t(1u | (1u << 31)), 0.0f, id, 0} : b.eval[id];
             const PNode r = b.nodes[id];
             FNode f;
             f.lx0 = f.rx0 = r.lo[0];
@@ -874,15 +884,16 @@ struct TailLds {
 
 template <int THREADS, int CAP>
 __device__ int TailIterations(const BuildBuffers& b, const PNode* in, int c, int R, int max_leaf, const TailLds& L,
-                              int32_t* ccount_log, uint32_t* flags) {
+                              int32_t* ccount_log, uint32_t* flags, bool leaf_input = false) {
     constexpr int IPT = (CAP + THREADS - 1) / THREADS;
     const int64_t n = b.n;
     for (int k = threadIdx.x; k < c; k += THREADS) {
         const float4* p = reinterpret_cast<const float4*>(in + k);
         const float4 q0 = p[0], q1 = p[1];
         L.lx[k] = q0.x, L.ly[k] = q0.y, L.lz[k] = q0.z, L.hx[k] = q0.w;
-        L.hy[k] = q1.x, L.hz[k] = q1.y, L.id[k] = __float_as_int(q1.z);
-        const NodeEval e = LoadEval(b, L.id[k]);
+        L.hy[k] = q1.x, L.hz[k] = q1.y, L.id[k] = leaf_input ? k : __float_as_int(q1.z);
+        const float lo[3] = {q0.x, q0.y, q0.z}, hi[3] = {q0.w, q1.x, q1.y};
+        const NodeEval e = EvalOf(b, L.id[k], lo, hi);
         L.cnt[k] = e.count, L.cost[k] = e.cost, L.ml[k] = e.minleaf, L.ht[k] = e.height;
     }
     __syncthreads();
