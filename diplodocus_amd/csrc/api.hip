@@ -366,7 +366,7 @@ void BuildImpl(dpl_context* ctx, dpl_bvh* bvh, const float* tri_in, int64_t n, d
     b.results = reinterpret_cast<BuildResults*>(C.pinned_dev);
     b.leafslot = C.leafslot.ensure<int32_t>(size_t(n) * 4);
     b.keptflag = C.keptflag.ensure<int32_t>(size_t(n) * 4);
-    b.scan_tmp = C.scan_tmp.ensure<int32_t>(ScanTempInts(n) * 4);
+    b.scan_tmp = C.scan_tmp.ensure<int32_t>(2 * ScanTempInts(n) * 4);  // the layout scans two arrays at once
     const size_t sort_bytes = RadixSortTempBytes(n, key_bits, p.sort_digit_bits, key64);
     void* sort_tmp = C.sort_tmp.ensure(sort_bytes);
     b.fnodes = bvh->fnodes.ensure<FNode>(size_t(std::max<int64_t>(1, n - 1)) * sizeof(FNode));

@@ -741,12 +741,9 @@ hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s) {
     hipLaunchKernelGGL(KClassify, dim3(grid), dim3(256), 0, s, b, max_leaf);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = LaunchExclusiveScan(b.leafslot, b.leafslot, b.n, b.scan_tmp, b.counters + 4, s);
+    // leafslot (n) and keptflag (n - 1, nothing when n = 1; its total stays 0) in one scan set
+    e = LaunchExclusiveScan2(b.leafslot, b.n, b.keptflag, b.n - 1, b.scan_tmp, b.counters + 4, b.counters + 5, s);
     if (e != hipSuccess) return e;
-    if (b.n > 1) {
-        e = LaunchExclusiveScan(b.keptflag, b.keptflag, b.n - 1, b.scan_tmp, b.counters + 5, s);
-        if (e != hipSuccess) return e;
-    }
     hipLaunchKernelGGL(KEmit, dim3(grid), dim3(256), 0, s, b);
     return hipGetLastError();
 }
@@ -800,7 +797,7 @@ hipError_t LaunchSah(const FNode* nodes, const int32_t* counters, double* part, 
 }
 
 // ---------------------------------------------------------------------------------------
-// exclusive scan of int32 (in place allowed): block sums, one-workgroup scan, downsweep
+// exclusive scans of int32 in place: block sums, one-workgroup scan, downsweep
 // ---------------------------------------------------------------------------------------
 namespace {
 constexpr int kScanBlock = 256;
@@ -810,24 +807,27 @@ constexpr int kScanTile = kScanBlock * kScanIpt;
 
 size_t ScanTempInts(int64_t n) { return size_t((n + kScanTile - 1) / kScanTile) + 1; }
 
-__global__ __launch_bounds__(kScanBlock) void KScanReduce(const int32_t* in, int64_t n, int32_t* sums) {
-    const int64_t base = int64_t(blockIdx.x) * kScanTile;
+// Sum of tile `tile` of in[0, n), returned to thread 0.
+__device__ __forceinline__ int32_t ScanTileSum(const int32_t* in, int64_t n, int64_t tile, int32_t* s) {
+    const int64_t base = tile * kScanTile;
     int32_t acc = 0;
     for (int k = threadIdx.x; k < kScanTile; k += kScanBlock) {
         int64_t i = base + k;
         if (i < n) acc += in[i];
     }
-    __shared__ int32_t s[kScanBlock];
     s[threadIdx.x] = acc;
     __syncthreads();
     for (int d = kScanBlock / 2; d > 0; d >>= 1) {
         if (int(threadIdx.x) < d) s[threadIdx.x] += s[threadIdx.x + d];
         __syncthreads();
     }
-    if (threadIdx.x == 0) sums[blockIdx.x] = s[0];
+    const int32_t r = s[0];
+    __syncthreads();
+    return r;
 }
 
-__global__ __launch_bounds__(1024) void KScanSums(int32_t* sums, int64_t m, int32_t* total) {
+// One exclusive scan of the tile sums sums[0, m) in place, its total to *total.
+__device__ __forceinline__ void ScanSums(int32_t* sums, int64_t m, int32_t* total) {
     __shared__ uint32_t s_scan[1024 / kWave + 1];
     __shared__ int32_t s_carry;
     if (threadIdx.x == 0) s_carry = 0;
@@ -846,10 +846,10 @@ __global__ __launch_bounds__(1024) void KScanSums(int32_t* sums, int64_t m, int3
     if (threadIdx.x == 0) total[0] = s_carry;
 }
 
-__global__ __launch_bounds__(kScanBlock) void KScanDown(const int32_t* in, int32_t* out, int64_t n,
-                                                        const int32_t* sums) {
-    __shared__ uint32_t s_scan[kScanBlock / kWave + 1];
-    const int64_t base = int64_t(blockIdx.x) * kScanTile + int64_t(threadIdx.x) * kScanIpt;
+// Tile `tile` of the exclusive scan of in[0, n) into out, from the tile's base `sum`.
+__device__ __forceinline__ void ScanTileDown(const int32_t* in, int32_t* out, int64_t n, int64_t tile, int32_t sum,
+                                             uint32_t* s_scan) {
+    const int64_t base = tile * kScanTile + int64_t(threadIdx.x) * kScanIpt;
     int32_t v[kScanIpt];
     int32_t acc = 0;
 #pragma unroll
@@ -859,7 +859,7 @@ __global__ __launch_bounds__(kScanBlock) void KScanDown(const int32_t* in, int32
         acc += v[q];
     }
     uint32_t tot;
-    int32_t run = sums[blockIdx.x] + int32_t(BlockExclusiveScan<kScanBlock>(uint32_t(acc), s_scan, tot));
+    int32_t run = sum + int32_t(BlockExclusiveScan<kScanBlock>(uint32_t(acc), s_scan, tot));
 #pragma unroll
     for (int q = 0; q < kScanIpt; q++) {
         int64_t i = base + q;
@@ -868,12 +868,40 @@ __global__ __launch_bounds__(kScanBlock) void KScanDown(const int32_t* in, int32
     }
 }
 
-hipError_t LaunchExclusiveScan(const int32_t* in, int32_t* out, int64_t n, int32_t* tmp, int32_t* total,
-                               hipStream_t s) {
-    const int64_t blocks = (n + kScanTile - 1) / kScanTile;
-    hipLaunchKernelGGL(KScanReduce, dim3(unsigned(blocks)), dim3(kScanBlock), 0, s, in, n, tmp);
-    hipLaunchKernelGGL(KScanSums, dim3(1), dim3(1024), 0, s, tmp, blocks, total);
-    hipLaunchKernelGGL(KScanDown, dim3(unsigned(blocks)), dim3(kScanBlock), 0, s, in, out, n, tmp);
+// Two in-place exclusive scans (x[0, nx) and y[0, ny), ny <= nx) in the same three launches:
+// the tile sums of x in tmp[0, blocks), of y in tmp[blocks, 2 blocks); workgroup 0 of the
+// middle launch scans x's sums, workgroup 1 y's.
+__global__ __launch_bounds__(kScanBlock) void KScanReduce2(const int32_t* x, int64_t nx, const int32_t* y, int64_t ny,
+                                                           int32_t* sums, int64_t blocks) {
+    __shared__ int32_t s[kScanBlock];
+    const int32_t tx = ScanTileSum(x, nx, blockIdx.x, s);
+    const int32_t ty = ScanTileSum(y, ny, blockIdx.x, s);
+    if (threadIdx.x == 0) {
+        sums[blockIdx.x] = tx;
+        sums[blocks + blockIdx.x] = ty;
+    }
+}
+
+__global__ __launch_bounds__(1024) void KScanSums2(int32_t* sums, int64_t blocks, int32_t* total_x, int32_t* total_y) {
+    if (blockIdx.x == 0)
+        ScanSums(sums, blocks, total_x);
+    else
+        ScanSums(sums + blocks, blocks, total_y);
+}
+
+__global__ __launch_bounds__(kScanBlock) void KScanDown2(int32_t* x, int64_t nx, int32_t* y, int64_t ny,
+                                                         const int32_t* sums, int64_t blocks) {
+    __shared__ uint32_t s_scan[kScanBlock / kWave + 1];
+    ScanTileDown(x, x, nx, blockIdx.x, sums[blockIdx.x], s_scan);
+    ScanTileDown(y, y, ny, blockIdx.x, sums[blocks + blockIdx.x], s_scan);
+}
+
+hipError_t LaunchExclusiveScan2(int32_t* x, int64_t nx, int32_t* y, int64_t ny, int32_t* tmp, int32_t* total_x,
+                                int32_t* total_y, hipStream_t s) {
+    const int64_t blocks = std::max<int64_t>(1, (nx + kScanTile - 1) / kScanTile);
+    hipLaunchKernelGGL(KScanReduce2, dim3(unsigned(blocks)), dim3(kScanBlock), 0, s, x, nx, y, ny, tmp, blocks);
+    hipLaunchKernelGGL(KScanSums2, dim3(2), dim3(1024), 0, s, tmp, blocks, total_x, total_y);
+    hipLaunchKernelGGL(KScanDown2, dim3(unsigned(blocks)), dim3(kScanBlock), 0, s, x, nx, y, ny, tmp, blocks);
     return hipGetLastError();
 }
 

@@ -262,8 +262,10 @@ int PlocTail();
 hipError_t LaunchLayout(const BuildBuffers& b, int max_leaf, hipStream_t s);
 hipError_t LaunchFitSobb(const BuildBuffers& b, hipStream_t s);
 hipError_t LaunchFinish(const BuildBuffers& b, hipStream_t s);
-hipError_t LaunchExclusiveScan(const int32_t* in, int32_t* out, int64_t n, int32_t* tmp, int32_t* total,
-                               hipStream_t s);
+// Two in-place exclusive scans (ny <= nx) in one set of three launches; tmp holds
+// 2 ScanTempInts(nx) ints.
+hipError_t LaunchExclusiveScan2(int32_t* x, int64_t nx, int32_t* y, int64_t ny, int32_t* tmp, int32_t* total_x,
+                                int32_t* total_y, hipStream_t s);
 size_t ScanTempInts(int64_t n);
 
 hipError_t LaunchRadixSort(uint64_t* keys, uint32_t* vals, uint64_t* keys_alt, uint32_t* vals_alt, int64_t n,
