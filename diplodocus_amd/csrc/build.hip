@@ -423,6 +423,76 @@ __global__ __launch_bounds__(BLOCK * LANES) void KPlocIteration(BuildBuffers b, 
 // ---------------------------------------------------------------------------------------
 constexpr uint32_t kPairSurvives = 1u << 30, kPairMerges = 1u << 31, kPairIndex = kPairSurvives - 1;
 
+// Dynamic LDS of KPlocNeighbours: R rows of `threads` pair areas, or 0 (each cluster searches
+// its candidates on both sides itself). Measured on the 2M-triangle build: radius 16 with
+// 512 threads (32 KiB) 2.21 -> 2.16 ms, radius 8 1.97 -> 1.94 ms, radius 4 1.80 -> 1.78 ms;
+// radius 32 (64 KiB, fewer workgroups per CU) 2.80 -> 3.31 ms; radius 1 (the barrier and
+// stores cost more than the areas saved) 1.56 -> 1.58 ms.
+constexpr size_t kNnPairBytesMax = 32 * 1024;
+constexpr int kNnPairMinRadius = 4;
+__host__ __device__ constexpr size_t NnPairBytes(int R, int threads) {
+    const size_t bytes = size_t(R) * size_t(threads) * sizeof(float);
+    return R >= kNnPairMinRadius && bytes <= kNnPairBytesMax ? bytes : 0;
+}
+
+// Nearest neighbours of the searched clusters i = n0 + t, t < rows, with each pair's area
+// computed once (the area of a union does not depend on the order of the two boxes). Cluster i
+// computes SA(box_i U box_{i+k}) for k = 1..R, keeps the least in its right chain and leaves
+// it in dist[(k - 1) NT + t]; after the barrier it takes its candidates i - k >= n0 from
+// cluster t - k's row (the candidates before n0, which no searched cluster covers, it computes
+// itself). Both chains visit their candidates in ascending j, as NnOffer's tie rule needs, and
+// the two chain minima are compared with the full key: the same neighbour as NearestNeighbour.
+template <int NT>
+__device__ __forceinline__ void NearestNeighboursPaired(const WindowLds& w, float* __restrict__ dist, int n0, int rows,
+                                                        int w0, int c, int R, int32_t* nn_out) {
+    constexpr float kInf = __builtin_huge_valf();
+    const int t = threadIdx.x;
+    const int i = n0 + t;
+    float bdl = kInf, bdr = kInf;
+    int32_t bjl = -1, bjr = -1;
+    if (t < rows) {
+        const int ii = i - w0;
+        const float alx = w.lx[ii], aly = w.ly[ii], alz = w.lz[ii], ahx = w.hx[ii], ahy = w.hy[ii], ahz = w.hz[ii];
+        for (int j = i - R < 0 ? 0 : i - R; j < n0; ++j)
+            NnOffer(i, j, NnArea(alx, aly, alz, ahx, ahy, ahz, w, j - w0), true, bdl, bjl);
+        const int jr = i + R < c - 1 ? i + R : c - 1;
+        for (int k = 1; k <= R; k += 4) {
+            const int j0 = i + k, j1 = j0 + 1, j2 = j0 + 2, j3 = j0 + 3;
+            // past the last cluster: re-read cluster jr (in the window) and discard the area
+            const float d0 = NnArea(alx, aly, alz, ahx, ahy, ahz, w, (j0 <= jr ? j0 : jr) - w0);
+            const float d1 = NnArea(alx, aly, alz, ahx, ahy, ahz, w, (j1 <= jr ? j1 : jr) - w0);
+            const float d2 = NnArea(alx, aly, alz, ahx, ahy, ahz, w, (j2 <= jr ? j2 : jr) - w0);
+            const float d3 = NnArea(alx, aly, alz, ahx, ahy, ahz, w, (j3 <= jr ? j3 : jr) - w0);
+            float* row = dist + (k - 1) * NT + t;
+            row[0] = d0;
+            if (k + 1 <= R) row[NT] = d1;
+            if (k + 2 <= R) row[2 * NT] = d2;
+            if (k + 3 <= R) row[3 * NT] = d3;
+            NnOffer(i, j0, d0, j0 <= jr, bdr, bjr);
+            NnOffer(i, j1, d1, j1 <= jr, bdr, bjr);
+            NnOffer(i, j2, d2, j2 <= jr, bdr, bjr);
+            NnOffer(i, j3, d3, j3 <= jr, bdr, bjr);
+        }
+    }
+    __syncthreads();
+    if (t < rows) {
+        // candidates i - k for k = min(R, t) down to 1; row t - k holds SA(box_{i-k} U box_i) at k
+        const int kmax = R < t ? R : t;
+        int k = kmax;
+        for (; k >= 4; k -= 4) {
+            const float d0 = dist[(k - 1) * NT + t - k], d1 = dist[(k - 2) * NT + t - k + 1];
+            const float d2 = dist[(k - 3) * NT + t - k + 2], d3 = dist[(k - 4) * NT + t - k + 3];
+            NnOffer(i, i - k, d0, true, bdl, bjl);
+            NnOffer(i, i - k + 1, d1, true, bdl, bjl);
+            NnOffer(i, i - k + 2, d2, true, bdl, bjl);
+            NnOffer(i, i - k + 3, d3, true, bdl, bjl);
+        }
+        for (; k >= 1; --k) NnOffer(i, i - k, dist[(k - 1) * NT + t - k], true, bdl, bjl);
+        if (bjl >= 0 && (bjr < 0 || NnBefore(i, bdl, bjl, bdr, bjr))) bjr = bjl;
+        *nn_out = bjr;
+    }
+}
+
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK * kIpt) void KPlocNeighbours(BuildBuffers b, int iter, int R, int tail) {
     constexpr int CHUNK = BLOCK * kIpt;
@@ -440,6 +510,8 @@ __global__ __launch_bounds__(BLOCK * kIpt) void KPlocNeighbours(BuildBuffers b, 
     const int T = (c + chunk - 1) / chunk;
     uint64_t* counts = b.status[iter & 1];
     const WindowLds w{s_lx, s_ly, s_lz, s_hx, s_hy, s_hz};
+    HIP_DYNAMIC_SHARED(float, s_dist)
+    const bool paired = NnPairBytes(R, NT) != 0;  // the launch sized s_dist by the same rule
     for (int tile = blockIdx.x; tile < T; tile += gridDim.x) {
         const int a = tile * chunk;
         const int e = a + chunk < c ? a + chunk : c;
@@ -457,7 +529,11 @@ __global__ __launch_bounds__(BLOCK * kIpt) void KPlocNeighbours(BuildBuffers b, 
         __syncthreads();
         const int n0 = a - R < 0 ? 0 : a - R;
         const int n1 = e + R > c ? c : e + R;
-        for (int i = n0 + threadIdx.x; i < n1; i += NT) s_nn[i - (a - R)] = NearestNeighbour(w, i, w0, c, R);
+        if (paired) {  // n1 - n0 <= NT: one searched cluster per thread
+            NearestNeighboursPaired<NT>(w, s_dist, n0, n1 - n0, w0, c, R, s_nn + (n0 + int(threadIdx.x) - (a - R)));
+        } else {
+            for (int i = n0 + threadIdx.x; i < n1; i += NT) s_nn[i - (a - R)] = NearestNeighbour(w, i, w0, c, R);
+        }
         __syncthreads();
         uint32_t ns = 0, nm = 0;
         const int i = a + int(threadIdx.x);
@@ -582,15 +658,18 @@ hipError_t LaunchPlocIterationSplit(const BuildBuffers& b, int iter, int radius,
     if (grid < 1) grid = 1;
     switch (block) {
         case 128:
-            hipLaunchKernelGGL(KPlocNeighbours<128>, dim3(grid), dim3(128 * kIpt), 0, s, b, iter, radius, tail);
+            hipLaunchKernelGGL(KPlocNeighbours<128>, dim3(grid), dim3(128 * kIpt), NnPairBytes(radius, 128 * kIpt), s, b, iter,
+                               radius, tail);
             hipLaunchKernelGGL(KPlocMerge<128>, dim3(grid), dim3(128 * kIpt), 0, s, b, iter, radius, tail, max_leaf);
             break;
         case 512:
-            hipLaunchKernelGGL(KPlocNeighbours<512>, dim3(grid), dim3(512 * kIpt), 0, s, b, iter, radius, tail);
+            hipLaunchKernelGGL(KPlocNeighbours<512>, dim3(grid), dim3(512 * kIpt), NnPairBytes(radius, 512 * kIpt), s, b, iter,
+                               radius, tail);
             hipLaunchKernelGGL(KPlocMerge<512>, dim3(grid), dim3(512 * kIpt), 0, s, b, iter, radius, tail, max_leaf);
             break;
         default:
-            hipLaunchKernelGGL(KPlocNeighbours<256>, dim3(grid), dim3(256 * kIpt), 0, s, b, iter, radius, tail);
+            hipLaunchKernelGGL(KPlocNeighbours<256>, dim3(grid), dim3(256 * kIpt), NnPairBytes(radius, 256 * kIpt), s, b, iter,
+                               radius, tail);
             hipLaunchKernelGGL(KPlocMerge<256>, dim3(grid), dim3(256 * kIpt), 0, s, b, iter, radius, tail, max_leaf);
             break;
     }

@@ -77,11 +77,14 @@ def test_large_soup_all_blocks_bitexact(oracle, ctx, block, digit):
     build_and_compare(oracle, ctx, soup(1_200_000, block + digit), 0, 0, 16, block=block, digit=digit)
 
 
-@pytest.mark.parametrize("radius,mt,bv", [(1, 1, 1), (32, 1, 1), (2, 0, 0)])
+@pytest.mark.parametrize("radius,mt,bv", [(1, 1, 1), (32, 1, 1), (2, 0, 0), (4, 0, 1), (5, 1, 0)])
 def test_two_launch_iterations_radius_extremes(oracle, ctx, radius, mt, bv):
     """300k triangles: the iterations above ~98k clusters run as a neighbour launch and a merge
     launch; at the smallest and largest radii the tiles' halos (2 r on each side) and the range
-    of searched clusters reach their bounds."""
+    of searched clusters reach their bounds. From radius 4 up (and at most 32 KiB of areas)
+    the neighbour launch computes each pair's area once and shares it between the pair's two
+    clusters: radius 4 is the smallest such radius, radius 5 leaves a partial group of four
+    candidates at the end of each row."""
     build_and_compare(oracle, ctx, soup(300_000, 7 + radius), mt, bv, radius)
 
 
